@@ -1,0 +1,150 @@
+/*
+ * saxs_engine.h -- C ABI of the MI355X SAXS scattering engine.
+ *
+ * Plain pointers and sizes only.  All device pointers are HIP device memory
+ * owned by the caller; the plan owns its own workspace.  Every call is
+ * asynchronous on the given stream (hipStream_t passed as void*; NULL is the
+ * default stream).  Functions return 0 on success or a negative code:
+ *   SAXS_EINVAL  (-1) invalid argument   -> Python ValueError
+ *   SAXS_ENOMEM  (-2) allocation failed  -> Python RuntimeError
+ *   SAXS_EHIP    (-3) HIP runtime error  -> Python RuntimeError
+ *   SAXS_EUNSUP  (-4) unsupported shape  -> Python ValueError
+ * saxs_last_error() returns the calling thread's last message.
+ *
+ * One plan is used by one stream at a time; calls on one plan must not
+ * overlap.  No global mutable state is shared between plans.
+ *
+ * Reference interfaces this ABI replaces (octupole/pyCuSaxs, cpp-src/):
+ *   saxsKernel class and its per-run setup      Saxs/saxsKernel.h:30-114,
+ *     scaledCell / createMemory / setcufftPlan    Saxs/saxsKernel.cu:240-366,470-488
+ *   per-frame pipeline runPKernel               Saxs/saxsKernel.cu:59-209
+ *   histogram read-out getSaxs                  Saxs/saxsKernel.cu:210-225
+ *   NVT spectrum accumulation getHistogram       Saxs/saxsKernel.cu:17-39
+ * The Python-visible entry point pycusaxs_cuda.run (pybind/cuda_bindings.cpp:
+ * 172-195) is kept by the Python host layer on top of this ABI.
+ */
+#ifndef SAXS_ENGINE_H
+#define SAXS_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAXS_OK 0
+#define SAXS_EINVAL (-1)
+#define SAXS_ENOMEM (-2)
+#define SAXS_EHIP (-3)
+#define SAXS_EUNSUP (-4)
+
+typedef struct saxs_plan saxs_plan;
+
+/* Geometry and host tables of one system (one cell grid, one FFT grid).
+ * Tables are copied into plan-owned device memory by saxs_plan_create.
+ *
+ *  n[3], nn[3]  cell grid and FFT grid (x, y, z; z is the fastest axis).
+ *               nn comes from the 2,3,5,7-smooth search of
+ *               saxsKernel.cu:449-469 or the caller's explicit scaled grid.
+ *  order        B-spline order, 4 or 6 (Splines.cu:63-74 defines only 4).
+ *  num_elements element types E; atoms carry ids 0..E-1.
+ *  num_bins     histogram bins (int(qcut/dq)+1, saxsKernel.cu:247).
+ *  border       z-slab half width d of the bulk-solvent border
+ *               (saxsKernel.h:107-110).
+ *  lut_len      entries of the shell tables, indexed by n = |m|^2; shells
+ *               n >= lut_len are outside q <= qcut.
+ *  bin_lut      [lut_len] bin of shell n, or -1 (outside qcut/bins).
+ *  ff_lut       [num_elements * lut_len] form factor f_e(q(n)) (opsfact.cu:29-35).
+ *  mod_x/y/z    [nn[a]] B-spline modulus tables lambda^2/|b|^2 (BSpmod.cpp:45-129).
+ *  max_atoms, max_frames  capacity of the plan's workspace.
+ *  tile_x, tile_y spreading column tile (0 = engine default).
+ */
+typedef struct saxs_desc {
+  int32_t n[3];
+  int32_t nn[3];
+  int32_t order;
+  int32_t num_elements;
+  int32_t num_bins;
+  int32_t border;
+  int32_t lut_len;
+  const int32_t* bin_lut;
+  const float* ff_lut;
+  const float* mod_x;
+  const float* mod_y;
+  const float* mod_z;
+  int32_t max_atoms;
+  int32_t max_frames;
+  int32_t tile_x;
+  int32_t tile_y;
+} saxs_desc;
+
+/* Creates a plan: copies tables, builds FFT radix plans and twiddles,
+ * allocates the workspace for max_frames x max_atoms.  Replaces
+ * saxsKernel::scaledCell/createMemory/setcufftPlan (saxsKernel.cu:240-366,
+ * 470-488; saxsKernel.h:55-58). */
+int saxs_plan_create(const saxs_desc* desc, saxs_plan** out);
+
+/* Replaces the shell tables (bin_lut, ff_lut) after a box change; lut_len
+ * may differ from the plan's (kept-frequency ranges are recomputed; it must
+ * not exceed what the workspace was sized for).  Synchronous. */
+int saxs_plan_set_tables(saxs_plan* plan, int32_t lut_len, const int32_t* bin_lut,
+                         const float* ff_lut);
+
+/* Forward scattering of n_frames frames of one system, accumulated into the
+ * caller's histogram: H[b] += sum over frames of the S6 weighted intensity,
+ * C[b] += matching point counts (NPT mode, saxsDeviceKernels.cu:67-125).
+ *   xyz     device float32 [n_frames][n_atoms][3], Angstrom
+ *   elem    device int32 [n_atoms], element id per atom
+ *   oc_diag host float32 [3], diagonal of the inverse box (1/L_a)
+ *   H, C    device float64 [num_bins]
+ * Replaces saxsKernel::runPKernel (saxsKernel.cu:59-209) for whole batches. */
+int saxs_forward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                 int32_t n_frames, const float* oc_diag, double* H, double* C,
+                 void* stream);
+
+/* Per-frame histograms instead of an accumulated one: Hf, Cf are device
+ * float64 [n_frames][num_bins] and are overwritten.  Used by the adjoint
+ * (per-frame binned intensities). */
+int saxs_forward_frames(saxs_plan* plan, const float* xyz, const int32_t* elem,
+                        int32_t n_atoms, int32_t n_frames, const float* oc_diag,
+                        double* Hf, double* Cf, void* stream);
+
+/* Adjoint of saxs_forward_frames: given gH = dL/dH per frame
+ * (device float64 [n_frames][num_bins]), writes dL/dx per atom into grad
+ * (device float32 [n_frames][n_atoms][3]; overwritten).  Recomputes the
+ * forward spectra internally (no stored state between calls).  No reference
+ * counterpart: pyCuSaxs has no gradient. */
+int saxs_backward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                  int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
+                  void* stream);
+
+/* Reference-structured per-frame path: per element, spread into the n^3
+ * grid with global atomics, border mean, explicit embed into nn^3, full
+ * 3-D R2C FFT, form-factor accumulate, modulus, histogram -- the structure
+ * of saxsKernel::runPKernel (saxsKernel.cu:59-209) on this engine's
+ * kernels.  One frame per call.  Kept as the per-frame baseline. */
+int saxs_forward_plain(saxs_plan* plan, const float* xyz, const int32_t* elem,
+                       int32_t n_atoms, const float* oc_diag, double* H, double* C,
+                       void* stream);
+
+/* Workspace bytes held by the plan (device memory). */
+int64_t saxs_plan_workspace_bytes(const saxs_plan* plan);
+
+/* Kept frequency counts (x, y, z) of the pruned transform, for measurement. */
+int saxs_plan_kept(const saxs_plan* plan, int32_t* kx, int32_t* ky, int32_t* kz);
+
+/* Frees the plan and its device memory (saxsKernel::~saxsKernel,
+ * saxsKernel.cu:546-553). */
+int saxs_plan_destroy(saxs_plan* plan);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* saxs_last_error(void);
+
+/* ABI version, for the loader's compatibility check. */
+int32_t saxs_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAXS_ENGINE_H */

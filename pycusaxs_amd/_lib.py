@@ -1,0 +1,92 @@
+"""ctypes binding of the C ABI declared in include/saxs_engine.h.
+
+The library is loaded from the package tree (``pycusaxs_amd/_lib``).  There
+is no fallback: if it is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from ._build import LIB_PATH
+
+SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP = 0, -1, -2, -3, -4
+ABI_VERSION = 1
+
+# every symbol include/saxs_engine.h declares
+EXPORTED = (
+    "saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
+    "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
+    "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version",
+)
+
+
+class SaxsDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32 * 3),
+        ("nn", C.c_int32 * 3),
+        ("order", C.c_int32),
+        ("num_elements", C.c_int32),
+        ("num_bins", C.c_int32),
+        ("border", C.c_int32),
+        ("lut_len", C.c_int32),
+        ("bin_lut", C.POINTER(C.c_int32)),
+        ("ff_lut", C.POINTER(C.c_float)),
+        ("mod_x", C.POINTER(C.c_float)),
+        ("mod_y", C.POINTER(C.c_float)),
+        ("mod_z", C.POINTER(C.c_float)),
+        ("max_atoms", C.c_int32),
+        ("max_frames", C.c_int32),
+        ("tile_x", C.c_int32),
+        ("tile_y", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path: Path = LIB_PATH):
+    """Loads the engine library (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not Path(path).exists():
+        raise ImportError(
+            f"MI355X SAXS engine library not built: {path} is missing "
+            "(run __graft_entry__.build() or python -m pycusaxs_amd._build)")
+    lib = C.CDLL(str(path))
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    lib.saxs_plan_create.argtypes = [C.POINTER(SaxsDesc), C.POINTER(vp)]
+    lib.saxs_plan_set_tables.argtypes = [vp, i32, vp, vp]
+    lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_forward_plain.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
+    lib.saxs_plan_workspace_bytes.argtypes = [vp]
+    lib.saxs_plan_workspace_bytes.restype = i64
+    lib.saxs_plan_kept.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    lib.saxs_plan_destroy.argtypes = [vp]
+    lib.saxs_last_error.restype = C.c_char_p
+    lib.saxs_abi_version.restype = i32
+    for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
+                 "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy"):
+        getattr(lib, name).restype = C.c_int
+    if lib.saxs_abi_version() != ABI_VERSION:
+        raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    """Maps return codes to Python exceptions like cuda_bindings.cpp:151-167:
+    invalid arguments and unsupported shapes -> ValueError, else RuntimeError."""
+    if rc == SAXS_OK:
+        return
+    msg = load().saxs_last_error().decode(errors="replace")
+    if rc in (SAXS_EINVAL, SAXS_EUNSUP):
+        raise ValueError(f"{what}: {msg}")
+    raise EngineError(f"{what}: {msg}")

@@ -1,0 +1,198 @@
+// Batched mixed-radix complex FFT of lines held in LDS (CDNA4 / gfx950).
+//
+// Self-sorting Stockham formulation: stage s with radix R combines R
+// sub-transforms of size Ns into size Ns*R.  Butterfly j (0 <= j < N/R)
+// reads x[j + r*N/R], applies twiddles W_{Ns R}^{(j mod Ns) r}, does a
+// radix-R DFT and writes y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  Output is in
+// natural order after the last stage.  Each stage runs in place: every
+// thread first pulls all of its butterfly inputs into registers, the block
+// synchronises, then results are written back.  A thread therefore holds at
+// most MAXC complex values, so a call may cover nlines*N <= MAXC*blockDim.
+//
+// Twiddles are a table tw[t] = exp(-2 pi i t / N) (forward sign, the sign of
+// the reference's cufftExecR2C), computed in double on the host and staged
+// in LDS by the caller.  Inverse transforms conjugate input and output.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace saxs {
+
+constexpr int kMaxStages = 12;
+
+struct FftDesc {
+  int N;
+  int nst;
+  int radix[kMaxStages];
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (-i)
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+template <int R>
+struct Radix;
+
+template <>
+struct Radix<2> {
+  static __device__ __forceinline__ void dft(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <>
+struct Radix<4> {
+  static __device__ __forceinline__ void dft(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = cmul_mi(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <>
+struct Radix<8> {
+  static __device__ __forceinline__ void dft(float2* v) {
+    constexpr float c = 0.70710678118654752f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Radix<4>::dft(e);
+    Radix<4>::dft(o);
+    // o_k *= exp(-i pi k / 4)
+    float2 o1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+    float2 o2 = cmul_mi(o[2]);
+    float2 o3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// Direct O(R^2) DFT for the odd radices; C[t] = cos(2 pi t/R), S[t] = -sin(2 pi t/R).
+template <int R>
+struct OddTables;
+template <>
+struct OddTables<3> {
+  static constexpr float C[3] = {1.0f, -0.5f, -0.5f};
+  static constexpr float S[3] = {0.0f, -0.86602540378443865f, 0.86602540378443865f};
+};
+template <>
+struct OddTables<5> {
+  static constexpr float C[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f,
+                                 -0.80901699437494742f, 0.30901699437494742f};
+  static constexpr float S[5] = {0.0f, -0.95105651629515357f, -0.58778525229247313f,
+                                 0.58778525229247313f, 0.95105651629515357f};
+};
+template <>
+struct OddTables<7> {
+  static constexpr float C[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f,
+                                 -0.90096886790241913f, -0.90096886790241913f,
+                                 -0.22252093395631440f, 0.62348980185873353f};
+  static constexpr float S[7] = {0.0f, -0.78183148246802981f, -0.97492791218182361f,
+                                 -0.43388373911755812f, 0.43388373911755812f,
+                                 0.97492791218182361f, 0.78183148246802981f};
+};
+
+template <int R>
+struct RadixOdd {
+  static __device__ __forceinline__ void dft(float2* v) {
+    float2 out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        const int t = (j * k) % R;
+        const float c = OddTables<R>::C[t], s = OddTables<R>::S[t];
+        acc.x += v[j].x * c - v[j].y * s;
+        acc.y += v[j].x * s + v[j].y * c;
+      }
+      out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+};
+template <>
+struct Radix<3> : RadixOdd<3> {};
+template <>
+struct Radix<5> : RadixOdd<5> {};
+template <>
+struct Radix<7> : RadixOdd<7> {};
+
+// One Stockham stage over nlines lines (line stride ls) of length N.
+template <int R, int MAXC>
+__device__ __forceinline__ void fft_stage(float2* buf, int ls, int nlines, int N, int Ns,
+                                          const float2* __restrict__ tw) {
+  constexpr int IT = MAXC / R;
+  const unsigned nb = (unsigned)(N / R);
+  const unsigned total = (unsigned)nlines * nb;
+  const unsigned nsu = (unsigned)Ns;
+  float2 v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const unsigned item = threadIdx.x + (unsigned)it * blockDim.x;
+    if (item < total) {
+      const unsigned line = item / nb, j = item - line * nb;
+      const float2* src = buf + line * ls;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = src[j + r * nb];
+    }
+  }
+  __syncthreads();
+  const unsigned tstride = (unsigned)N / (nsu * R);
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const unsigned item = threadIdx.x + (unsigned)it * blockDim.x;
+    if (item < total) {
+      const unsigned line = item / nb, j = item - line * nb;
+      const unsigned q = j / nsu, k = j - q * nsu;
+      if (k != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], tw[k * r * tstride]);
+      }
+      Radix<R>::dft(v[it]);
+      float2* dst = buf + line * ls + q * nsu * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * nsu] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Forward FFT of nlines lines of length d.N in LDS.  The caller must have
+// synchronised after filling buf; the routine ends synchronised.
+template <int MAXC>
+__device__ void fft_lines(float2* buf, int ls, int nlines, const FftDesc& d,
+                          const float2* __restrict__ tw) {
+  int Ns = 1;
+  for (int s = 0; s < d.nst; ++s) {
+    const int R = d.radix[s];
+    switch (R) {
+      case 8: fft_stage<8, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      case 4: fft_stage<4, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      case 2: fft_stage<2, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      case 3: fft_stage<3, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      case 5: fft_stage<5, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      case 7: fft_stage<7, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+}
+
+}  // namespace saxs

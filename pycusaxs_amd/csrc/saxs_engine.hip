@@ -1,0 +1,531 @@
+// C ABI of the MI355X SAXS engine: plan creation, workspace, launches.
+// The declarations and the reference interfaces each call replaces are in
+// include/saxs_engine.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "saxs_engine.h"
+#include "saxs_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char msg[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(msg, sizeof(msg), fmt, ap);
+  va_end(ap);
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(SAXS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));              \
+  } while (0)
+
+constexpr size_t kLdsLimit = 64 * 1024;
+
+bool factor_radices(int N, saxs::FftDesc& d) {
+  d.N = N;
+  d.nst = 0;
+  int m = N, twos = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++twos;
+  }
+  std::vector<int> rs;
+  int eights = twos / 3, rem = twos % 3;
+  if (rem == 1 && eights > 0) {
+    --eights;
+    rs.push_back(4);
+    rs.push_back(4);
+  } else if (rem == 1) {
+    rs.push_back(2);
+  } else if (rem == 2) {
+    rs.push_back(4);
+  }
+  for (int i = 0; i < eights; ++i) rs.insert(rs.begin(), 8);
+  for (int r : {3, 5, 7})
+    while (m % r == 0) {
+      m /= r;
+      rs.push_back(r);
+    }
+  if (m != 1 || (int)rs.size() > saxs::kMaxStages) return false;
+  if (rs.empty()) rs.push_back(1);  // N == 1 is rejected by the caller
+  for (size_t i = 0; i < rs.size(); ++i) d.radix[i] = rs[i];
+  d.nst = (int)rs.size();
+  return true;
+}
+
+// Largest number of lines of length N one fft_lines call may hold.
+int fft_line_cap(const saxs::FftDesc& d) {
+  int per_thread = saxs::kMaxC;
+  for (int s = 0; s < d.nst; ++s) per_thread = std::min(per_thread, (saxs::kMaxC / d.radix[s]) * d.radix[s]);
+  return per_thread * saxs::kWG / d.N;
+}
+
+int lds_stride(int N) { return (N % 16 == 0) ? N + 1 : N; }
+
+std::vector<float2> twiddles(int N) {
+  std::vector<float2> t(N);
+  for (int i = 0; i < N; ++i) {
+    const double a = -2.0 * M_PI * (double)i / (double)N;
+    t[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+// For each column tile of one axis: the source bins (by wrapped stencil
+// base) whose B-spline stencils reach the tile.
+bool tile_sources(int n, int t, int P, std::vector<int4>& out) {
+  const int nt = (n + t - 1) / t;
+  out.assign(nt, make_int4(0, 0, 0, 0));
+  for (int T = 0; T < nt; ++T) {
+    std::vector<int> bins;
+    for (int base = 0; base < n; ++base) {
+      bool hit = false;
+      for (int o = 0; o < P && !hit; ++o) {
+        const int g = (base + o) % n;
+        hit = g >= T * t && g < std::min(n, (T + 1) * t);
+      }
+      const int bin = base / t;
+      if (hit && std::find(bins.begin(), bins.end(), bin) == bins.end()) bins.push_back(bin);
+    }
+    if (bins.size() > 3) return false;
+    int4 v = make_int4((int)bins.size(), 0, 0, 0);
+    if (bins.size() > 0) v.y = bins[0];
+    if (bins.size() > 1) v.z = bins[1];
+    if (bins.size() > 2) v.w = bins[2];
+    out[T] = v;
+  }
+  return true;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count, size_t& total) {
+  *p = nullptr;
+  if (count == 0) return SAXS_OK;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SAXS_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+  }
+  total += count * sizeof(T);
+  return SAXS_OK;
+}
+
+template <typename T>
+int upload(T* dst, const T* src, size_t count) {
+  if (count == 0) return SAXS_OK;
+  HIP_TRY(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return SAXS_OK;
+}
+
+}  // namespace
+
+struct saxs_plan {
+  saxs::Params p{};            // geometry; F/N/oc set per call
+  saxs_desc desc{};
+  int max_frames = 0, max_atoms = 0, nkeys_max = 0;
+  int lut_cap = 0;             // lut_len the workspace was sized for
+  size_t bytes = 0;
+  // tables
+  int4* srcx = nullptr;
+  int4* srcy = nullptr;
+  float2* twz = nullptr;
+  float2* twy = nullptr;
+  float2* twx = nullptr;
+  float* modx = nullptr;
+  float* mody = nullptr;
+  float* modz = nullptr;
+  int* lut_bin = nullptr;
+  float* lut_ff = nullptr;
+  // workspace
+  int* counts = nullptr;
+  int* offsets = nullptr;
+  int2* keyslot = nullptr;
+  float4* sorted = nullptr;
+  double* pad_sum = nullptr;
+  float2* zbuf = nullptr;
+  float2* ybuf = nullptr;
+  double* partial = nullptr;
+  double* hframe = nullptr;    // [max_frames][2*num_bins] scratch for gradients
+  float* grid = nullptr;       // [E][n^3] adjoint density / plain-path grid
+  float* sup = nullptr;        // [nn^3] plain-path embedded grid
+  float2* spec = nullptr;      // [nn_x][nn_y][h] plain-path spectrum
+  float2* acc = nullptr;       // [nn_x][nn_y][h] plain-path accumulator
+  double* padg = nullptr;      // [F*E] adjoint pad terms
+};
+
+namespace {
+
+// Kept frequencies and pass chunking from lut_len (shells n < lut_len).
+int derive_ranges(saxs_plan* pl, int lut_len) {
+  saxs::Params& p = pl->p;
+  p.lut_len = lut_len;
+  int K = 0;
+  while ((long long)(K + 1) * (K + 1) < (long long)lut_len) ++K;  // K^2 <= lut_len-1
+  auto kept = [&](int N, int& k, int& Kout) {
+    if (2 * K + 1 >= N) {
+      k = N;
+      Kout = N / 2;
+    } else {
+      k = 2 * K + 1;
+      Kout = K;
+    }
+  };
+  kept(p.nn[0], p.kx, p.Kx);
+  kept(p.nn[1], p.ky, p.Ky);
+  const int h = p.nn[2] / 2 + 1;
+  p.kz = std::min(K + 1, h);
+
+  const int capy = fft_line_cap(p.fy), capx = fft_line_cap(p.fx);
+  const size_t budget = 48 * 1024;
+  int kcy = std::min({p.kz, capy, (int)(budget / (sizeof(float2) * p.lsy))});
+  int kcx = std::min({p.kz, capx, (int)(budget / (sizeof(float2) * p.lsx)),
+                      saxs::kOutMax * saxs::kWG / p.kx});
+  if (kcy < 1 || kcx < 1) return fail(SAXS_EUNSUP, "FFT lines do not fit the LDS budget");
+  p.nchy = (p.kz + kcy - 1) / kcy;
+  p.kcy = (p.kz + p.nchy - 1) / p.nchy;
+  p.nchx = (p.kz + kcx - 1) / kcx;
+  p.kcx = (p.kz + p.nchx - 1) / p.nchx;
+  if (saxs::yfft_lds_bytes(p) > kLdsLimit || saxs::xfft_lds_bytes(p) > kLdsLimit)
+    return fail(SAXS_EUNSUP, "pass LDS exceeds 64 KiB (num_bins %d too large?)", p.num_bins);
+  return SAXS_OK;
+}
+
+void free_plan(saxs_plan* pl) {
+  void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
+                  pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
+                  pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
+                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete pl;
+}
+
+saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem) {
+  saxs::Buffers b{};
+  b.xyz = xyz;
+  b.elem = elem;
+  b.counts = pl->counts;
+  b.offsets = pl->offsets;
+  b.keyslot = pl->keyslot;
+  b.sorted = pl->sorted;
+  b.pad_sum = pl->pad_sum;
+  b.srcx = pl->srcx;
+  b.srcy = pl->srcy;
+  b.zbuf = pl->zbuf;
+  b.ybuf = pl->ybuf;
+  b.twz = pl->twz;
+  b.twy = pl->twy;
+  b.twx = pl->twx;
+  b.modx = pl->modx;
+  b.mody = pl->mody;
+  b.modz = pl->modz;
+  b.lut_bin = pl->lut_bin;
+  b.lut_ff = pl->lut_ff;
+  b.partial = pl->partial;
+  return b;
+}
+
+int check_call(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms, int n_frames,
+               const float* oc_diag) {
+  if (!pl) return fail(SAXS_EINVAL, "plan is NULL");
+  if (n_atoms < 0 || n_atoms > pl->max_atoms)
+    return fail(SAXS_EINVAL, "n_atoms %d outside [0, %d]", n_atoms, pl->max_atoms);
+  if (n_frames < 1 || n_frames > pl->max_frames)
+    return fail(SAXS_EINVAL, "n_frames %d outside [1, %d]", n_frames, pl->max_frames);
+  if (n_atoms > 0 && (!xyz || !elem)) return fail(SAXS_EINVAL, "xyz/elem pointer is NULL");
+  if (!oc_diag) return fail(SAXS_EINVAL, "oc_diag is NULL");
+  for (int a = 0; a < 3; ++a)
+    if (!(oc_diag[a] > 0.0f) || !std::isfinite(oc_diag[a]))
+      return fail(SAXS_EINVAL, "oc_diag[%d] = %g is not a positive finite inverse length", a,
+                  (double)oc_diag[a]);
+  return SAXS_OK;
+}
+
+// Enqueues the forward pipeline up to the per-block histograms.
+int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms,
+                        int n_frames, const float* oc_diag, hipStream_t st) {
+  saxs::Params p = pl->p;
+  p.F = n_frames;
+  p.N = n_atoms;
+  for (int a = 0; a < 3; ++a) p.oc[a] = oc_diag[a];
+  saxs::Buffers b = make_buffers(pl, xyz, elem);
+  const int nkeys = n_frames * p.E * p.nty * p.ntx;
+  const int nclear = std::max(nkeys + 1, n_frames * p.E);
+  hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, pl->counts,
+                     nkeys + 1, pl->pad_sum, n_frames * p.E);
+  if (n_atoms > 0) {
+    dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
+    if (p.order == 4)
+      hipLaunchKernelGGL(saxs::k_prep<4>, ga, dim3(saxs::kWG), 0, st, p, b);
+    else
+      hipLaunchKernelGGL(saxs::k_prep<6>, ga, dim3(saxs::kWG), 0, st, p, b);
+  }
+  hipLaunchKernelGGL(saxs::k_scan, dim3(1), dim3(1024), 0, st, pl->counts, pl->offsets, nkeys);
+  if (n_atoms > 0) {
+    dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
+    if (p.order == 4)
+      hipLaunchKernelGGL(saxs::k_scatter<4>, ga, dim3(saxs::kWG), 0, st, p, b);
+    else
+      hipLaunchKernelGGL(saxs::k_scatter<6>, ga, dim3(saxs::kWG), 0, st, p, b);
+  }
+  dim3 gs(p.ntx * p.nty, p.E, n_frames);
+  const size_t lds_s = saxs::spread_lds_bytes(p);
+  if (p.order == 4)
+    hipLaunchKernelGGL(saxs::k_spread_z<4>, gs, dim3(saxs::kWG), lds_s, st, p, b);
+  else
+    hipLaunchKernelGGL(saxs::k_spread_z<6>, gs, dim3(saxs::kWG), lds_s, st, p, b);
+  hipLaunchKernelGGL(saxs::k_yfft, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
+                     saxs::yfft_lds_bytes(p), st, p, b);
+  hipLaunchKernelGGL(saxs::k_xfft_epi, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
+                     saxs::xfft_lds_bytes(p), st, p, b);
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t saxs_abi_version(void) { return 1; }
+
+const char* saxs_last_error(void) { return g_err.c_str(); }
+
+int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
+  g_err.clear();
+  if (!d || !out) return fail(SAXS_EINVAL, "desc/out is NULL");
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    if (d->n[a] < 1 || d->nn[a] < 1) return fail(SAXS_EINVAL, "grid sizes must be positive");
+    if (d->nn[a] < d->n[a])
+      return fail(SAXS_EUNSUP, "FFT grid %d smaller than cell grid %d on axis %d", d->nn[a],
+                  d->n[a], a);
+  }
+  if (d->order != 4 && d->order != 6)
+    return fail(SAXS_EUNSUP, "spline order %d unsupported (4 or 6)", d->order);
+  for (int a = 0; a < 3; ++a)
+    if (d->n[a] < 2 * d->order)
+      return fail(SAXS_EUNSUP, "cell grid %d too small for order %d", d->n[a], d->order);
+  if (d->num_elements < 1 || d->num_elements > saxs::kMaxElements)
+    return fail(SAXS_EINVAL, "num_elements %d outside [1, %d]", d->num_elements, saxs::kMaxElements);
+  if (d->num_bins < 1 || d->num_bins > 4096)
+    return fail(SAXS_EUNSUP, "num_bins %d outside [1, 4096]", d->num_bins);
+  if (d->border < 0 || 2 * d->border + 1 > d->n[2])
+    return fail(SAXS_EINVAL, "border %d invalid for n_z %d", d->border, d->n[2]);
+  if (d->lut_len < 1 || !d->bin_lut || !d->ff_lut || !d->mod_x || !d->mod_y || !d->mod_z)
+    return fail(SAXS_EINVAL, "tables missing");
+  if (d->max_atoms < 0 || d->max_frames < 1) return fail(SAXS_EINVAL, "capacity invalid");
+
+  saxs_plan* pl = new (std::nothrow) saxs_plan();
+  if (!pl) return fail(SAXS_ENOMEM, "host allocation failed");
+  pl->desc = *d;
+  saxs::Params& p = pl->p;
+  for (int a = 0; a < 3; ++a) {
+    p.n[a] = d->n[a];
+    p.nn[a] = d->nn[a];
+    p.nf[a] = (d->nn[a] % 2 == 0) ? d->nn[a] / 2 : d->nn[a] / 2 + 1;
+  }
+  p.order = d->order;
+  p.E = d->num_elements;
+  p.border = d->border;
+  p.num_bins = d->num_bins;
+  p.inv_count = 1.0 / ((double)d->n[0] * d->n[1] * (2 * d->border + 1));
+  if (!factor_radices(p.nn[2], p.fz) || !factor_radices(p.nn[1], p.fy) ||
+      !factor_radices(p.nn[0], p.fx) || p.nn[0] < 2 || p.nn[1] < 2 || p.nn[2] < 2) {
+    free_plan(pl);
+    return fail(SAXS_EUNSUP, "FFT sizes must be 2,3,5,7-smooth and >= 2");
+  }
+  p.lsz = lds_stride(p.nn[2]);
+  p.lsy = lds_stride(p.nn[1]);
+  p.lsx = lds_stride(p.nn[0]);
+
+  // Column tile: even ty (two real columns per complex line), LDS <= 64 KiB.
+  int tx = d->tile_x > 0 ? d->tile_x : 4, ty = d->tile_y > 0 ? d->tile_y : 8;
+  tx = std::min(tx, p.n[0]);
+  ty = std::min(ty, p.n[1]);
+  if (ty % 2) ty = std::max(2, ty - 1);
+  for (;;) {
+    p.tx = tx;
+    p.ty = ty;
+    const int nl = tx * ty / 2;
+    if (saxs::spread_lds_bytes(p) <= kLdsLimit && nl <= fft_line_cap(p.fz)) break;
+    if (tx >= ty && tx > 1)
+      tx = (tx + 1) / 2;
+    else if (ty > 2)
+      ty = std::max(2, (ty / 2) & ~1);
+    else {
+      free_plan(pl);
+      return fail(SAXS_EUNSUP, "cell column n_z=%d does not fit the LDS budget", p.n[2]);
+    }
+  }
+  if (p.n[1] < 2) {
+    free_plan(pl);
+    return fail(SAXS_EUNSUP, "n_y must be >= 2");
+  }
+  p.ntx = (p.n[0] + p.tx - 1) / p.tx;
+  p.nty = (p.n[1] + p.ty - 1) / p.ty;
+  std::vector<int4> sx, sy;
+  if (!tile_sources(p.n[0], p.tx, p.order, sx) || !tile_sources(p.n[1], p.ty, p.order, sy)) {
+    free_plan(pl);
+    return fail(SAXS_EUNSUP, "spreading tile %dx%d too narrow for order %d", p.tx, p.ty, p.order);
+  }
+  int rc = derive_ranges(pl, d->lut_len);
+  if (rc) {
+    free_plan(pl);
+    return rc;
+  }
+  pl->lut_cap = d->lut_len;
+  pl->max_frames = d->max_frames;
+  pl->max_atoms = d->max_atoms;
+  pl->nkeys_max = d->max_frames * p.E * p.nty * p.ntx;
+
+  const size_t F = d->max_frames, N = d->max_atoms, E = p.E;
+  const size_t n3 = (size_t)p.n[0] * p.n[1] * p.n[2];
+  size_t& tot = pl->bytes;
+#define ALLOC(ptr, cnt)                  \
+  do {                                   \
+    int r_ = dev_alloc(&(ptr), (cnt), tot); \
+    if (r_) {                            \
+      free_plan(pl);                     \
+      return r_;                         \
+    }                                    \
+  } while (0)
+  ALLOC(pl->srcx, sx.size());
+  ALLOC(pl->srcy, sy.size());
+  ALLOC(pl->twz, (size_t)p.nn[2]);
+  ALLOC(pl->twy, (size_t)p.nn[1]);
+  ALLOC(pl->twx, (size_t)p.nn[0]);
+  ALLOC(pl->modx, (size_t)p.nn[0]);
+  ALLOC(pl->mody, (size_t)p.nn[1]);
+  ALLOC(pl->modz, (size_t)p.nn[2]);
+  ALLOC(pl->lut_bin, (size_t)d->lut_len);
+  ALLOC(pl->lut_ff, E * d->lut_len);
+  ALLOC(pl->counts, (size_t)pl->nkeys_max + 1);
+  ALLOC(pl->offsets, (size_t)pl->nkeys_max + 1);
+  ALLOC(pl->keyslot, F * N);
+  ALLOC(pl->sorted, F * N);
+  ALLOC(pl->pad_sum, F * E);
+  ALLOC(pl->padg, F * E);
+  ALLOC(pl->zbuf, F * E * p.n[0] * p.n[1] * p.kz);
+  ALLOC(pl->ybuf, F * E * p.ky * p.n[0] * p.kz);
+  ALLOC(pl->partial, F * p.ky * p.nchx * 2 * p.num_bins);
+  ALLOC(pl->hframe, F * 2 * p.num_bins);
+#undef ALLOC
+  const std::vector<float2> tz = twiddles(p.nn[2]), ty_ = twiddles(p.nn[1]), tx_ = twiddles(p.nn[0]);
+  int r = 0;
+  if ((r = upload(pl->srcx, sx.data(), sx.size())) || (r = upload(pl->srcy, sy.data(), sy.size())) ||
+      (r = upload(pl->twz, tz.data(), tz.size())) || (r = upload(pl->twy, ty_.data(), ty_.size())) ||
+      (r = upload(pl->twx, tx_.data(), tx_.size())) ||
+      (r = upload(pl->modx, d->mod_x, (size_t)p.nn[0])) ||
+      (r = upload(pl->mody, d->mod_y, (size_t)p.nn[1])) ||
+      (r = upload(pl->modz, d->mod_z, (size_t)p.nn[2])) ||
+      (r = upload(pl->lut_bin, d->bin_lut, (size_t)d->lut_len)) ||
+      (r = upload(pl->lut_ff, d->ff_lut, E * d->lut_len))) {
+    free_plan(pl);
+    return r;
+  }
+  *out = pl;
+  return SAXS_OK;
+}
+
+int saxs_plan_set_tables(saxs_plan* pl, int32_t lut_len, const int32_t* bin_lut, const float* ff_lut) {
+  g_err.clear();
+  if (!pl || !bin_lut || !ff_lut) return fail(SAXS_EINVAL, "NULL argument");
+  if (lut_len < 1 || lut_len > pl->lut_cap)
+    return fail(SAXS_EINVAL, "lut_len %d outside [1, %d] (recreate the plan)", lut_len, pl->lut_cap);
+  saxs::Params saved = pl->p;
+  int rc = derive_ranges(pl, lut_len);
+  if (rc) {
+    pl->p = saved;
+    return rc;
+  }
+  if (pl->p.kz > saved.kz || pl->p.ky > saved.ky || pl->p.nchx > saved.nchx) {
+    pl->p = saved;
+    return fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
+  }
+  HIP_TRY(hipMemcpy(pl->lut_bin, bin_lut, sizeof(int32_t) * lut_len, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pl->lut_ff, ff_lut, sizeof(float) * lut_len * pl->p.E, hipMemcpyHostToDevice));
+  return SAXS_OK;
+}
+
+int saxs_forward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                 int32_t n_frames, const float* oc_diag, double* H, double* C, void* stream) {
+  g_err.clear();
+  int rc = check_call(pl, xyz, elem, n_atoms, n_frames, oc_diag);
+  if (rc) return rc;
+  if (!H || !C) return fail(SAXS_EINVAL, "H/C pointer is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
+  if (rc) return rc;
+  const int nblk = pl->p.ky * pl->p.nchx;
+  hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
+                     pl->partial, nblk, n_frames, pl->p.num_bins, H, C, 0);
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                        int32_t n_frames, const float* oc_diag, double* Hf, double* Cf,
+                        void* stream) {
+  g_err.clear();
+  int rc = check_call(pl, xyz, elem, n_atoms, n_frames, oc_diag);
+  if (rc) return rc;
+  if (!Hf || !Cf) return fail(SAXS_EINVAL, "Hf/Cf pointer is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
+  if (rc) return rc;
+  const int nblk = pl->p.ky * pl->p.nchx;
+  hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, n_frames), dim3(saxs::kWG), 0, st,
+                     pl->partial, nblk, 1, pl->p.num_bins, Hf, Cf, 1);
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+int saxs_backward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                  int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
+                  void* stream) {
+  g_err.clear();
+  (void)pl; (void)xyz; (void)elem; (void)n_atoms; (void)n_frames; (void)oc_diag; (void)gH;
+  (void)grad; (void)stream;
+  return fail(SAXS_EUNSUP, "saxs_backward is not available in this build");
+}
+
+int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                       const float* oc_diag, double* H, double* C, void* stream) {
+  g_err.clear();
+  (void)pl; (void)xyz; (void)elem; (void)n_atoms; (void)oc_diag; (void)H; (void)C; (void)stream;
+  return fail(SAXS_EUNSUP, "saxs_forward_plain is not available in this build");
+}
+
+int64_t saxs_plan_workspace_bytes(const saxs_plan* pl) { return pl ? (int64_t)pl->bytes : 0; }
+
+int saxs_plan_kept(const saxs_plan* pl, int32_t* kx, int32_t* ky, int32_t* kz) {
+  if (!pl || !kx || !ky || !kz) return fail(SAXS_EINVAL, "NULL argument");
+  *kx = pl->p.kx;
+  *ky = pl->p.ky;
+  *kz = pl->p.kz;
+  return SAXS_OK;
+}
+
+int saxs_plan_destroy(saxs_plan* pl) {
+  if (pl) free_plan(pl);
+  return SAXS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,487 @@
+// Device kernels of the forward scattering path (gfx950).
+//
+// Pipeline for a batch of F frames of one system (E element types):
+//   k_clear         zero bin counters and pad sums
+//   k_prep<P>       per atom: grid cell and fraction (saxsDeviceKernels.cu:352-368),
+//                   column-tile key, slot in its bin, border-slab weight sum
+//   k_scan          exclusive scan of the per-(frame, element, tile) counts
+//   k_scatter<P>    counting-sort the atoms by (frame, element, tile)
+//   k_spread_z<P>   per (frame, element, column tile): gather-spread the atoms
+//                   into an LDS column block (saxsDeviceKernels.cu:338-398),
+//                   subtract the bulk-solvent border mean (Q5 of the reference's
+//                   pad/embed, :413-465), R2C along z, keep kz <= Kz
+//   k_yfft          FFT along y, keep |m_y| <= K
+//   k_xfft_epi      FFT along x, keep |m_x| <= K, then per point:
+//                   A = sum_e f_e(q) F_e (:177-216), I = |A|^2 modX modY modZ
+//                   (:142-162), weighted histogram into LDS (:67-125)
+//   k_hist_reduce   deterministic reduction of the per-block histograms
+//
+// Layouts (complex64, x slowest):
+//   zbuf [F][E][n_x][n_y][kz]     ybuf [F][E][ky][n_x][kz]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_lds.hpp"
+
+namespace saxs {
+
+constexpr int kWG = 256;       // threads per workgroup (4 wave64)
+constexpr int kMaxC = 32;      // complex values per thread in one FFT stage
+constexpr int kOutMax = 16;    // x-pass output points per thread
+constexpr int kMaxElements = 64;
+
+struct Params {
+  int n[3], nn[3];
+  int order, E, border;
+  int tx, ty, ntx, nty;
+  int kz, ky, Ky, kx, Kx;
+  int nf[3];
+  int num_bins, lut_len;
+  int F, N;
+  float oc[3];
+  int kcy, nchy, kcx, nchx;
+  int lsz, lsy, lsx;     // LDS line strides (complex) of the three passes
+  double inv_count;      // 1 / number of border voxels
+  FftDesc fz, fy, fx;
+};
+
+struct Buffers {
+  const float* xyz;
+  const int* elem;
+  int* counts;           // [F*E*nty*ntx + 1]
+  int* offsets;          // [F*E*nty*ntx + 1]
+  int2* keyslot;         // [F*N]
+  float4* sorted;        // [F*N]  (r_x, r_y, r_z, atom index bits)
+  double* pad_sum;       // [F*E]
+  const int4* srcx;      // [ntx] {count, bin0, bin1, bin2}
+  const int4* srcy;      // [nty]
+  float2* zbuf;
+  float2* ybuf;
+  const float2* twz;
+  const float2* twy;
+  const float2* twx;
+  const float* modx;
+  const float* mody;
+  const float* modz;
+  const int* lut_bin;
+  const float* lut_ff;   // [E][lut_len]
+  double* partial;       // [F][ky][nchx][2*num_bins]
+};
+
+// ----------------------------------------------------------- atom helpers --
+
+// Cell coordinate r_a = float(n_a (u - rint(u - 0.5))), u = float(oc_a x_a),
+// the float/double mix of saxsDeviceKernels.cu:355-361 (orthorhombic oc).
+__device__ __forceinline__ float cell_coord(float x, float oc, int n) {
+  const float u = __fmul_rn(oc, x);
+  const double ud = (double)u;
+  const double r = (double)n * (ud - rint(ud - 0.5));
+  return (float)r;
+}
+
+// Cardinal B-spline weights th[k] = M_P(w + P - 1 - k) by the recursion of
+// Splines.cu:47-80 (Init then OnePass for k = 3..P).
+template <int P>
+__device__ __forceinline__ void bspline(float w, float* th) {
+  th[0] = 1.0f - w;
+  th[1] = w;
+#pragma unroll
+  for (int i = 2; i < P; ++i) th[i] = 0.0f;
+#pragma unroll
+  for (int k = 3; k <= P; ++k) {
+    const float div = 1.0f / (float)(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+#pragma unroll
+    for (int j = 1; j <= k - 2; ++j)
+      th[k - j - 1] = div * ((w + (float)j) * th[k - j - 2] + ((float)(k - j) - w) * th[k - j - 1]);
+    th[0] = div * (1.0f - w) * th[0];
+  }
+}
+
+// dth/dw: order P-1 weights differenced (Splines.cu:84-92).
+template <int P>
+__device__ __forceinline__ void bspline_d(float w, float* dth) {
+  float t[P];
+  bspline<P - 1>(w, t);
+  dth[0] = -t[0];
+#pragma unroll
+  for (int o = 1; o < P - 1; ++o) dth[o] = t[o - 1] - t[o];
+  dth[P - 1] = t[P - 2];
+}
+
+__device__ __forceinline__ int wrap_base(int m, int P, int n) {
+  int b = m - P;
+  return b < 0 ? b + n : b;
+}
+
+// ------------------------------------------------------------------ kernels --
+
+__global__ void k_clear(int* counts, int nkeys, double* pad_sum, int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nkeys) counts[i] = 0;
+  if (i < npad) pad_sum[i] = 0.0;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
+  __shared__ double padacc[kMaxElements];
+  const int f = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0.0;
+  __syncthreads();
+  if (i < p.N) {
+    const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
+    const int e = b.elem[i];
+    int base[3];
+    float w[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float r = cell_coord(x[a], p.oc[a], p.n[a]);
+      const int m = (int)r;
+      w[a] = r - (float)m;
+      base[a] = wrap_base(m, P, p.n[a]);
+    }
+    const int key = ((f * p.E + e) * p.nty + base[1] / p.ty) * p.ntx + base[0] / p.tx;
+    const int slot = atomicAdd(&b.counts[key], 1);
+    b.keyslot[(size_t)f * p.N + i] = make_int2(key, slot);
+    // Border-slab share of this atom's unit weight: sum over the stencil of
+    // th_x th_y th_z restricted to z <= d or z >= n_z - d.
+    float th[P];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    bspline<P>(w[0], th);
+#pragma unroll
+    for (int o = 0; o < P; ++o) sx += (double)th[o];
+    bspline<P>(w[1], th);
+#pragma unroll
+    for (int o = 0; o < P; ++o) sy += (double)th[o];
+    bspline<P>(w[2], th);
+#pragma unroll
+    for (int o = 0; o < P; ++o) {
+      int z = base[2] + o;
+      if (z >= p.n[2]) z -= p.n[2];
+      if (z <= p.border || z >= p.n[2] - p.border) sz += (double)th[o];
+    }
+    atomicAdd(&padacc[e], sx * sy * sz);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
+    const double v = padacc[e];
+    if (v != 0.0) atomicAdd(&b.pad_sum[f * p.E + e], v);
+  }
+}
+
+// Exclusive scan of counts[0..K) into offsets[0..K], one workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int* offsets, int K) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int chunk = (K + 1023) / 1024;
+  const int beg = min(K, t * chunk), end = min(K, beg + chunk);
+  int sum = 0;
+  for (int i = beg; i < end; ++i) sum += counts[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+  for (int i = beg; i < end; ++i) {
+    offsets[i] = run;
+    run += counts[i];
+  }
+  if (t == 1023) offsets[K] = s[1023];
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
+  const int f = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.N) return;
+  const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
+  const int2 ks = b.keyslot[(size_t)f * p.N + i];
+  float4 rec;
+  rec.x = cell_coord(x[0], p.oc[0], p.n[0]);
+  rec.y = cell_coord(x[1], p.oc[1], p.n[1]);
+  rec.z = cell_coord(x[2], p.oc[2], p.n[2]);
+  rec.w = __int_as_float(i);
+  b.sorted[b.offsets[ks.x] + ks.y] = rec;
+}
+
+// LDS bytes of k_spread_z.
+__host__ __device__ inline size_t spread_lds_bytes(const Params& p) {
+  const size_t rho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
+  const size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t tw = (size_t)p.nn[2] * sizeof(float2);
+  return ((rho + 15) & ~size_t(15)) + buf + tw;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], nnz = p.nn[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  float* rho = reinterpret_cast<float*>(smem);
+  float2* buf = reinterpret_cast<float2*>(smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15)));
+  float2* tw = buf + (size_t)nl * p.lsz;
+
+  __shared__ int s_start[9], s_pref[10], s_nseg;
+  for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
+  for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
+  if (threadIdx.x == 0) {
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int bxs[3] = {sx.y, sx.z, sx.w}, bys[3] = {sy.y, sy.z, sy.w};
+    int ns = 0, acc = 0;
+    for (int iy = 0; iy < sy.x; ++iy)
+      for (int ix = 0; ix < sx.x; ++ix) {
+        const int key = ((f * p.E + e) * p.nty + bys[iy]) * p.ntx + bxs[ix];
+        const int o0 = b.offsets[key], o1 = b.offsets[key + 1];
+        s_start[ns] = o0;
+        s_pref[ns] = acc;
+        acc += (o1 - o0) * P * P;
+        ++ns;
+      }
+    s_pref[ns] = acc;
+    s_nseg = ns;
+  }
+  __syncthreads();
+  const int nseg = s_nseg;
+  const int total = s_pref[nseg];
+  for (int item = threadIdx.x; item < total; item += blockDim.x) {
+    int s = 0;
+    while (s + 1 < nseg && item >= s_pref[s + 1]) ++s;
+    const int local = item - s_pref[s];
+    const int a = local / (P * P);
+    const int col = local - a * (P * P);
+    const int o = col / P, o2 = col - o * P;
+    const float4 rec = b.sorted[s_start[s] + a];
+    const int mx = (int)rec.x, my = (int)rec.y;
+    int gx = mx - P + o;
+    if (gx < 0) gx += p.n[0];
+    const int lx = gx - X0;
+    if (lx < 0 || lx >= wx) continue;
+    int gy = my - P + o2;
+    if (gy < 0) gy += p.n[1];
+    const int ly = gy - Y0;
+    if (ly < 0 || ly >= wy) continue;
+    float th[P], tz[P];
+    bspline<P>(rec.x - (float)mx, th);
+    const float fo = th[o];
+    bspline<P>(rec.y - (float)my, th);
+    const float fp = fo * th[o2];
+    const int mz = (int)rec.z;
+    bspline<P>(rec.z - (float)mz, tz);
+    float* colp = rho + (lx * p.ty + ly) * nz;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      int gz = mz - P + q;
+      if (gz < 0) gz += nz;
+      atomicAdd(colp + gz, fp * tz[q]);
+    }
+  }
+  __syncthreads();
+
+  // Bulk-solvent subtraction inside the cell (split in two floats so the
+  // rounding of the mean does not leave a systematic box-shaped residue),
+  // two real columns packed per complex line, zero padding beyond n_z.
+  const double pad = b.pad_sum[f * p.E + e] * p.inv_count;
+  const float pad_hi = (float)pad;
+  const float pad_lo = (float)(pad - (double)pad_hi);
+  for (int i = threadIdx.x; i < nl * nnz; i += blockDim.x) {
+    const int l = i / nnz, z = i - l * nnz;
+    const int c0 = 2 * l;
+    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+    float re = 0.0f, im = 0.0f;
+    if (z < nz && lx < wx) {
+      if (ly < wy) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
+      if (ly + 1 < wy) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
+    }
+    buf[l * p.lsz + z] = make_float2(re, im);
+  }
+  __syncthreads();
+  fft_lines<kMaxC>(buf, p.lsz, nl, p.fz, tw);
+
+  // Unpack the two real transforms: X_a(k) = (Z(k) + conj Z(-k))/2,
+  // X_b(k) = (Z(k) - conj Z(-k))/(2i).
+  const size_t fe = (size_t)f * p.E + e;
+  for (int i = threadIdx.x; i < nl * p.kz; i += blockDim.x) {
+    const int l = i / p.kz, k = i - l * p.kz;
+    const int c0 = 2 * l;
+    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+    if (lx >= wx || ly >= wy) continue;
+    const float2 Z = buf[l * p.lsz + k];
+    const float2 Zm = buf[l * p.lsz + (k == 0 ? 0 : nnz - k)];
+    const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+    const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
+    b.zbuf[row * p.kz + k] = Xa;
+    if (ly + 1 < wy) b.zbuf[(row + 1) * p.kz + k] = Xb;
+  }
+}
+
+__host__ __device__ inline size_t yfft_lds_bytes(const Params& p) {
+  return (size_t)p.kcy * p.lsy * sizeof(float2) + (size_t)p.nn[1] * sizeof(float2);
+}
+
+__global__ void __launch_bounds__(kWG) k_yfft(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int x = blockIdx.x, ch = blockIdx.y;
+  const size_t fe = blockIdx.z;
+  const int k0 = ch * p.kcy;
+  const int kc = min(p.kcy, p.kz - k0);
+  const int ny = p.n[1], nny = p.nn[1];
+  float2* buf = reinterpret_cast<float2*>(smem);
+  float2* tw = buf + (size_t)p.kcy * p.lsy;
+  for (int i = threadIdx.x; i < nny; i += blockDim.x) tw[i] = b.twy[i];
+  const float2* src = b.zbuf + (fe * p.n[0] + x) * (size_t)ny * p.kz;
+  for (int i = threadIdx.x; i < ny * kc; i += blockDim.x) {
+    const int y = i / kc, k = i - y * kc;
+    buf[k * p.lsy + y] = src[(size_t)y * p.kz + k0 + k];
+  }
+  for (int i = threadIdx.x; i < (nny - ny) * kc; i += blockDim.x) {
+    const int y = ny + i / kc, k = i % kc;
+    buf[k * p.lsy + y] = make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  fft_lines<kMaxC>(buf, p.lsy, kc, p.fy, tw);
+  for (int i = threadIdx.x; i < p.ky * kc; i += blockDim.x) {
+    const int j = i / kc, k = i - j * kc;
+    const int iy = (p.ky == nny) ? j : (j <= p.Ky ? j : nny - (p.ky - j));
+    b.ybuf[((fe * p.ky + j) * p.n[0] + x) * p.kz + k0 + k] = buf[k * p.lsy + iy];
+  }
+}
+
+__host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
+  return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) + (size_t)p.kcx * p.lsx * sizeof(float2) +
+         (size_t)p.nn[0] * sizeof(float2);
+}
+
+__global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int k0 = ch * p.kcx;
+  const int kc = min(p.kcx, p.kz - k0);
+  const int nx = p.n[0], nnx = p.nn[0];
+  const int nb = p.num_bins;
+  double* Hs = reinterpret_cast<double*>(smem);
+  unsigned int* Cs = reinterpret_cast<unsigned int*>(Hs + nb);
+  float2* buf = reinterpret_cast<float2*>(smem + (((size_t)nb * 12 + 15) & ~size_t(15)));
+  float2* tw = buf + (size_t)p.kcx * p.lsx;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    Hs[i] = 0.0;
+    Cs[i] = 0u;
+  }
+  for (int i = threadIdx.x; i < nnx; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = (p.ky == p.nn[1]) ? jy : (jy <= p.Ky ? jy : p.nn[1] - (p.ky - jy));
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const int nout = p.kx * kc;
+  float2 A[kOutMax];
+#pragma unroll
+  for (int u = 0; u < kOutMax; ++u) A[u] = make_float2(0.0f, 0.0f);
+
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * p.kz;
+    for (int i = threadIdx.x; i < nx * kc; i += blockDim.x) {
+      const int x = i / kc, k = i - x * kc;
+      buf[k * p.lsx + x] = src[(size_t)x * p.kz + k0 + k];
+    }
+    for (int i = threadIdx.x; i < (nnx - nx) * kc; i += blockDim.x) {
+      const int x = nx + i / kc, k = i % kc;
+      buf[k * p.lsx + x] = make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    fft_lines<kMaxC>(buf, p.lsx, kc, p.fx, tw);
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kOutMax; ++u) {
+      const int idx = threadIdx.x + u * blockDim.x;
+      if (idx < nout) {
+        const int jx = idx / kc, k = idx - jx * kc;
+        const int ix = (p.kx == nnx) ? jx : (jx <= p.Kx ? jx : nnx - (p.kx - jx));
+        const int mx = ix < p.nf[0] ? ix : ix - nnx;
+        const int mz = k0 + k;
+        const int nsq = mx * mx + my * my + mz * mz;
+        if (nsq < p.lut_len) {
+          const float fv = ff[nsq];
+          const float2 Fv = buf[k * p.lsx + ix];
+          A[u].x += fv * Fv.x;
+          A[u].y += fv * Fv.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kOutMax; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    if (idx < nout) {
+      const int jx = idx / kc, k = idx - jx * kc;
+      const int ix = (p.kx == nnx) ? jx : (jx <= p.Kx ? jx : nnx - (p.kx - jx));
+      const int mx = ix < p.nf[0] ? ix : ix - nnx;
+      const int kz = k0 + k;
+      const int nsq = mx * mx + my * my + kz * kz;
+      if (nsq < p.lut_len) {
+        const int bin = b.lut_bin[nsq];
+        if (bin >= 0) {
+          const float mod = b.modx[ix] * b.mody[iy] * b.modz[kz];
+          const float I = (A[u].x * A[u].x + A[u].y * A[u].y) * mod;
+          const bool origin = (ix == 0) && (iy == 0) && (kz == 0);
+          atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
+          atomicAdd(&Cs[bin], origin ? 1u : 2u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = b.partial + (((size_t)f * p.ky + jy) * p.nchx + ch) * 2 * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    out[i] = Hs[i];
+    out[nb + i] = (double)Cs[i];
+  }
+}
+
+// Sums the per-block histograms of each frame (or of all frames) in a fixed
+// order.  grid = (num_bins, frames_out); per_frame selects overwrite of
+// Hf[f][b] vs accumulation into H[b].
+__global__ void __launch_bounds__(kWG) k_hist_reduce(const double* __restrict__ partial, int nblk,
+                                                     int frames_per_out, int nb, double* H,
+                                                     double* C, int per_frame) {
+  __shared__ double sh[kWG], sc[kWG];
+  const int bin = blockIdx.x, fo = blockIdx.y;
+  const size_t first = (size_t)fo * frames_per_out * nblk;
+  const size_t count = (size_t)frames_per_out * nblk;
+  double h = 0.0, c = 0.0;
+  for (size_t i = threadIdx.x; i < count; i += blockDim.x) {
+    const double* row = partial + (first + i) * 2 * nb;
+    h += row[bin];
+    c += row[nb + bin];
+  }
+  sh[threadIdx.x] = h;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = kWG / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[threadIdx.x] += sh[threadIdx.x + s];
+      sc[threadIdx.x] += sc[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (per_frame) {
+      H[(size_t)fo * nb + bin] = sh[0];
+      C[(size_t)fo * nb + bin] = sc[0];
+    } else {
+      H[bin] += sh[0];
+      C[bin] += sc[0];
+    }
+  }
+}
+
+}  // namespace saxs
