@@ -1,0 +1,282 @@
+"""Drop-in implementation of the ``pycusaxs_cuda.run`` entry point on the
+MI355X engine.
+
+Mirrors, in order:
+  argument validation and error mapping   pybind/cuda_bindings.cpp:11-168
+  config, banner and summary string       cudaSAXS.cu:48-232
+  frame loop                              Exec/RunSaxs.cu:94-271
+      get_atom_index -> element groups (std::map order, RunSaxs.cu:108-115)
+      read_frame(0) / get_box -> first box (RunSaxs.cu:124-127)
+      iter_frames_stream(begin, end+1, stride) (RunSaxs.cu:137-138)
+      per frame: "--> Frame: ..." line, forward (saxsKernel.cu:110-113)
+      saxs.dat rows "%10.5f%12.5e" (RunSaxs.cu:217-240), timing banner
+Frames are batched into one engine launch chain per batch; the Python side
+reads frame k+1 while the GPU works on frame k (launches are asynchronous),
+which is the reference's double buffer (RunSaxs.cu:141-170).
+
+Deliberate differences (the reference's behaviour there is a silent failure
+or undefined): a malformed frame dict, an exception raised by the frame
+iterator, an element with no form factor, and a triclinic box raise instead
+of ending the run early / reading garbage.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .formfactors import coefficients
+
+# Defaults of the reference's static Options (Exec/Options.cpp:3-19).
+DEFAULT_OUTPUT = "saxs.dat"
+DEFAULT_SIMULATION = "npt"
+FRAMES_PER_BATCH = int(os.environ.get("PYCUSAXS_AMD_BATCH", "16"))
+
+
+def _validate(grid, scaled_grid, begin, end, stride, order, scale_factor, bin_size, qcut):
+    """cuda_bindings.cpp:32-122: std::invalid_argument -> ValueError."""
+    grid = list(grid)
+    if len(grid) == 0:
+        raise ValueError("Grid must have at least one dimension")
+    if len(grid) == 1:
+        if grid[0] <= 0:
+            raise ValueError("Grid dimension must be positive")
+        grid = [int(grid[0])] * 3
+    elif len(grid) == 3:
+        if grid[0] <= 0 or grid[1] <= 0 or grid[2] <= 0:
+            raise ValueError("Grid dimensions must be positive")
+        grid = [int(v) for v in grid]
+    else:
+        raise ValueError("Grid must have either 1 or 3 dimensions (nx, ny, nz)")
+    scaled = list(scaled_grid or [])
+    if scaled:
+        if len(scaled) == 1:
+            if scaled[0] <= 0:
+                raise ValueError("Scaled grid dimension must be positive")
+            scaled = [int(scaled[0])] * 3
+        elif len(scaled) == 3:
+            if scaled[0] <= 0 or scaled[1] <= 0 or scaled[2] <= 0:
+                raise ValueError("Scaled grid dimensions must be positive")
+            scaled = [int(v) for v in scaled]
+        else:
+            raise ValueError("Scaled grid must have either 1 or 3 dimensions")
+    if stride <= 0:
+        raise ValueError("Stride must be positive")
+    if order < 1 or order > 8:
+        raise ValueError("Spline order must be between 1 and 8")
+    if bin_size < 0.0:
+        raise ValueError("Bin size must be non-negative")
+    if qcut < 0.0:
+        raise ValueError("Q cutoff must be non-negative")
+    if scale_factor < 0.0:
+        raise ValueError("Scale factor must be non-negative")
+    if not scaled and scale_factor == 0.0:
+        raise ValueError("Scale factor must be positive when scaled grid is not provided")
+    if begin < 0 or end < 0:
+        raise ValueError("Frame indices must be non-negative")
+    if begin > end:
+        raise ValueError("Begin frame must be less than or equal to end frame")
+    if order not in (4, 6):
+        # orders other than 4 are undefined in the reference's device code
+        # (Splines.h:16 fixes 4); this engine implements 4 and 6 exactly
+        raise ValueError(f"Spline order {order} is not supported by the MI355X engine (4 or 6)")
+    return grid, scaled
+
+
+def summary_text(topology, trajectory, grid, scaled, begin, end, stride, order, sigma, dq, qcut,
+                 water_model, ions: Dict[str, int]) -> str:
+    """Configuration summary (cudaSAXS.cu:97-217), plain text."""
+    bar = "═" * 64
+    s = ["", "╔" + bar + "╗", "║                  CuSAXS Configuration                         ║",
+         "╚" + bar + "╝", "", "Input Files", f"  Topology      : {topology}",
+         f"  Trajectory    : {trajectory}", "", "Frame Selection", f"  Start Frame   : {begin}",
+         f"  End Frame     : {end}", f"  Stride        : {stride}",
+         f"  Total Frames  : {(end - begin) // stride + 1}", "", "Grid Parameters"]
+    if scaled:
+        s += [f"  Primary Grid  : ({grid[0]}, {grid[1]}, {grid[2]})",
+              f"  Scaled Grid   : ({scaled[0]}, {scaled[1]}, {scaled[2]})"]
+    else:
+        s += [f"  Grid Size     : ({grid[0]}, {grid[1]}, {grid[2]})"]
+    s += [f"  Spline Order  : {order}", "", "SAXS Parameters", f"  Scale Factor  : {sigma:.3f}",
+          f"  Bin Size      : {dq:.4f}", f"  Q-cutoff      : {qcut:.3f}", "", "Solvent Model"]
+    if water_model:
+        s.append(f"  Water Model   : {water_model}")
+    for ion, count in sorted(ions.items()):
+        if count > 0:
+            sym = ion + ("⁺" if ion in ("Na", "K") else "⁻" if ion == "Cl" else "")
+            s.append(f"  {sym} atoms     : {count}")
+    s.append("  Padding Mode  : average")
+    return "\n".join(s) + "\n\n" + "-" * 66 + "\n"
+
+
+def format_rows(rows) -> str:
+    """One line per bin: '%10.5f%12.5e' (RunSaxs.cu:224-227)."""
+    return "".join("%10.5f%12.5e\n" % (q, v) for q, v in rows)
+
+
+def element_groups(index_map) -> tuple:
+    """(symbols in std::map order, atom indices, element id per gathered atom)."""
+    if not isinstance(index_map, dict):
+        raise ValueError("get_atom_index() must return a dict of element -> atom indices")
+    symbols = sorted(str(k) for k in index_map.keys())
+    for s in symbols:
+        coefficients(s)                     # unknown symbols raise ValueError
+    idx, ids = [], []
+    for e, s in enumerate(symbols):
+        key = s if s in index_map else next(k for k in index_map if str(k) == s)
+        v = np.asarray(list(index_map[key]), dtype=np.int64)
+        idx.append(v)
+        ids.append(np.full(len(v), e, dtype=np.int32))
+    idx = np.concatenate(idx) if idx else np.zeros(0, dtype=np.int64)
+    ids = np.concatenate(ids) if ids else np.zeros(0, dtype=np.int32)
+    return symbols, idx, ids
+
+
+def _frame_fields(fr):
+    if not isinstance(fr, dict):
+        raise ValueError(f"frame record must be a dict, got {type(fr).__name__}")
+    try:
+        frame = int(fr["frame"])
+        t = float(fr["time"])
+        pos = np.asarray(fr["positions"], dtype=np.float32)
+        box = np.asarray(fr["box"], dtype=np.float32).reshape(3, 3)
+    except (KeyError, TypeError, ValueError) as exc:
+        raise ValueError(f"malformed frame record: {exc}") from exc
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError(f"frame positions must be [N, 3], got {pos.shape}")
+    return frame, t, pos, box
+
+
+def run(obj_topology, topology: str, trajectory: str, grid: Sequence[int],
+        scaled_grid: Sequence[int] = (), begin: int = 0, end: int = 0, stride: int = 1,
+        output: str = "", order: int = 4, scale_factor: float = 1.0, bin_size: float = 0.0,
+        qcut: float = 0.0, water_model: str = "", sodium: int = 0, chlorine: int = 0,
+        simulation: str = "", *, dq_policy: str = "reference", quiet: bool = False):
+    """pycusaxs_cuda.run (cuda_bindings.cpp:172-195) on the MI355X engine.
+
+    Returns {"summary": str, "grid": [nx, ny, nz], "scaled_grid": [...]} where
+    scaled_grid echoes the input ([0, 0, 0] when it was derived from
+    scale_factor), as the reference does (cudaSAXS.cu:219-229)."""
+    grid3, scaled3 = _validate(grid, scaled_grid, int(begin), int(end), int(stride), int(order),
+                               float(scale_factor), float(bin_size), float(qcut))
+    if not topology:
+        raise ValueError("Topology path must not be empty.")
+    if not trajectory:
+        raise ValueError("Trajectory path must not be empty.")
+    sigma = float(np.float32(scale_factor))
+    out_path = output or DEFAULT_OUTPUT
+    sim = simulation or DEFAULT_SIMULATION
+    ions = {"Na": int(sodium), "Cl": int(chlorine)}
+    summary = summary_text(topology, trajectory, grid3, scaled3, int(begin), int(end), int(stride),
+                           int(order), sigma, float(bin_size), float(qcut), water_model, ions)
+    log = (lambda *a, **k: None) if quiet else print
+    log(summary, end="")
+    result = {"summary": summary, "grid": list(grid3),
+              "scaled_grid": list(scaled3) if scaled3 else [0, 0, 0]}
+    _run_frames(obj_topology, grid3, scaled3, int(begin), int(end), int(stride), int(order), sigma,
+                float(bin_size), float(qcut), out_path, sim, water_model, dq_policy, log)
+    return result
+
+
+def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, qcut_req,
+                out_path, sim, water_model, dq_policy, log):
+    import torch
+
+    from .engine import SaxsEngine, SystemSpec, resolve_grid
+
+    t_start = time.perf_counter()
+    symbols, idx, ids = element_groups(topo.get_atom_index())
+    topo.read_frame(0)
+    box0 = np.asarray(topo.get_box(), dtype=np.float32).reshape(3, 3)
+    n, nn = resolve_grid(grid3, sigma, scaled3 or None)
+    spec = SystemSpec(n, nn, order, tuple(symbols), sigma, dq_req, qcut_req, dq_policy)
+    n_atoms = len(idx)
+    batch = max(1, FRAMES_PER_BATCH)
+    eng = SaxsEngine(spec, box0, max_atoms=max(n_atoms, 1), max_frames=batch)
+    h = eng.hist
+    if h.qcut_reset:
+        log("\n--------------------------------------------------")
+        log(f"----- Qcut had to be reset to {h.qcut:.2f} from  {qcut_req:.2f} ----")
+        log("--------------------------------------------------\n")
+    if h.dq_reset and dq_policy == "reference":
+        log("\n--------------------------------------------------")
+        log(f"----- Dq had to be reset to {h.dq:.3f} from  {dq_req:.3f} ----")
+        log("--------------------------------------------------\n")
+    log(banner(n, nn, order, sigma, h.dq, h.qcut, water_model))
+    dev = eng.device
+    ids_t = torch.from_numpy(ids).to(dev)
+    idx_t = torch.from_numpy(idx).to(dev)
+    gpu_ms = 0.0
+    n_done = 0
+    pending: List[np.ndarray] = []
+    pending_box = None
+
+    def flush():
+        nonlocal gpu_ms, pending, pending_box
+        if not pending:
+            return
+        t0 = time.perf_counter()
+        host = torch.from_numpy(np.stack(pending)).pin_memory()
+        full = host.to(dev, non_blocking=True)
+        xyz = full.index_select(1, idx_t) if n_atoms else full[:, :0]
+        try:
+            eng.forward(xyz, ids_t, box=pending_box)
+        except (ValueError, RuntimeError) as exc:
+            # RunSaxs.cu:195-199 logs a failing frame and goes on
+            print(f"Error processing frames: {exc}", file=sys.stderr)
+        gpu_ms += (time.perf_counter() - t0) * 1e3
+        pending = []
+        pending_box = None
+
+    frames_iter = topo.iter_frames_stream(begin, end + 1, stride)
+    for fr in frames_iter:
+        frame, t, pos, box = _frame_fields(fr)
+        log(f"--> Frame: {frame:<7}  Time Step: {t:.2f} ps")
+        if pending and (pending_box is None or not np.array_equal(box, pending_box)
+                        or len(pending) >= batch):
+            flush()
+        pending.append(pos)
+        pending_box = box
+        n_done += 1
+    flush()
+    torch.cuda.synchronize(dev)
+    rows = eng.rows()
+    with open(out_path, "w") as fh:
+        fh.write(format_rows(rows))
+    n_frames = len(range(begin, end + 1, stride))
+    log(f"Done {n_frames} Steps")
+    log(f"Results written to {out_path}")
+    total_ms = (time.perf_counter() - t_start) * 1e3 / max(n_frames, 1)
+    cuda_ms = gpu_ms / max(n_done, 1)
+    log(timing_banner(cuda_ms, total_ms - cuda_ms, total_ms))
+    eng.close()
+    return rows
+
+
+def banner(n, nn, order, sigma, dq, qcut, water_model) -> str:
+    """saxsKernel::writeBanner (saxsKernel.cu:527-544)."""
+    return ("*************************************************\n"
+            f"* {'Running CuSAXS':^45} *\n"
+            f"* {'Cell Grid':<19} {n[0]:>4} * {n[1]:>4} * {n[2]:>4}        *\n"
+            f"* {'Supercell Grid':<19} {nn[0]:>4} * {nn[1]:>4} * {nn[2]:>4}        *\n"
+            f"* {'Order':<10} {order:>4}      {'Sigma':<10} {sigma:>4.3g}          *\n"
+            f"* {'Bin Size':<10} {dq:>4.3f}     {'Q Cutoff ':<10}  {qcut:>3.1f}          *\n"
+            f"* {'Water Model':<19} {water_model:<25} *\n"
+            "*************************************************\n")
+
+
+def timing_banner(cuda_ms, read_ms, total_ms) -> str:
+    """RunSaxs.cu:253-263."""
+    return ("\n=========================================================\n"
+            "=                                                       =\n"
+            "=                    CuSAXS Timing                      =\n"
+            "=                                                       =\n"
+            f"=           CUDA Time:     {cuda_ms:<10.2f} ms/per step       =\n"
+            f"=           Read Time:     {read_ms:<10.2f} ms/per step       =\n"
+            f"=           Total Time:    {total_ms:<10.2f} ms/per step       =\n"
+            "=                                                       =\n"
+            "=========================================================\n")

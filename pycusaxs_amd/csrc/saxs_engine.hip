@@ -331,6 +331,8 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   if (d->lut_len < 1 || !d->bin_lut || !d->ff_lut || !d->mod_x || !d->mod_y || !d->mod_z)
     return fail(SAXS_EINVAL, "tables missing");
   if (d->max_atoms < 0 || d->max_frames < 1) return fail(SAXS_EINVAL, "capacity invalid");
+  if ((long long)d->max_frames * d->num_elements > 65535)
+    return fail(SAXS_EINVAL, "max_frames * num_elements must not exceed 65535");
 
   saxs_plan* pl = new (std::nothrow) saxs_plan();
   if (!pl) return fail(SAXS_ENOMEM, "host allocation failed");

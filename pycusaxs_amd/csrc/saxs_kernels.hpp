@@ -110,6 +110,13 @@ __device__ __forceinline__ void bspline_d(float w, float* dth) {
   dth[P - 1] = t[P - 2];
 }
 
+// Cell index m = int(r), clamped to [0, n] so that non-finite coordinates
+// cannot index outside the grid (finite inputs always give 0 <= r <= n).
+__device__ __forceinline__ int cell_index(float r, int n) {
+  const int m = (int)r;
+  return m < 0 ? 0 : (m > n ? n : m);
+}
+
 __device__ __forceinline__ int wrap_base(int m, int P, int n) {
   int b = m - P;
   return b < 0 ? b + n : b;
@@ -130,15 +137,16 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0.0;
   __syncthreads();
-  if (i < p.N) {
+  const int e = i < p.N ? b.elem[i] : -1;
+  if (i < p.N && (e < 0 || e >= p.E)) b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
+  if (i < p.N && e >= 0 && e < p.E) {
     const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
-    const int e = b.elem[i];
     int base[3];
     float w[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float r = cell_coord(x[a], p.oc[a], p.n[a]);
-      const int m = (int)r;
+      const int m = cell_index(r, p.n[a]);
       w[a] = r - (float)m;
       base[a] = wrap_base(m, P, p.n[a]);
     }
@@ -202,6 +210,7 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   if (i >= p.N) return;
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
   const int2 ks = b.keyslot[(size_t)f * p.N + i];
+  if (ks.x < 0) return;
   float4 rec;
   rec.x = cell_coord(x[0], p.oc[0], p.n[0]);
   rec.y = cell_coord(x[1], p.oc[1], p.n[1]);
@@ -262,7 +271,7 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
     const int col = local - a * (P * P);
     const int o = col / P, o2 = col - o * P;
     const float4 rec = b.sorted[s_start[s] + a];
-    const int mx = (int)rec.x, my = (int)rec.y;
+    const int mx = cell_index(rec.x, p.n[0]), my = cell_index(rec.y, p.n[1]);
     int gx = mx - P + o;
     if (gx < 0) gx += p.n[0];
     const int lx = gx - X0;
@@ -276,7 +285,7 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
     const float fo = th[o];
     bspline<P>(rec.y - (float)my, th);
     const float fp = fo * th[o2];
-    const int mz = (int)rec.z;
+    const int mz = cell_index(rec.z, p.n[2]);
     bspline<P>(rec.z - (float)mz, tz);
     float* colp = rho + (lx * p.ty + ly) * nz;
 #pragma unroll
@@ -429,7 +438,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
       const int nsq = mx * mx + my * my + kz * kz;
       if (nsq < p.lut_len) {
         const int bin = b.lut_bin[nsq];
-        if (bin >= 0) {
+        if (bin >= 0 && bin < nb) {
           const float mod = b.modx[ix] * b.mody[iy] * b.modz[kz];
           const float I = (A[u].x * A[u].x + A[u].y * A[u].y) * mod;
           const bool origin = (ix == 0) && (iy == 0) && (kz == 0);

@@ -1,0 +1,236 @@
+"""Python handle on one engine plan: a system's geometry, its shell tables and
+the device workspace, driven through the C ABI (include/saxs_engine.h).
+
+This is the per-run object the reference builds in RunSaxs::Run
+(RunSaxs.cu:117-133: saxsKernel construction, scaledCell, resetHistogram-
+Parameters, createMemory, cuFFT plan).  Device buffers are torch tensors;
+launches go to torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_table,
+                       hist_params, inverse_box_diag, shell_bins)
+
+
+@dataclass
+class SystemSpec:
+    """Everything that fixes one plan."""
+    n: Tuple[int, int, int]
+    nn: Tuple[int, int, int]
+    order: int
+    symbols: Tuple[str, ...]
+    sigma: float
+    dq_req: float
+    qcut_req: float
+    dq_policy: str = "reference"
+
+
+def resolve_grid(n: Sequence[int], sigma: float, scaled: Optional[Sequence[int]] = None):
+    n = tuple(int(v) for v in n)
+    if scaled:
+        nn = tuple(int(v) for v in scaled)
+    else:
+        nn = tuple(fft_size(v, sigma) for v in n)
+    return n, nn
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _stream_handle():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class SaxsEngine:
+    """One plan of the MI355X engine for a fixed system and first-frame box.
+
+    forward(xyz, elem) accumulates the S6 histogram of a batch of frames into
+    the engine's device accumulators H, C (float64), like saxsKernel's
+    d_histogram / d_nhist (saxsKernel.cu:248-254)."""
+
+    def __init__(self, spec: SystemSpec, box, max_atoms: int, max_frames: int = 1,
+                 tile: Tuple[int, int] = (0, 0), device: Optional[torch.device] = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the MI355X SAXS engine needs a ROCm GPU (torch.cuda is unavailable)")
+        self.lib = _lib.load()
+        self.spec = spec
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.max_atoms = int(max_atoms)
+        self.max_frames = int(max_frames)
+        self.oc = inverse_box_diag(box)
+        self.hist: Histogram = hist_params(spec.n, spec.nn, self.oc, spec.sigma, spec.dq_req,
+                                           spec.qcut_req, spec.dq_policy)
+        self._check_isotropic(self.hist)
+        self.border = border_planes(spec.n[2])
+        self.moduli = [bspline_moduli(spec.nn[a], spec.order) for a in range(3)]
+        bins, ff = self._shell_tables(self.hist, self.oc)
+        self._box_key = tuple(float(v) for v in self.oc)
+        mods = [np.ascontiguousarray(m, dtype=np.float32) for m in self.moduli]
+        desc = _lib.SaxsDesc()
+        desc.n[:] = list(spec.n)
+        desc.nn[:] = list(spec.nn)
+        desc.order = spec.order
+        desc.num_elements = len(spec.symbols)
+        desc.num_bins = self.hist.num_bins
+        desc.border = self.border
+        desc.lut_len = len(bins)
+        desc.bin_lut = bins.ctypes.data_as(C.POINTER(C.c_int32))
+        desc.ff_lut = ff.ctypes.data_as(C.POINTER(C.c_float))
+        desc.mod_x = mods[0].ctypes.data_as(C.POINTER(C.c_float))
+        desc.mod_y = mods[1].ctypes.data_as(C.POINTER(C.c_float))
+        desc.mod_z = mods[2].ctypes.data_as(C.POINTER(C.c_float))
+        desc.max_atoms = self.max_atoms
+        desc.max_frames = self.max_frames
+        desc.tile_x, desc.tile_y = int(tile[0]), int(tile[1])
+        plan = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.saxs_plan_create(C.byref(desc), C.byref(plan)), "saxs_plan_create")
+        self._plan = plan
+        nb = self.hist.num_bins
+        self.H = torch.zeros(nb, dtype=torch.float64, device=self.device)
+        self.C = torch.zeros(nb, dtype=torch.float64, device=self.device)
+        self.frames_done = 0
+
+    # -- tables ---------------------------------------------------------
+    @staticmethod
+    def _check_isotropic(h: Histogram):
+        s = h.spacing
+        if not (s[0] == s[1] == s[2]):
+            raise ValueError("the engine needs equal reciprocal spacings on the three axes "
+                             f"(L_a*nn_a/n_a equal); got {s}")
+
+    def _shell_tables(self, hist: Histogram, oc):
+        bins = shell_bins(hist.spacing[0], hist.dq, hist.qcut, hist.num_bins)
+        ff = form_factor_table(self.spec.symbols, hist.spacing[0], len(bins)).astype(np.float32)
+        return np.ascontiguousarray(bins, dtype=np.int32), np.ascontiguousarray(ff)
+
+    def set_box(self, box):
+        """Rebuilds the shell tables for a new box (NPT).  dq, qcut and
+        num_bins stay those of the first frame, as in the reference."""
+        oc = inverse_box_diag(box)
+        key = tuple(float(v) for v in oc)
+        if key == self._box_key:
+            return oc
+        h = hist_params(self.spec.n, self.spec.nn, oc, self.spec.sigma, self.spec.dq_req,
+                        self.spec.qcut_req, self.spec.dq_policy)
+        h = Histogram(self.hist.dq, self.hist.qcut, self.hist.num_bins, h.spacing,
+                      self.hist.dq_reset, self.hist.qcut_reset)
+        self._check_isotropic(h)
+        bins, ff = self._shell_tables(h, oc)
+        torch.cuda.current_stream().synchronize()
+        _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
+                                                 bins.ctypes.data_as(C.c_void_p),
+                                                 ff.ctypes.data_as(C.c_void_p)),
+                   "saxs_plan_set_tables")
+        self._box_key = key
+        self.oc = oc
+        return oc
+
+    # -- compute --------------------------------------------------------
+    def _prep(self, xyz: torch.Tensor, elem: torch.Tensor):
+        if xyz.dim() == 2:
+            xyz = xyz.unsqueeze(0)
+        if xyz.dim() != 3 or xyz.shape[-1] != 3:
+            raise ValueError(f"positions must be [frames, atoms, 3], got {tuple(xyz.shape)}")
+        xyz = xyz.to(device=self.device, dtype=torch.float32).contiguous()
+        elem = elem.to(device=self.device, dtype=torch.int32).contiguous()
+        if elem.dim() != 1 or elem.shape[0] != xyz.shape[1]:
+            raise ValueError("element ids must be [atoms] matching the positions")
+        return xyz, elem
+
+    def forward(self, xyz: torch.Tensor, elem: torch.Tensor, box=None):
+        """Accumulates frames into self.H / self.C (asynchronous)."""
+        xyz, elem = self._prep(xyz, elem)
+        oc = self.set_box(box) if box is not None else self.oc
+        F = xyz.shape[0]
+        for f0 in range(0, F, self.max_frames):
+            chunk = xyz[f0:f0 + self.max_frames]
+            _lib.check(self.lib.saxs_forward(
+                self._plan, _ptr(chunk), _ptr(elem), int(chunk.shape[1]), int(chunk.shape[0]),
+                oc.ctypes.data_as(C.c_void_p), _ptr(self.H), _ptr(self.C), _stream_handle()),
+                "saxs_forward")
+        self.frames_done += F
+
+    def forward_frames(self, xyz: torch.Tensor, elem: torch.Tensor, box=None):
+        """Per-frame (H, C) as [frames, num_bins] float64 tensors."""
+        xyz, elem = self._prep(xyz, elem)
+        oc = self.set_box(box) if box is not None else self.oc
+        F, nb = xyz.shape[0], self.hist.num_bins
+        Hf = torch.empty(F, nb, dtype=torch.float64, device=self.device)
+        Cf = torch.empty(F, nb, dtype=torch.float64, device=self.device)
+        for f0 in range(0, F, self.max_frames):
+            chunk = xyz[f0:f0 + self.max_frames]
+            _lib.check(self.lib.saxs_forward_frames(
+                self._plan, _ptr(chunk), _ptr(elem), int(chunk.shape[1]), int(chunk.shape[0]),
+                oc.ctypes.data_as(C.c_void_p), _ptr(Hf[f0:]), _ptr(Cf[f0:]), _stream_handle()),
+                "saxs_forward_frames")
+        return Hf, Cf
+
+    def forward_plain(self, xyz: torch.Tensor, elem: torch.Tensor, box=None):
+        """Reference-structured per-frame path (one launch chain per frame)."""
+        xyz, elem = self._prep(xyz, elem)
+        oc = self.set_box(box) if box is not None else self.oc
+        for f in range(xyz.shape[0]):
+            _lib.check(self.lib.saxs_forward_plain(
+                self._plan, _ptr(xyz[f]), _ptr(elem), int(xyz.shape[1]),
+                oc.ctypes.data_as(C.c_void_p), _ptr(self.H), _ptr(self.C), _stream_handle()),
+                "saxs_forward_plain")
+        self.frames_done += xyz.shape[0]
+
+    def backward(self, xyz: torch.Tensor, elem: torch.Tensor, gH: torch.Tensor, box=None):
+        """dL/dx [frames, atoms, 3] float32 for per-frame dL/dH [frames, num_bins]."""
+        xyz, elem = self._prep(xyz, elem)
+        oc = self.set_box(box) if box is not None else self.oc
+        gH = gH.to(device=self.device, dtype=torch.float64).contiguous()
+        F = xyz.shape[0]
+        grad = torch.empty_like(xyz)
+        for f0 in range(0, F, self.max_frames):
+            chunk = xyz[f0:f0 + self.max_frames]
+            _lib.check(self.lib.saxs_backward(
+                self._plan, _ptr(chunk), _ptr(elem), int(chunk.shape[1]), int(chunk.shape[0]),
+                oc.ctypes.data_as(C.c_void_p), _ptr(gH[f0:]), _ptr(grad[f0:]),
+                _stream_handle()), "saxs_backward")
+        return grad
+
+    def reset(self):
+        self.H.zero_()
+        self.C.zero_()
+        self.frames_done = 0
+
+    def kept(self):
+        kx, ky, kz = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self.lib.saxs_plan_kept(self._plan, C.byref(kx), C.byref(ky), C.byref(kz)),
+                   "saxs_plan_kept")
+        return kx.value, ky.value, kz.value
+
+    def workspace_bytes(self) -> int:
+        return int(self.lib.saxs_plan_workspace_bytes(self._plan))
+
+    def rows(self):
+        """Output rows (b*dq, H_b/C_b) for b >= 1 with C_b != 0
+        (saxsKernel.cu:210-225); q is the float32 product b*dq."""
+        H = self.H.cpu().numpy()
+        Cn = self.C.cpu().numpy()
+        dq = np.float32(self.hist.dq)
+        return [(float(np.float32(np.float32(b) * dq)), float(H[b] / Cn[b]))
+                for b in range(1, len(H)) if Cn[b] != 0.0]
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            self.lib.saxs_plan_destroy(self._plan)
+            self._plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

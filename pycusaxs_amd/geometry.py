@@ -1,0 +1,183 @@
+"""Host-side setup of one system: FFT grid, histogram parameters, shell
+tables and B-spline moduli.  These are the per-run computations the
+reference does on the host before its frame loop; the results are handed to
+the engine's C ABI as plain arrays.
+
+  fft_size             saxsKernel.cu:377-469 (findClosestProduct)
+  border_planes        saxsKernel.h:107-110
+  inverse_box_diag     Cell.cpp:111-126
+  hist_params          saxsKernel.cu:489-526 and :247
+  shell_bins           exact bin of every shell n = |m|^2
+  bspline_moduli       BSpmod.cpp:45-211
+  form_factor_table    opsfact.cu:29-35 on the shell q values
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Sequence, Tuple
+
+import numpy as np
+
+from .formfactors import coefficients
+
+F32 = np.float32
+_SHELL = F32(0.07)            # Parameters.h:8
+_GAMMA_TERMS = 50             # BSpmod.h:22
+_TINY = 1.0e-7                # BSpmod.cpp:104
+
+
+def _smooth_numbers(limit: int):
+    vals = []
+    p2 = 1
+    while p2 <= limit:
+        p3 = p2
+        while p3 <= limit:
+            p5 = p3
+            while p5 <= limit:
+                p7 = p5
+                while p7 <= limit:
+                    vals.append(p7)
+                    p7 *= 7
+                p5 *= 5
+            p3 *= 3
+        p2 *= 2
+    return sorted(vals)
+
+
+def fft_size(n: int, sigma: float) -> int:
+    """2,3,5,7-smooth size closest to round(n*sigma) (float32 product, halves
+    rounded away from zero); on ties the smaller size wins."""
+    t = float(F32(n) * F32(sigma))
+    target = int(math.floor(abs(t) + 0.5)) * (1 if t >= 0 else -1)
+    best, best_d = target, None
+    for v in _smooth_numbers(2 * target):
+        d = abs(v - target)
+        if best_d is None or d < best_d:
+            best, best_d = v, d
+    return best
+
+
+def border_planes(nz: int) -> int:
+    return int(F32(F32(_SHELL * F32(nz)) / F32(2.0)))
+
+
+def inverse_box_diag(box) -> np.ndarray:
+    """oc diagonal as float32(1/L_a); raises for a triclinic box."""
+    b = np.asarray(box, dtype=np.float64).reshape(3, 3)
+    off = b - np.diag(np.diag(b))
+    if np.any(off != 0.0):
+        raise ValueError("triclinic boxes are not supported: the box must be diagonal "
+                         "(orthorhombic)")
+    d = np.asarray(np.diag(b), dtype=np.float32)
+    if np.any(~np.isfinite(d)) or np.any(d <= 0):
+        raise ValueError(f"box lengths must be positive, got {d.tolist()}")
+    return np.array([F32(1.0 / float(x)) for x in d], dtype=np.float32)
+
+
+@dataclass(frozen=True)
+class Histogram:
+    dq: float                 # float32 value
+    qcut: float               # float32 value
+    num_bins: int
+    spacing: Tuple[float, float, float]   # float32 reciprocal spacing per axis
+    dq_reset: bool
+    qcut_reset: bool
+
+
+def hist_params(n: Sequence[int], nn: Sequence[int], oc_diag, sigma: float, dq_req: float,
+                qcut_req: float, dq_policy: str = "reference") -> Histogram:
+    """Bin width and cutoff after the reference's reset: dq is raised to the
+    coarsest grid spacing 2*pi*oc/sigma, qcut lowered to the largest grid
+    frequency (float32 arithmetic throughout)."""
+    tp = F32(F32(2.0) * F32(math.pi))
+    sig = F32(sigma)
+    arg = [F32(F32(tp * F32(oc_diag[a])) / sig) for a in range(3)]
+    nf = [v // 2 if v % 2 == 0 else v // 2 + 1 for v in nn]
+    cut = [F32(arg[a] * F32(nf[a] - 1)) for a in range(3)]
+    if dq_policy == "reference":
+        dq = max(arg + [F32(dq_req)])
+    elif dq_policy == "exact":
+        if not dq_req > 0:
+            raise ValueError("dq_policy='exact' needs a positive bin_size")
+        dq = F32(dq_req)
+    else:
+        raise ValueError(f"unknown dq_policy {dq_policy!r}")
+    qcut = min(cut + [F32(qcut_req)])
+    num_bins = int(F32(qcut / dq)) + 1
+    spacing = tuple(float(F32(F32(tp * F32(oc_diag[a])) / F32(F32(nn[a]) / F32(n[a]))))
+                    for a in range(3))
+    return Histogram(float(dq), float(qcut), num_bins, spacing,
+                     bool(F32(dq) != F32(dq_req)), bool(F32(qcut) != F32(qcut_req)))
+
+
+def shell_bins(spacing: float, dq: float, qcut: float, num_bins: int) -> np.ndarray:
+    """bin[n] for shells n = 0..n_max, where n_max is the largest n with
+    n*s^2 <= qcut^2 and bin[n] = max{b : b^2 dq^2 <= n s^2} (or -1 when it
+    reaches num_bins).  Exact: the float values are expanded to integer
+    ratios and compared in integer arithmetic."""
+    sn, sd = float(spacing).as_integer_ratio()
+    dn, dd = float(dq).as_integer_ratio()
+    qn, qd = float(qcut).as_integer_ratio()
+    if sn <= 0 or dn <= 0 or qn < 0:
+        return np.full(1, -1 if qn < 0 else 0, dtype=np.int32)
+    n_max = (qn * qn * sd * sd) // (sn * sn * qd * qd)
+    if n_max > 50_000_000:
+        raise ValueError("q range too large for the shell tables")
+    num = sn * sn * dd * dd          # n * s^2 / dq^2 = n * num / den
+    den = dn * dn * sd * sd
+    out = np.empty(n_max + 1, dtype=np.int32)
+    for v in range(n_max + 1):
+        b = math.isqrt((v * num) // den)
+        out[v] = b if b < num_bins else -1
+    return out
+
+
+def bspline_moduli(N: int, order: int) -> np.ndarray:
+    """lambda(k)^2 / |b(k)|^2 for k = 0..N-1, float64."""
+    w0 = _bspline_at_zero(order)
+    A = np.zeros(order + 1)
+    A[1:] = w0
+    k = np.arange(N, dtype=np.float64)[:, None]
+    t = np.arange(order + 1, dtype=np.float64)[None, :]
+    ang = 2.0 * np.pi * k * t / N
+    b2 = (A * np.cos(ang)).sum(1) ** 2 + (A * np.sin(ang)).sum(1) ** 2
+    for i in range(1, N - 1):
+        if not b2[i] > _TINY:
+            b2[i] = 0.5 * (b2[i - 1] + b2[i + 1])
+    m = np.arange(N)
+    m = np.where(m < N // 2, m, m - N).astype(np.float64)
+    x = np.pi * m / N
+    j = np.arange(1, _GAMMA_TERMS + 1, dtype=np.float64)[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        xp = x[:, None] / (x[:, None] + np.pi * j)
+        xm = x[:, None] / (x[:, None] - np.pi * j)
+        g1 = 1.0 + (xp ** order + xm ** order).sum(1)
+        g2 = 1.0 + (xp ** (2 * order) + xm ** (2 * order)).sum(1)
+        lam = np.where(m == 0, 1.0, g1 / g2)
+    return lam * lam / b2
+
+
+def _bspline_at_zero(order: int) -> np.ndarray:
+    """M_p(p-1), ..., M_p(0): the weights of an atom on a grid node."""
+    th = np.zeros(order)
+    th[0], th[1] = 1.0, 0.0
+    for k in range(3, order + 1):
+        div = 1.0 / (k - 1)
+        th[k - 1] = 0.0
+        for j in range(1, k - 1):
+            th[k - j - 1] = div * (j * th[k - j - 2] + (k - j) * th[k - j - 1])
+        th[0] = div * th[0]
+    return th
+
+
+def form_factor_table(symbols: Sequence[str], spacing: float, n_shells: int) -> np.ndarray:
+    """f_e(q_n), q_n = s*sqrt(n), for every element and shell (float64)."""
+    q = float(spacing) * np.sqrt(np.arange(n_shells, dtype=np.float64))
+    s2 = (q / (4.0 * np.pi)) ** 2
+    out = np.empty((len(symbols), n_shells))
+    for e, sym in enumerate(symbols):
+        a1, a2, a3, a4, b1, b2, b3, b4, c = coefficients(sym)
+        out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
+                  + a4 * np.exp(-b4 * s2) + c)
+    return out

@@ -1,0 +1,155 @@
+"""CPU checks of the engine's C ABI library and host-side tables.
+
+The library must load and export every symbol include/saxs_engine.h
+declares (no compute call is made without a GPU).  The host tables the
+engine hands to the GPU (FFT sizes, exact shell bins, moduli, form factors)
+must match the oracle and, where it is built, the reference's own helpers
+compiled in oracle/_ref."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import saxs_oracle as O
+from pycusaxs_amd import _lib, formfactors
+from pycusaxs_amd import geometry as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+HEADER = os.path.join(ROOT, "include", "saxs_engine.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|const char\*)\s+(saxs_\w+)\(",
+                                 txt, flags=re.M)))
+
+
+def test_header_and_binding_agree():
+    assert set(declared_symbols()) == set(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    L = _lib.load()
+    assert L.saxs_abi_version() == _lib.ABI_VERSION
+    assert L.saxs_last_error() == b""
+
+
+def test_plan_create_rejects_bad_desc_without_gpu():
+    L = _lib.load()
+    d = _lib.SaxsDesc()
+    d.n[:] = [32, 32, 32]
+    d.nn[:] = [64, 64, 64]
+    d.order = 5
+    plan = ctypes.c_void_p()
+    rc = L.saxs_plan_create(ctypes.byref(d), ctypes.byref(plan))
+    assert rc == _lib.SAXS_EUNSUP and b"order" in L.saxs_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc, "saxs_plan_create")
+
+
+@pytest.mark.parametrize("n,sigma", [(64, 2.0), (128, 2.0), (27, 2.0), (21, 2.5), (37, 2.0),
+                                     (100, 2.5), (50, 1.7), (192, 2.0)])
+def test_fft_size_matches_oracle(n, sigma):
+    assert G.fft_size(n, sigma) == O.find_closest_product(n, sigma)
+
+
+@pytest.mark.parametrize("L,n,dq,qcut,policy", [(37.0, 64, 0.01, 0.5, "reference"),
+                                                (100.0, 128, 0.005, 0.6, "reference"),
+                                                (80.0, 128, 0.005, 0.5, "exact"),
+                                                (25.0, 21, 0.0, 100.0, "reference"),
+                                                (46.4, 75, 0.02, 0.9, "reference")])
+def test_shell_bins_match_oracle(L, n, dq, qcut, policy):
+    oc = G.inverse_box_diag(np.diag([L] * 3))
+    nn = (G.fft_size(n, 2.0),) * 3
+    h = G.hist_params((n,) * 3, nn, oc, 2.0, dq, qcut, policy)
+    ho = O.hist_params((n,) * 3, nn, oc, 2.0, dq, qcut, policy)
+    assert (np.float32(h.dq), np.float32(h.qcut), h.num_bins) == (ho.dq, ho.qcut, ho.num_bins)
+    b = G.shell_bins(h.spacing[0], h.dq, h.qcut, h.num_bins)
+    bo = [O.bin_of_n(v, ho.spacing[0], ho.dq, ho.qcut, ho.num_bins) for v in range(len(b) + 5)]
+    assert list(b) == bo[:len(b)]
+    assert all(v == -1 for v in bo[len(b):])
+
+
+@pytest.mark.parametrize("N,p", [(128, 4), (256, 4), (135, 4), (64, 6), (140, 6)])
+def test_moduli_match_oracle(N, p):
+    assert np.allclose(G.bspline_moduli(N, p), O.bsp_moduli(N, p), rtol=1e-13)
+
+
+def test_form_factor_table_matches_oracle():
+    syms = list(O.FORM_FACTORS)
+    ff = G.form_factor_table(syms, 0.05, 400)
+    q = 0.05 * np.sqrt(np.arange(400))
+    for e, s in enumerate(syms):
+        assert np.allclose(ff[e], O.form_factor(s, q), rtol=1e-14, atol=1e-14)
+
+
+def test_unknown_element_raises():
+    with pytest.raises(ValueError):
+        formfactors.coefficients("Xx")
+
+
+def test_triclinic_box_rejected():
+    with pytest.raises(ValueError):
+        G.inverse_box_diag([[30, 0, 0], [5, 30, 0], [0, 0, 30]])
+
+
+# ---- against the reference's own helpers (compiled from its sources) ----
+
+needs_ref = pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="oracle/_ref not built")
+
+
+def ref(*args):
+    out = subprocess.run([REF_DRIVER, *map(str, args)], capture_output=True, text=True, check=True)
+    return out.stdout.strip().splitlines()
+
+
+@needs_ref
+def test_form_factor_data_matches_reference_table():
+    syms = list(formfactors.table())
+    for chunk in range(0, len(syms), 40):
+        part = syms[chunk:chunk + 40]
+        lines = ref("scatter", *part)
+        for s, line in zip(part, lines):
+            vals = np.array([float(v) for v in line.split()])
+            assert np.array_equal(vals.astype(np.float32),
+                                  np.array(formfactors.coefficients(s), dtype=np.float32)), s
+            assert np.array_equal(vals.astype(np.float32),
+                                  np.array(O.FORM_FACTORS.get(s, formfactors.coefficients(s)),
+                                           dtype=np.float32)), s
+
+
+@needs_ref
+@pytest.mark.parametrize("p", [4, 6])
+def test_bspline_weights_match_reference(p):
+    ws = [0.0, 0.125, 0.3, 0.5, 0.77, 0.999]
+    for w, line in zip(ws, ref("bspline", p, *ws)):
+        x, dx = line.split("|")
+        x = np.array([float(v) for v in x.split()])
+        dx = np.array([float(v) for v in dx.split()])
+        w32 = float(np.float32(w))
+        assert np.allclose(x, O.bspline_theta(w32, p), atol=3e-7)
+        assert np.allclose(dx, O.bspline_dtheta(w32, p), atol=3e-7)
+
+
+@needs_ref
+@pytest.mark.parametrize("N", [64, 128, 256, 135])
+def test_moduli_match_reference(N):
+    lines = ref("bspmod", 4, N, N, N)
+    mod = np.array([float(v) for v in lines[0].split()])
+    assert np.allclose(mod, O.bsp_moduli(N, 4), rtol=2e-5)
+    assert np.allclose(mod, G.bspline_moduli(N, 4), rtol=2e-5)
+
+
+@needs_ref
+def test_cell_inverse_matches_reference():
+    line = ref("cell", 37, 0, 0, 0, 41.5, 0, 0, 0, 100)[0]
+    oc = np.array([float(v) for v in line.split()]).reshape(3, 3)
+    assert np.array_equal(np.diag(oc).astype(np.float32), G.inverse_box_diag(np.diag([37, 41.5, 100])))
+    assert np.array_equal(np.diag(oc).astype(np.float32), O.inverse_box(np.diag([37, 41.5, 100])))

@@ -1,0 +1,244 @@
+"""Pins the FP64 oracle against facts the mathematics fixes independently of
+its own formulas: closed forms, brute-force sums, known shell counts."""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import saxs_oracle as O
+
+
+def cardinal_bspline(u, p):
+    """M_p(u) by the truncated-power closed form
+    M_p(u) = 1/(p-1)! sum_k (-1)^k C(p,k) (u-k)_+^(p-1)."""
+    s = 0.0
+    for k in range(p + 1):
+        t = u - k
+        if t > 0:
+            s += (-1) ** k * math.comb(p, k) * t ** (p - 1)
+    return s / math.factorial(p - 1)
+
+
+@pytest.mark.parametrize("p", [4, 6])
+def test_bspline_weights_closed_form(p):
+    for w in np.linspace(0.0, 1.0, 17):
+        th = O.bspline_theta(w, p)
+        ref = [cardinal_bspline(w + p - 1 - k, p) for k in range(p)]
+        assert np.allclose(th, ref, atol=1e-14, rtol=0)
+        assert abs(th.sum() - 1.0) < 1e-14             # partition of unity
+
+
+@pytest.mark.parametrize("p", [4, 6])
+def test_bspline_derivative_identity(p):
+    h = 1e-6
+    for w in np.linspace(0.05, 0.95, 9):
+        d = O.bspline_dtheta(w, p)
+        fd = (O.bspline_theta(w + h, p) - O.bspline_theta(w - h, p)) / (2 * h)
+        assert np.allclose(d, fd, atol=1e-8)
+        assert abs(d.sum()) < 1e-14
+
+
+def test_moduli_closed_form_order4():
+    N = 128
+    k = np.arange(N)
+    b2 = (2.0 / 3.0 + np.cos(2 * np.pi * k / N) / 3.0) ** 2
+    mod = O.bsp_moduli(N, 4)
+    lam2 = mod * b2
+    # lambda(0) = 1; lambda from the finite series G_4/G_8 evaluated with mpmath-free
+    # direct sums at a few points
+    assert abs(lam2[0] - 1.0) < 1e-15
+    for kk in (1, 7, 40, 100):
+        m = kk if kk < N // 2 else kk - N
+        x = math.pi * m / N
+        g = lambda r: 1 + sum((x / (x + math.pi * j)) ** r + (x / (x - math.pi * j)) ** r
+                              for j in range(1, 51))
+        lam = g(4) / g(8)
+        assert abs(lam2[kk] - lam * lam) < 1e-13
+
+
+def test_moduli_closed_form_order6():
+    N = 64
+    k = np.arange(N)
+    t = 2 * np.pi * k / N
+    b2 = ((66 + 52 * np.cos(t) + 2 * np.cos(2 * t)) / 120.0) ** 2
+    mod = O.bsp_moduli(N, 6)
+    assert abs(mod[0] * b2[0] - 1.0) < 1e-15
+    # mod * |b|^2 = lambda^2 with lambda = G_6/G_12 >= 1 (even orders)
+    lam2 = mod * b2
+    assert np.all(lam2 >= 1.0 - 1e-15)
+    for kk in (1, 9, 31, 50):
+        m = kk if kk < N // 2 else kk - N
+        x = math.pi * m / N
+        g = lambda r: 1 + sum((x / (x + math.pi * j)) ** r + (x / (x - math.pi * j)) ** r
+                              for j in range(1, 51))
+        assert abs(lam2[kk] - (g(6) / g(12)) ** 2) < 1e-13
+
+
+def test_rfftn_matches_bruteforce_dft():
+    rng = np.random.default_rng(0)
+    n = 6
+    x = rng.normal(size=(n, n, n))
+    F = np.fft.rfftn(x)
+    g = np.arange(n)
+    for (a, b, c) in [(0, 0, 0), (1, 2, 3), (5, 1, 2), (3, 3, 0)]:
+        ph = np.exp(-2j * np.pi * (a * g[:, None, None] + b * g[None, :, None]
+                                   + c * g[None, None, :]) / n)
+        assert abs((x * ph).sum() - F[a, b, c]) < 1e-10
+
+
+def test_spread_single_atom_stencil_and_mass():
+    n = (12, 12, 12)
+    oc = np.array([1 / 24.0] * 3, dtype=np.float32)
+    pos = np.array([[5.3, 13.9, 0.4]], dtype=np.float32)
+    g = O.spread(pos, oc, n, 4)
+    assert abs(g.sum() - 1.0) < 1e-14
+    m, w = O.grid_coords(pos, oc, n)
+    th = [O.bspline_theta(float(w[0, a]), 4) for a in range(3)]
+    for o, p2, q in itertools.product(range(4), repeat=3):
+        idx = [(m[0, a] - 4 + (o, p2, q)[a]) % n[a] for a in range(3)]
+        assert abs(g[tuple(idx)] - th[0][o] * th[1][p2] * th[2][q]) < 1e-15
+    assert np.count_nonzero(g) == 64
+
+
+def test_spread_wraps_only_negative_indices():
+    # an atom at u just below 1 puts m = n, stencil n-4..n-1, no wrap
+    n = (16, 16, 16)
+    oc = np.array([1 / 16.0] * 3, dtype=np.float32)
+    pos = np.array([[15.99, 0.01, 8.0]], dtype=np.float32)
+    m, w = O.grid_coords(pos, oc, n)
+    assert m[0, 0] == 15 and m[0, 1] == 0
+    g = O.spread(pos, oc, n, 4)
+    assert g[12:16, :, :].sum() > 0.999
+
+
+def test_on_node_atoms_direct_structure_factor():
+    """Atoms on grid nodes with pad 0 (no density in the border slabs): the
+    pipeline must reproduce |sum_j f_j e^{-2 pi i m.g_j/nn}|^2 prod lambda^2."""
+    L = 32.0
+    n = 16
+    box = np.diag([L] * 3)
+    plan = O.make_plan((n,) * 3, box, 4, 2.0, None, 0.0, 100.0)
+    d = plan.border                          # z <= d or z >= n-d is border
+    rng = np.random.default_rng(5)
+    # nodes whose stencils (g-4 .. g-1) neither wrap around the cell (x, y >= 4)
+    # nor reach the border slabs (z >= d + 5), so pad = 0 and the spread is a
+    # pure translation of theta(0) per atom
+    zs = rng.integers(d + 5, n - d, size=6)
+    nodes = np.stack([rng.integers(4, n, 6), rng.integers(4, n, 6), zs], axis=1)
+    pos = (nodes * (L / n)).astype(np.float32)
+    syms = ["C", "O", "C", "N", "O", "C"]
+    symbols, ids = O.element_ids(syms)
+    I, inr, bins = O.frame_intensity(plan, pos, ids, symbols, box)
+    nn = plan.nn[0]
+    mx, my, mz = O.half_spectrum_m(plan.nn)
+    # on a node the stencil is theta(0) = (1/6, 4/6, 1/6, 0) at offsets -4..-1,
+    # i.e. the atom's weight sits at g-3, g-2, g-1: the grid-node structure
+    # factor times b(m)^* per axis; |.|^2 * mod gives lambda^2 exactly
+    ii, jj, kk = np.nonzero(inr)
+    for t in range(0, len(ii), max(1, len(ii) // 60)):
+        i, j, k = ii[t], jj[t], kk[t]
+        m = np.array([mx[i], my[j], mz[k]])
+        q = float(plan.hp.spacing[0]) * math.sqrt(float(m @ m))
+        A = 0j
+        for a, s in zip(nodes, syms):
+            A += O.form_factor(s, q) * np.exp(-2j * np.pi * (m @ a) / nn)
+        lam2 = 1.0
+        for ax in range(3):
+            x = math.pi * m[ax] / nn
+            if m[ax] != 0:
+                g = lambda r: 1 + sum((x / (x + math.pi * jj_)) ** r + (x / (x - math.pi * jj_)) ** r
+                                      for jj_ in range(1, 51))
+                lam2 *= (g(4) / g(8)) ** 2
+        expect = abs(A) ** 2 * lam2
+        assert abs(I[i, j, k] - expect) <= 1e-9 * max(1.0, expect), (m, I[i, j, k], expect)
+
+
+def _count_reference_loop(n_cell, nn, L, qcut, dq):
+    """Bin counts by enumerating the reference's histogram loop over the half
+    spectrum with exact arithmetic (counts 2 per point, 1 at the origin)."""
+    plan = O.make_plan((n_cell,) * 3, np.diag([L] * 3), 4, 2.0, None, dq, qcut)
+    s = Fraction(float(plan.hp.spacing[0]))
+    D = Fraction(float(plan.hp.dq))
+    Q = Fraction(float(plan.hp.qcut))
+    C = [0] * plan.hp.num_bins
+    M = int(float(Q / s)) + 1
+    for i in range(-M, M + 1):
+        for j in range(-M, M + 1):
+            for k in range(0, M + 1):
+                n = i * i + j * j + k * k
+                q2 = n * s * s
+                if q2 > Q * Q:
+                    continue
+                b = 0
+                while (b + 1) ** 2 * D * D <= q2:
+                    b += 1
+                if b >= plan.hp.num_bins:
+                    continue
+                C[b] += 1 if n == 0 else 2
+    return C, plan
+
+
+def test_known_shell_counts_c1():
+    C, plan = _count_reference_loop(64, 128, 37.0, 0.5, 0.01)
+    assert plan.nn == (128, 128, 128)
+    assert plan.hp.num_bins == 6
+    assert C[1:] == [34, 82, 178, 258, 402]
+
+
+def test_known_shell_counts_c2():
+    C, plan = _count_reference_loop(128, 256, 100.0, 0.6, 0.005)
+    assert plan.hp.num_bins == 20
+    assert C[1:] == [34, 82, 178, 258, 450, 506, 786, 922, 1226, 1426, 1690, 2050, 2458,
+                     2626, 3170, 3402, 4042, 4362, 506]
+
+
+def test_shell_counts_are_sums_of_squares():
+    """C_b = sum_{n in b} r3(n) + r2(n) (the k_z = 0 plane counted twice)."""
+    def r(dim, n):
+        M = math.isqrt(n)
+        return sum(1 for v in itertools.product(range(-M, M + 1), repeat=dim)
+                   if sum(x * x for x in v) == n)
+    C, plan = _count_reference_loop(64, 128, 37.0, 0.5, 0.01)
+    for b in range(1, 6):
+        # shells of bin b, cut at q <= qcut (n s^2 <= 0.25: n <= 34 here)
+        ns = [n for n in range(0, 40) if math.isqrt(n) == b and n <= 34]
+        assert C[b] == sum(r(3, n) + r(2, n) for n in ns)
+
+
+def test_bin_rule_matches_oracle_histogram_counts():
+    pos, syms = O.synthetic_system(500, 37.0, seed=1)
+    symbols, ids = O.element_ids(syms)
+    plan = O.make_plan((64,) * 3, np.diag([37.0] * 3), 4, 2.0, None, 0.01, 0.5)
+    H, C = O.frame_histogram(plan, pos, ids, symbols, np.diag([37.0] * 3))
+    assert list(C[1:].astype(int)) == [34, 82, 178, 258, 402]
+
+
+def test_pad_subtraction_equivalence():
+    """Literal pad embedding and (rho - pad) zero padding agree for m != 0."""
+    pos, syms = O.synthetic_system(800, 24.0, seed=2)
+    symbols, ids = O.element_ids(syms)
+    box = np.diag([24.0] * 3)
+    plan = O.make_plan((16,) * 3, box, 4, 2.0, None, 0.0, 100.0)
+    H1, C1 = O.frame_histogram(plan, pos, ids, symbols, box, literal=True)
+    H2, C2 = O.frame_histogram(plan, pos, ids, symbols, box, literal=False)
+    assert np.array_equal(C1, C2)
+    assert np.allclose(H1[1:], H2[1:], rtol=1e-12)
+
+
+def test_find_closest_product():
+    assert O.find_closest_product(64, 2.0) == 128
+    assert O.find_closest_product(128, 2.5) == 320
+    assert O.find_closest_product(37, 2.0) == 75          # 74 -> 75 (|1|) beats 72 (|2|)
+    assert O.find_closest_product(13, 1.0) == 12          # tie 12/14 -> smaller
+    for v in O.generate_multiples(500):
+        m = v
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        assert m == 1
+
+
+def test_border_planes():
+    assert [O.border_planes(n) for n in (64, 128, 256)] == [2, 4, 8]
