@@ -127,6 +127,31 @@ int saxs_forward_plain(saxs_plan* plan, const float* xyz, const int32_t* elem,
                        int32_t n_atoms, const float* oc_diag, double* H, double* C,
                        void* stream);
 
+/* Kernel ids of the measurement hook. */
+#define SAXS_K_CLEAR 0
+#define SAXS_K_PREP 1
+#define SAXS_K_SCAN 2
+#define SAXS_K_SCATTER 3
+#define SAXS_K_SPREAD_Z 4
+#define SAXS_K_YFFT 5
+#define SAXS_K_XFFT 6
+#define SAXS_K_REDUCE 7
+#define SAXS_K_ADJ_X 8
+#define SAXS_K_ADJ_Y 9
+#define SAXS_K_ADJ_Z 10
+#define SAXS_K_GATHER 11
+#define SAXS_K_PLAIN 12
+#define SAXS_NKERNELS 13
+
+/* Measurement hook (no reference counterpart; the reference times whole
+ * frames with cudaEvents, saxsKernel.cu:64-67,197-208).  When enabled,
+ * every launch of the plan is bracketed by a pair of hipEvents on its
+ * stream.  saxs_plan_kernel_times waits for the recorded events, adds the
+ * elapsed milliseconds and launch counts per kernel id into ms[] and
+ * launches[] (SAXS_NKERNELS entries each) and forgets the events. */
+int saxs_plan_set_timing(saxs_plan* plan, int32_t enable);
+int saxs_plan_kernel_times(saxs_plan* plan, double* ms, int64_t* launches);
+
 /* Workspace bytes held by the plan (device memory). */
 int64_t saxs_plan_workspace_bytes(const saxs_plan* plan);
 

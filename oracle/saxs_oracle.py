@@ -49,6 +49,22 @@ import numpy as np
 
 F32 = np.float32
 
+# FFT used by the oracle: numpy's pocketfft by default; the CPU baseline in
+# bench.py may switch it to scipy.fft with several worker threads.
+_rfftn = np.fft.rfftn
+
+
+def set_fft_workers(workers: int) -> int:
+    """Use scipy.fft.rfftn with this many threads (1 = numpy); returns the
+    number of threads in effect."""
+    global _rfftn
+    if workers <= 1:
+        _rfftn = np.fft.rfftn
+        return 1
+    import scipy.fft as sfft
+    _rfftn = lambda a: sfft.rfftn(a, workers=workers)  # noqa: E731
+    return workers
+
 # Cromer-Mann coefficients {a1..a4, b1..b4, c} for the species the tests use,
 # transcribed independently of the engine's data file from
 # Scattering.cpp:10 (Du), :11 (H), :20 (C), :22 (N), :23 (O), :30 (Na),
@@ -340,7 +356,7 @@ def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=
         else:
             sup = np.zeros(plan.nn, dtype=np.float64)
             sup[: plan.n[0], : plan.n[1], : plan.n[2]] = g - pad
-        Fs.append(np.fft.rfftn(sup))
+        Fs.append(_rfftn(sup))
     return Fs, inr, bins, q
 
 

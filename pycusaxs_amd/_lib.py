@@ -17,8 +17,11 @@ ABI_VERSION = 1
 EXPORTED = (
     "saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
     "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
-    "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version",
+    "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version", "saxs_plan_set_timing",
+    "saxs_plan_kernel_times",
 )
+KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
+                "adj_x", "adj_y", "adj_z", "gather", "plain")
 
 
 class SaxsDesc(C.Structure):
@@ -70,10 +73,13 @@ def load(path: Path = LIB_PATH):
     lib.saxs_plan_workspace_bytes.restype = i64
     lib.saxs_plan_kept.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     lib.saxs_plan_destroy.argtypes = [vp]
+    lib.saxs_plan_set_timing.argtypes = [vp, i32]
+    lib.saxs_plan_kernel_times.argtypes = [vp, vp, vp]
     lib.saxs_last_error.restype = C.c_char_p
     lib.saxs_abi_version.restype = i32
     for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
-                 "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy"):
+                 "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy",
+                 "saxs_plan_set_timing", "saxs_plan_kernel_times"):
         getattr(lib, name).restype = C.c_int
     if lib.saxs_abi_version() != ABI_VERSION:
         raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")

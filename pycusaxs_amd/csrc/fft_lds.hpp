@@ -174,25 +174,72 @@ __device__ __forceinline__ void fft_stage(float2* buf, int ls, int nlines, int N
   __syncthreads();
 }
 
-// Forward FFT of nlines lines of length d.N in LDS.  The caller must have
-// synchronised after filling buf; the routine ends synchronised.
-template <int MAXC>
+// First stage (Ns = 1, no twiddles) fed by a loader instead of LDS:
+// ld(line, t) returns input element t of a line.  Results go to buf in
+// place of the stage's output; the routine ends synchronised.  Reading from
+// a different buffer than it writes, it needs no intermediate barrier.
+template <int R, class Load>
+__device__ __forceinline__ void fft_stage0(float2* buf, int ls, int nlines, int N, Load ld) {
+  const unsigned nb = (unsigned)(N / R);
+  const unsigned total = (unsigned)nlines * nb;
+  for (unsigned item = threadIdx.x; item < total; item += blockDim.x) {
+    const unsigned line = item / nb, j = item - line * nb;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = ld(line, j + r * nb);
+    Radix<R>::dft(v);
+    float2* dst = buf + line * ls + j * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r] = v[r];
+  }
+  __syncthreads();
+}
+
+// Forward FFT of nlines lines of length d.N in LDS, from stage s0 on (the
+// sub-transform size before stage s0 is Ns0).  POW2 restricts the code to
+// radices 8/4/2 (fewer registers).  The caller must have synchronised after
+// filling buf; the routine ends synchronised.
+template <int MAXC, bool POW2 = false>
 __device__ void fft_lines(float2* buf, int ls, int nlines, const FftDesc& d,
-                          const float2* __restrict__ tw) {
-  int Ns = 1;
-  for (int s = 0; s < d.nst; ++s) {
+                          const float2* __restrict__ tw, int s0 = 0, int Ns0 = 1) {
+  int Ns = Ns0;
+  for (int s = s0; s < d.nst; ++s) {
     const int R = d.radix[s];
-    switch (R) {
-      case 8: fft_stage<8, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      case 4: fft_stage<4, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      case 2: fft_stage<2, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      case 3: fft_stage<3, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      case 5: fft_stage<5, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      case 7: fft_stage<7, MAXC>(buf, ls, nlines, d.N, Ns, tw); break;
-      default: break;
+    if (R == 8) {
+      fft_stage<8, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+    } else if (R == 4) {
+      fft_stage<4, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+    } else if (R == 2) {
+      fft_stage<2, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+    } else if constexpr (!POW2) {
+      if (R == 3) fft_stage<3, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+      else if (R == 5) fft_stage<5, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+      else if (R == 7) fft_stage<7, MAXC>(buf, ls, nlines, d.N, Ns, tw);
     }
     Ns *= R;
   }
+}
+
+// Whole transform with the first stage fed by a loader.
+template <int MAXC, bool POW2, class Load>
+__device__ void fft_lines_loaded(float2* buf, int ls, int nlines, const FftDesc& d,
+                                 const float2* __restrict__ tw, Load ld) {
+  const int R = d.radix[0];
+  if (R == 8) fft_stage0<8>(buf, ls, nlines, d.N, ld);
+  else if (R == 4) fft_stage0<4>(buf, ls, nlines, d.N, ld);
+  else if (R == 2) fft_stage0<2>(buf, ls, nlines, d.N, ld);
+  else if constexpr (!POW2) {
+    if (R == 3) fft_stage0<3>(buf, ls, nlines, d.N, ld);
+    else if (R == 5) fft_stage0<5>(buf, ls, nlines, d.N, ld);
+    else if (R == 7) fft_stage0<7>(buf, ls, nlines, d.N, ld);
+  }
+  fft_lines<MAXC, POW2>(buf, ls, nlines, d, tw, 1, R);
+}
+
+__host__ __device__ inline bool fft_is_pow2(const FftDesc& d) {
+  for (int s = 0; s < d.nst; ++s)
+    if (d.radix[s] != 2 && d.radix[s] != 4 && d.radix[s] != 8) return false;
+  return true;
 }
 
 }  // namespace saxs

@@ -136,7 +136,24 @@ int upload(T* dst, const T* src, size_t count) {
 
 }  // namespace
 
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;                 // free events
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> rec;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+
 struct saxs_plan {
+  KernelTimer timer;
   saxs::Params p{};            // geometry; F/N/oc set per call
   saxs_desc desc{};
   int max_frames = 0, max_atoms = 0, nkeys_max = 0;
@@ -207,7 +224,30 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
   return SAXS_OK;
 }
 
+// Brackets one launch with events when the timing hook is on.
+struct Probe {
+  saxs_plan* pl;
+  int id;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  Probe(saxs_plan* pl_, int id_, hipStream_t st_) : pl(pl_), id(id_), st(st_) {
+    if (pl->timer.on && (a = pl->timer.get())) (void)hipEventRecord(a, st);
+  }
+  ~Probe() {
+    if (!a) return;
+    hipEvent_t b = pl->timer.get();
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    pl->timer.rec.push_back({id, {a, b}});
+  }
+};
+
 void free_plan(saxs_plan* pl) {
+  for (auto& r : pl->timer.rec) {
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
+  }
+  for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
   void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
@@ -268,17 +308,22 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   saxs::Buffers b = make_buffers(pl, xyz, elem);
   const int nkeys = n_frames * p.E * p.nty * p.ntx;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
+  { Probe pr(pl, SAXS_K_CLEAR, st);
   hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, pl->counts,
-                     nkeys + 1, pl->pad_sum, n_frames * p.E);
+                     nkeys + 1, pl->pad_sum, n_frames * p.E); }
   if (n_atoms > 0) {
+    Probe pr(pl, SAXS_K_PREP, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
     if (p.order == 4)
       hipLaunchKernelGGL(saxs::k_prep<4>, ga, dim3(saxs::kWG), 0, st, p, b);
     else
       hipLaunchKernelGGL(saxs::k_prep<6>, ga, dim3(saxs::kWG), 0, st, p, b);
   }
-  hipLaunchKernelGGL(saxs::k_scan, dim3(1), dim3(1024), 0, st, pl->counts, pl->offsets, nkeys);
+  { Probe pr(pl, SAXS_K_SCAN, st);
+  hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
+                     p.E * p.nty * p.ntx, n_atoms); }
   if (n_atoms > 0) {
+    Probe pr(pl, SAXS_K_SCATTER, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
     if (p.order == 4)
       hipLaunchKernelGGL(saxs::k_scatter<4>, ga, dim3(saxs::kWG), 0, st, p, b);
@@ -287,14 +332,30 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   }
   dim3 gs(p.ntx * p.nty, p.E, n_frames);
   const size_t lds_s = saxs::spread_lds_bytes(p);
-  if (p.order == 4)
-    hipLaunchKernelGGL(saxs::k_spread_z<4>, gs, dim3(saxs::kWG), lds_s, st, p, b);
+  { Probe pr(pl, SAXS_K_SPREAD_Z, st);
+  const bool pow2 = saxs::fft_is_pow2(p.fz);
+  if (p.order == 4 && pow2)
+    hipLaunchKernelGGL((saxs::k_spread_z<4, true>), gs, dim3(saxs::kWG), lds_s, st, p, b);
+  else if (p.order == 4)
+    hipLaunchKernelGGL((saxs::k_spread_z<4, false>), gs, dim3(saxs::kWG), lds_s, st, p, b);
+  else if (pow2)
+    hipLaunchKernelGGL((saxs::k_spread_z<6, true>), gs, dim3(saxs::kWG), lds_s, st, p, b);
   else
-    hipLaunchKernelGGL(saxs::k_spread_z<6>, gs, dim3(saxs::kWG), lds_s, st, p, b);
-  hipLaunchKernelGGL(saxs::k_yfft, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
-                     saxs::yfft_lds_bytes(p), st, p, b);
-  hipLaunchKernelGGL(saxs::k_xfft_epi, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
-                     saxs::xfft_lds_bytes(p), st, p, b);
+    hipLaunchKernelGGL((saxs::k_spread_z<6, false>), gs, dim3(saxs::kWG), lds_s, st, p, b); }
+  { Probe pr(pl, SAXS_K_YFFT, st);
+  if (saxs::fft_is_pow2(p.fy))
+    hipLaunchKernelGGL(saxs::k_yfft<true>, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
+                       saxs::yfft_lds_bytes(p), st, p, b);
+  else
+    hipLaunchKernelGGL(saxs::k_yfft<false>, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
+                       saxs::yfft_lds_bytes(p), st, p, b); }
+  { Probe pr(pl, SAXS_K_XFFT, st);
+  if (saxs::fft_is_pow2(p.fx))
+    hipLaunchKernelGGL(saxs::k_xfft_epi<true>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
+                       saxs::xfft_lds_bytes(p), st, p, b);
+  else
+    hipLaunchKernelGGL(saxs::k_xfft_epi<false>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
+                       saxs::xfft_lds_bytes(p), st, p, b); }
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
@@ -476,8 +537,11 @@ int saxs_forward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n
   rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
   if (rc) return rc;
   const int nblk = pl->p.ky * pl->p.nchx;
-  hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
-                     pl->partial, nblk, n_frames, pl->p.num_bins, H, C, 0);
+  {
+    Probe pr(pl, SAXS_K_REDUCE, st);
+    hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
+                       pl->partial, nblk, n_frames, pl->p.num_bins, H, C, 0);
+  }
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
@@ -493,8 +557,11 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
   if (rc) return rc;
   const int nblk = pl->p.ky * pl->p.nchx;
-  hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, n_frames), dim3(saxs::kWG), 0, st,
-                     pl->partial, nblk, 1, pl->p.num_bins, Hf, Cf, 1);
+  {
+    Probe pr(pl, SAXS_K_REDUCE, st);
+    hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, n_frames), dim3(saxs::kWG), 0,
+                       st, pl->partial, nblk, 1, pl->p.num_bins, Hf, Cf, 1);
+  }
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
@@ -513,6 +580,29 @@ int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t* elem, int
   g_err.clear();
   (void)pl; (void)xyz; (void)elem; (void)n_atoms; (void)oc_diag; (void)H; (void)C; (void)stream;
   return fail(SAXS_EUNSUP, "saxs_forward_plain is not available in this build");
+}
+
+int saxs_plan_set_timing(saxs_plan* pl, int32_t enable) {
+  if (!pl) return fail(SAXS_EINVAL, "plan is NULL");
+  pl->timer.on = enable != 0;
+  return SAXS_OK;
+}
+
+int saxs_plan_kernel_times(saxs_plan* pl, double* ms, int64_t* launches) {
+  if (!pl || !ms || !launches) return fail(SAXS_EINVAL, "NULL argument");
+  for (auto& r : pl->timer.rec) {
+    HIP_TRY(hipEventSynchronize(r.second.second));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, r.second.first, r.second.second));
+    if (r.first >= 0 && r.first < SAXS_NKERNELS) {
+      ms[r.first] += t;
+      launches[r.first] += 1;
+    }
+    pl->timer.pool.push_back(r.second.first);
+    pl->timer.pool.push_back(r.second.second);
+  }
+  pl->timer.rec.clear();
+  return SAXS_OK;
 }
 
 int64_t saxs_plan_workspace_bytes(const saxs_plan* pl) { return pl ? (int64_t)pl->bytes : 0; }

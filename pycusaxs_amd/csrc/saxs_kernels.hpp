@@ -27,8 +27,8 @@
 namespace saxs {
 
 constexpr int kWG = 256;       // threads per workgroup (4 wave64)
-constexpr int kMaxC = 32;      // complex values per thread in one FFT stage
-constexpr int kOutMax = 16;    // x-pass output points per thread
+constexpr int kMaxC = 16;      // complex values per thread in one FFT stage
+constexpr int kOutMax = 8;     // x-pass output points per thread
 constexpr int kMaxElements = 64;
 
 struct Params {
@@ -179,14 +179,18 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
   }
 }
 
-// Exclusive scan of counts[0..K) into offsets[0..K], one workgroup of 1024.
-__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int* offsets, int K) {
+// Exclusive scan of one frame's counts (blockIdx.x = frame, K keys per
+// frame) into offsets, starting at frame * N: every frame owns the sorted
+// range [f*N, (f+1)*N).
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int* offsets, int K,
+                                               int N) {
   __shared__ int s[1024];
   const int t = threadIdx.x;
+  const size_t off0 = (size_t)blockIdx.x * K;
   const int chunk = (K + 1023) / 1024;
   const int beg = min(K, t * chunk), end = min(K, beg + chunk);
   int sum = 0;
-  for (int i = beg; i < end; ++i) sum += counts[i];
+  for (int i = beg; i < end; ++i) sum += counts[off0 + i];
   s[t] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -195,12 +199,11 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, i
     s[t] += v;
     __syncthreads();
   }
-  int run = s[t] - sum;
+  int run = (int)blockIdx.x * N + s[t] - sum;
   for (int i = beg; i < end; ++i) {
-    offsets[i] = run;
-    run += counts[i];
+    offsets[off0 + i] = run;
+    run += counts[off0 + i];
   }
-  if (t == 1023) offsets[K] = s[1023];
 }
 
 template <int P>
@@ -219,15 +222,22 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   b.sorted[b.offsets[ks.x] + ks.y] = rec;
 }
 
-// LDS bytes of k_spread_z.
+// Atom records staged per round in k_spread_z.
+constexpr int kSpreadRecs = 512;
+
+// LDS bytes of k_spread_z: density columns, FFT lines (the staged atom
+// records alias the FFT lines; they are dead before the FFT starts) and
+// twiddles.
 __host__ __device__ inline size_t spread_lds_bytes(const Params& p) {
-  const size_t rho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
-  const size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t rho = (((size_t)p.tx * p.ty * p.n[2] * sizeof(float)) + 15) & ~size_t(15);
+  size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t recs = (size_t)kSpreadRecs * sizeof(float4);
+  if (buf < recs) buf = recs;
   const size_t tw = (size_t)p.nn[2] * sizeof(float2);
-  return ((rho + 15) & ~size_t(15)) + buf + tw;
+  return rho + buf + tw;
 }
 
-template <int P>
+template <int P, bool POW2>
 __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
@@ -239,83 +249,111 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   const int nl = ncol / 2;
   float* rho = reinterpret_cast<float*>(smem);
   float2* buf = reinterpret_cast<float2*>(smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15)));
-  float2* tw = buf + (size_t)nl * p.lsz;
+  float4* recs = reinterpret_cast<float4*>(buf);
+  const size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
+  float2* tw = reinterpret_cast<float2*>(reinterpret_cast<unsigned char*>(buf) +
+                                          (buf_bytes > kSpreadRecs * sizeof(float4)
+                                               ? buf_bytes : kSpreadRecs * sizeof(float4)));
 
-  __shared__ int s_start[9], s_pref[10], s_nseg;
+  // source segments: up to 3x3 bins whose stencils reach this tile
+  __shared__ int s_start[9], s_cnt[9], s_pref[10];
+  if (threadIdx.x < 9) {
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[threadIdx.x] = st;
+    s_cnt[threadIdx.x] = cnt;
+  }
   for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
   for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int bxs[3] = {sx.y, sx.z, sx.w}, bys[3] = {sy.y, sy.z, sy.w};
-    int ns = 0, acc = 0;
-    for (int iy = 0; iy < sy.x; ++iy)
-      for (int ix = 0; ix < sx.x; ++ix) {
-        const int key = ((f * p.E + e) * p.nty + bys[iy]) * p.ntx + bxs[ix];
-        const int o0 = b.offsets[key], o1 = b.offsets[key + 1];
-        s_start[ns] = o0;
-        s_pref[ns] = acc;
-        acc += (o1 - o0) * P * P;
-        ++ns;
-      }
-    s_pref[ns] = acc;
-    s_nseg = ns;
-  }
-  __syncthreads();
-  const int nseg = s_nseg;
-  const int total = s_pref[nseg];
-  for (int item = threadIdx.x; item < total; item += blockDim.x) {
-    int s = 0;
-    while (s + 1 < nseg && item >= s_pref[s + 1]) ++s;
-    const int local = item - s_pref[s];
-    const int a = local / (P * P);
-    const int col = local - a * (P * P);
-    const int o = col / P, o2 = col - o * P;
-    const float4 rec = b.sorted[s_start[s] + a];
-    const int mx = cell_index(rec.x, p.n[0]), my = cell_index(rec.y, p.n[1]);
-    int gx = mx - P + o;
-    if (gx < 0) gx += p.n[0];
-    const int lx = gx - X0;
-    if (lx < 0 || lx >= wx) continue;
-    int gy = my - P + o2;
-    if (gy < 0) gy += p.n[1];
-    const int ly = gy - Y0;
-    if (ly < 0 || ly >= wy) continue;
-    float th[P], tz[P];
-    bspline<P>(rec.x - (float)mx, th);
-    const float fo = th[o];
-    bspline<P>(rec.y - (float)my, th);
-    const float fp = fo * th[o2];
-    const int mz = cell_index(rec.z, p.n[2]);
-    bspline<P>(rec.z - (float)mz, tz);
-    float* colp = rho + (lx * p.ty + ly) * nz;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      int gz = mz - P + q;
-      if (gz < 0) gz += nz;
-      atomicAdd(colp + gz, fp * tz[q]);
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[k] = acc;
+      acc += s_cnt[k];
     }
+    s_pref[9] = acc;
   }
   __syncthreads();
+  const int total = s_pref[9];
 
-  // Bulk-solvent subtraction inside the cell (split in two floats so the
-  // rounding of the mean does not leave a systematic box-shaped residue),
-  // two real columns packed per complex line, zero padding beyond n_z.
+  // Gather-spread in rounds of kSpreadRecs atoms: stage the records in LDS
+  // with all threads, then one thread per atom computes its B-spline weights
+  // once and adds its stencil columns that fall inside this tile.
+  for (int base = 0; base < total; base += kSpreadRecs) {
+    const int chunk = min(kSpreadRecs, total - base);
+    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
+      const int a = base + i;
+      int sgm = 0;
+#pragma unroll
+      for (int k = 1; k < 9; ++k) sgm += (a >= s_pref[k]) ? 1 : 0;
+      recs[i] = b.sorted[s_start[sgm] + (a - s_pref[sgm])];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
+      const float4 rec = recs[i];
+      const int mx = cell_index(rec.x, p.n[0]);
+      const int my = cell_index(rec.y, p.n[1]);
+      const int mz = cell_index(rec.z, p.n[2]);
+      float thx[P], thy[P], thz[P];
+      bspline<P>(rec.x - (float)mx, thx);
+      bspline<P>(rec.y - (float)my, thy);
+      bspline<P>(rec.z - (float)mz, thz);
+      int gz0 = mz - P;
+      if (gz0 < 0) gz0 += nz;
+#pragma unroll
+      for (int o = 0; o < P; ++o) {
+        int gx = mx - P + o;
+        if (gx < 0) gx += p.n[0];
+        const int lx = gx - X0;
+        if (lx < 0 || lx >= wx) continue;
+#pragma unroll
+        for (int o2 = 0; o2 < P; ++o2) {
+          int gy = my - P + o2;
+          if (gy < 0) gy += p.n[1];
+          const int ly = gy - Y0;
+          if (ly < 0 || ly >= wy) continue;
+          const float fxy = thx[o] * thy[o2];
+          float* colp = rho + (lx * p.ty + ly) * nz;
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            int gz = gz0 + q;
+            if (gz >= nz) gz -= nz;
+            atomicAdd(colp + gz, fxy * thz[q]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // z-transform: two real columns packed per complex line, the bulk-solvent
+  // mean subtracted inside the cell (split in two floats so that its
+  // rounding leaves no systematic box-shaped residue), zero padding beyond
+  // n_z.  The first FFT stage reads the columns directly.
   const double pad = b.pad_sum[f * p.E + e] * p.inv_count;
   const float pad_hi = (float)pad;
   const float pad_lo = (float)(pad - (double)pad_hi);
-  for (int i = threadIdx.x; i < nl * nnz; i += blockDim.x) {
-    const int l = i / nnz, z = i - l * nnz;
-    const int c0 = 2 * l;
-    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+  const int ty = p.ty;
+  auto load = [&](unsigned l, unsigned z) -> float2 {
+    const int c0 = 2 * (int)l;
+    const int lx = c0 / ty, ly = c0 - lx * ty;
     float re = 0.0f, im = 0.0f;
-    if (z < nz && lx < wx) {
+    if ((int)z < nz && lx < wx) {
       if (ly < wy) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
       if (ly + 1 < wy) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
     }
-    buf[l * p.lsz + z] = make_float2(re, im);
-  }
-  __syncthreads();
-  fft_lines<kMaxC>(buf, p.lsz, nl, p.fz, tw);
+    return make_float2(re, im);
+  };
+  fft_lines_loaded<kMaxC, POW2>(buf, p.lsz, nl, p.fz, tw, load);
 
   // Unpack the two real transforms: X_a(k) = (Z(k) + conj Z(-k))/2,
   // X_b(k) = (Z(k) - conj Z(-k))/(2i).
@@ -339,6 +377,7 @@ __host__ __device__ inline size_t yfft_lds_bytes(const Params& p) {
   return (size_t)p.kcy * p.lsy * sizeof(float2) + (size_t)p.nn[1] * sizeof(float2);
 }
 
+template <bool POW2>
 __global__ void __launch_bounds__(kWG) k_yfft(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int x = blockIdx.x, ch = blockIdx.y;
@@ -359,7 +398,7 @@ __global__ void __launch_bounds__(kWG) k_yfft(Params p, Buffers b) {
     buf[k * p.lsy + y] = make_float2(0.0f, 0.0f);
   }
   __syncthreads();
-  fft_lines<kMaxC>(buf, p.lsy, kc, p.fy, tw);
+  fft_lines<kMaxC, POW2>(buf, p.lsy, kc, p.fy, tw);
   for (int i = threadIdx.x; i < p.ky * kc; i += blockDim.x) {
     const int j = i / kc, k = i - j * kc;
     const int iy = (p.ky == nny) ? j : (j <= p.Ky ? j : nny - (p.ky - j));
@@ -372,6 +411,7 @@ __host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
          (size_t)p.nn[0] * sizeof(float2);
 }
 
+template <bool POW2>
 __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
@@ -406,7 +446,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
       buf[k * p.lsx + x] = make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    fft_lines<kMaxC>(buf, p.lsx, kc, p.fx, tw);
+    fft_lines<kMaxC, POW2>(buf, p.lsx, kc, p.fx, tw);
     const float* ff = b.lut_ff + (size_t)e * p.lut_len;
 #pragma unroll
     for (int u = 0; u < kOutMax; ++u) {

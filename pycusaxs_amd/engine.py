@@ -212,6 +212,18 @@ class SaxsEngine:
                    "saxs_plan_kept")
         return kx.value, ky.value, kz.value
 
+    def set_timing(self, on: bool):
+        _lib.check(self.lib.saxs_plan_set_timing(self._plan, 1 if on else 0), "saxs_plan_set_timing")
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} since the last call."""
+        ms = np.zeros(len(_lib.KERNEL_NAMES), dtype=np.float64)
+        n = np.zeros(len(_lib.KERNEL_NAMES), dtype=np.int64)
+        _lib.check(self.lib.saxs_plan_kernel_times(self._plan, ms.ctypes.data_as(C.c_void_p),
+                                                   n.ctypes.data_as(C.c_void_p)),
+                   "saxs_plan_kernel_times")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(_lib.KERNEL_NAMES) if n[i]}
+
     def workspace_bytes(self) -> int:
         return int(self.lib.saxs_plan_workspace_bytes(self._plan))
 
