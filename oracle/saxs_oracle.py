@@ -276,9 +276,21 @@ def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
 
 # ------------------------------------------------------------ per frame ---
 
+# When True, grid coordinates are computed in float64 from float64
+# positions (used only to pin the adjoint against finite differences, which
+# float32 coordinate rounding would blur); the default follows the reference.
+EXACT_COORDS = False
+
+
 def grid_coords(pos, oc_diag, n):
     """Cell index m and fraction w per atom and axis, following the
     reference's float32/float64 mix (saxsDeviceKernels.cu:352-368)."""
+    if EXACT_COORDS:
+        pos = np.asarray(pos, dtype=np.float64)
+        ud = pos * np.asarray(oc_diag, dtype=np.float64)[None, :]
+        r = np.asarray(n, dtype=np.float64)[None, :] * (ud - np.rint(ud - 0.5))
+        m = np.floor(r).astype(np.int64)
+        return m, r - m
     pos = np.asarray(pos, dtype=np.float32)
     u = (pos * oc_diag[None, :].astype(np.float32)).astype(np.float32)
     ud = u.astype(np.float64)
@@ -297,7 +309,7 @@ def spread(pos, oc_diag, n, p) -> np.ndarray:
     if len(pos) == 0:
         return grid.reshape(n)
     m, w = grid_coords(pos, oc_diag, n)
-    th = [bspline_theta(w[:, a].astype(np.float64), p) for a in range(3)]
+    th = [bspline_theta(np.asarray(w[:, a], dtype=np.float64), p) for a in range(3)]
     idx = []
     for a in range(3):
         i = m[:, a][:, None] - p + np.arange(p)[None, :]
@@ -434,11 +446,18 @@ def binned_intensity(plan, pos, elem_ids, symbols, box):
 
 
 def grad_binned_intensity(plan, pos, elem_ids, symbols, box, gbar):
-    """Analytic gradient of L = sum_b gbar_b * I_b (I_b = H_b/C_b) with
-    respect to atom positions, float64, by the transposed pipeline:
-    histogram^T -> |.|^2 -> form factor -> R2C^T (half-spectrum inverse
-    DFT, real part) -> crop + pad-mean term -> B-spline gradient gather."""
-    pos = np.asarray(pos, dtype=np.float32)
+    """Gradient of L = sum_b gbar_b * I_b with I_b = H_b/C_b."""
+    H, C = frame_histogram(plan, pos, elem_ids, symbols, box, literal=False)
+    gH = np.where(C > 0, np.asarray(gbar, dtype=np.float64) / np.where(C > 0, C, 1.0), 0.0)
+    return grad_linear_H(plan, pos, elem_ids, symbols, box, gH)
+
+
+def grad_linear_H(plan, pos, elem_ids, symbols, box, gH):
+    """Analytic gradient of L = sum_b gH_b * H_b with respect to atom
+    positions, float64, by the transposed pipeline: histogram^T -> |.|^2 ->
+    form factor -> R2C^T (half-spectrum inverse DFT, real part) -> crop +
+    pad-mean term -> B-spline gradient gather."""
+    pos = np.asarray(pos, dtype=np.float64 if EXACT_COORDS else np.float32)
     oc = inverse_box(box)
     n, nn, p = plan.n, plan.nn, plan.order
     Fs, inr, bins, q = frame_spectrum_terms(plan, pos, elem_ids, symbols, box, literal=False)
@@ -446,14 +465,13 @@ def grad_binned_intensity(plan, pos, elem_ids, symbols, box, gbar):
     ffs = [form_factor(sym, q) for sym in symbols]
     for e in range(len(symbols)):
         A[inr] += ffs[e][inr] * Fs[e][inr]
-    _, C = histogram(plan, np.zeros(inr.shape), inr, bins)
     mod = plan.moduli[0][:, None, None] * plan.moduli[1][None, :, None] \
         * plan.moduli[2][None, None, : inr.shape[2]]
-    # dL/dI at each half-spectrum point: weight 2 (origin 1) * gbar_b / C_b
+    # dL/dI at each half-spectrum point: weight 2 (origin 1) * gH_b
     w = np.where(inr, 2.0, 0.0)
     w[0, 0, 0] = 1.0 if inr[0, 0, 0] else 0.0
     bsafe = np.where(inr, bins, 0)
-    dI = w * np.asarray(gbar, dtype=np.float64)[bsafe] / np.where(C[bsafe] > 0, C[bsafe], 1.0)
+    dI = w * np.asarray(gH, dtype=np.float64)[bsafe]
     Abar = 2.0 * mod * dI * A                       # dL/d(Re A) + i dL/d(Im A)
     grad = np.zeros((len(pos), 3))
     elem_ids = np.asarray(elem_ids)
@@ -475,8 +493,8 @@ def grad_binned_intensity(plan, pos, elem_ids, symbols, box, gbar):
         if len(sel) == 0:
             continue
         m, wv = grid_coords(pos[sel], oc, n)
-        th = [bspline_theta(wv[:, a].astype(np.float64), p) for a in range(3)]
-        dth = [bspline_dtheta(wv[:, a].astype(np.float64), p) for a in range(3)]
+        th = [bspline_theta(np.asarray(wv[:, a], dtype=np.float64), p) for a in range(3)]
+        dth = [bspline_dtheta(np.asarray(wv[:, a], dtype=np.float64), p) for a in range(3)]
         idx = []
         for a in range(3):
             i = m[:, a][:, None] - p + np.arange(p)[None, :]

@@ -1,0 +1,53 @@
+"""Differentiable SAXS profile: torch.autograd.Function over the engine's
+forward (saxs_forward_frames) and adjoint (saxs_backward) entry points.
+
+    I, C = saxs_profile(xyz, engine, elem)      # xyz [frames, atoms, 3]
+    loss = chi2(I.mean(0), target, sigma, bins)
+    loss.backward()                              # xyz.grad = dL/dx
+
+The reference computes I(q) only; the gradient is an addition that lets a
+chi^2 restraint against a measured profile drive coordinate refinement.
+"""
+from __future__ import annotations
+
+import torch
+
+
+class _SaxsHistogram(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, engine, elem):
+        H, C = engine.forward_frames(xyz.detach(), elem)
+        ctx.engine = engine
+        ctx.elem = elem
+        ctx.save_for_backward(xyz.detach())
+        ctx.mark_non_differentiable(C)
+        return H, C
+
+    @staticmethod
+    def backward(ctx, gH, gC):
+        (xyz,) = ctx.saved_tensors
+        if gH is None:
+            return None, None, None
+        grad = ctx.engine.backward(xyz, ctx.elem, gH.contiguous())
+        return grad.to(xyz.dtype), None, None
+
+
+def saxs_histogram(xyz: torch.Tensor, engine, elem: torch.Tensor):
+    """Per-frame (H, C) [frames, num_bins] float64; H is differentiable."""
+    if xyz.dim() == 2:
+        xyz = xyz.unsqueeze(0)
+    return _SaxsHistogram.apply(xyz, engine, elem)
+
+
+def saxs_profile(xyz: torch.Tensor, engine, elem: torch.Tensor):
+    """Per-frame mean intensity per bin I_b = H_b / C_b (0 where C_b = 0)."""
+    H, C = saxs_histogram(xyz, engine, elem)
+    safe = torch.where(C > 0, C, torch.ones_like(C))
+    return torch.where(C > 0, H / safe, torch.zeros_like(H)), C
+
+
+def chi2(I: torch.Tensor, target: torch.Tensor, sigma: torch.Tensor, bins=None) -> torch.Tensor:
+    """chi^2 = sum_b ((I_b - T_b) / s_b)^2 over the selected bins."""
+    if bins is not None:
+        I, target, sigma = I[..., bins], target[..., bins], sigma[..., bins]
+    return (((I - target) / sigma) ** 2).sum()

@@ -1,0 +1,411 @@
+// Adjoint of the forward scattering path: dL/dx per atom for a loss whose
+// gradient with respect to the per-frame histogram, gH = dL/dH_b, is given.
+// The reference has no gradient; this is the exact transpose of the forward
+// kernels in saxs_kernels.hpp:
+//
+//   forward  rho_e -(R2C z)-> Z_e -(FFT y)-> Y_e -(FFT x)-> F_e
+//            A = sum_e f_e F_e, I = |A|^2 mod, H_b = sum_p w_p I_p (w = 2,
+//            origin 1)
+//   adjoint  Abar = 2 mod w gH_b A, Fbar_e = f_e Abar
+//            Ybar_e = x-transpose (inverse DFT over kept m_x, crop x < n)
+//            Zbar_e = y-transpose (same over m_y)
+//            drho~_e(z) = Re sum_{k < kz} Zbar_e(k) e^{+2 pi i k z / nn}
+//            (each half-spectrum point once: the transpose of R2C, not a
+//            Hermitian C2R)
+//            dL/dx_j = sum_stencil drho~ grad(theta theta theta)
+//                      - (sum_cell drho~ / count) grad(border weight of j)
+//
+// The forward spectra are recomputed (spread_z, yfft) rather than stored;
+// the x-pass adjoint recomputes F_e on the fly and overwrites Y with Ybar in
+// place, the y-pass adjoint overwrites Z with Zbar.
+#pragma once
+#include "saxs_kernels.hpp"
+
+namespace saxs {
+
+struct AdjBuffers {
+  const double* gH;     // [F][num_bins]
+  float* grad;          // [F][N][3]
+  double* padg;         // [F][E] sum over the cell of drho~
+};
+
+__device__ __forceinline__ int kept_to_index(int j, int k, int K, int N) {
+  return (k == N) ? j : (j <= K ? j : N - (k - j));
+}
+
+__global__ void k_fill_zero(float* a, size_t n, double* d, int nd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0f;
+  if (i < (size_t)nd) d[i] = 0.0;
+}
+
+// x-pass adjoint: per (kept m_y, kz chunk, frame).  Recomputes A from Y,
+// forms Abar, and for every element writes Ybar_e over ybuf in place.
+template <bool POW2>
+__global__ void __launch_bounds__(kWG) k_adj_x(Params p, Buffers b, AdjBuffers g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int k0 = ch * p.kcx;
+  const int kc = min(p.kcx, p.kz - k0);
+  const int nx = p.n[0], nnx = p.nn[0];
+  const int nb = p.num_bins;
+  float2* buf = reinterpret_cast<float2*>(smem);
+  float2* tw = buf + (size_t)p.kcx * p.lsx;
+  for (int i = threadIdx.x; i < nnx; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = kept_to_index(jy, p.ky, p.Ky, p.nn[1]);
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const int nout = p.kx * kc;
+  float2 A[kOutMax];
+#pragma unroll
+  for (int u = 0; u < kOutMax; ++u) A[u] = make_float2(0.0f, 0.0f);
+
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * p.kz;
+    for (int i = threadIdx.x; i < nx * kc; i += blockDim.x) {
+      const int x = i / kc, k = i - x * kc;
+      buf[k * p.lsx + x] = src[(size_t)x * p.kz + k0 + k];
+    }
+    for (int i = threadIdx.x; i < (nnx - nx) * kc; i += blockDim.x) {
+      const int x = nx + i / kc, k = i % kc;
+      buf[k * p.lsx + x] = make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    fft_lines<kMaxC, POW2>(buf, p.lsx, kc, p.fx, tw);
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kOutMax; ++u) {
+      const int idx = threadIdx.x + u * blockDim.x;
+      if (idx < nout) {
+        const int jx = idx / kc, k = idx - jx * kc;
+        const int ix = kept_to_index(jx, p.kx, p.Kx, nnx);
+        const int mx = ix < p.nf[0] ? ix : ix - nnx;
+        const int mz = k0 + k;
+        const int nsq = mx * mx + my * my + mz * mz;
+        if (nsq < p.lut_len) {
+          const float fv = ff[nsq];
+          const float2 Fv = buf[k * p.lsx + ix];
+          A[u].x += fv * Fv.x;
+          A[u].y += fv * Fv.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Abar = 2 * mod * w * gH_b * A at in-range points, 0 elsewhere
+  int nsqs[kOutMax];
+#pragma unroll
+  for (int u = 0; u < kOutMax; ++u) {
+    const int idx = threadIdx.x + u * blockDim.x;
+    float2 ab = make_float2(0.0f, 0.0f);
+    int nsq = 1 << 30;
+    if (idx < nout) {
+      const int jx = idx / kc, k = idx - jx * kc;
+      const int ix = kept_to_index(jx, p.kx, p.Kx, nnx);
+      const int mx = ix < p.nf[0] ? ix : ix - nnx;
+      const int kz = k0 + k;
+      nsq = mx * mx + my * my + kz * kz;
+      if (nsq < p.lut_len) {
+        const int bin = b.lut_bin[nsq];
+        if (bin >= 0 && bin < nb) {
+          const float mod = b.modx[ix] * b.mody[iy] * b.modz[kz];
+          const bool origin = (ix == 0) && (iy == 0) && (kz == 0);
+          const float s = 2.0f * mod * (origin ? 1.0f : 2.0f) * (float)g.gH[(size_t)f * nb + bin];
+          ab = make_float2(s * A[u].x, s * A[u].y);
+        }
+      }
+    }
+    A[u] = ab;
+    nsqs[u] = nsq;
+  }
+  // per element: Fbar_e = f_e Abar at the kept m_x, inverse DFT along x,
+  // crop to x < n_x, overwrite Y_e with Ybar_e
+  for (int e = 0; e < p.E; ++e) {
+    for (int i = threadIdx.x; i < kc * nnx; i += blockDim.x) {
+      const int k = i / nnx, x = i - k * nnx;
+      buf[k * p.lsx + x] = make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kOutMax; ++u) {
+      const int idx = threadIdx.x + u * blockDim.x;
+      if (idx < nout && nsqs[u] < p.lut_len) {
+        const int jx = idx / kc, k = idx - jx * kc;
+        const int ix = kept_to_index(jx, p.kx, p.Kx, nnx);
+        const float fv = ff[nsqs[u]];
+        buf[k * p.lsx + ix] = make_float2(fv * A[u].x, -fv * A[u].y);   // conj
+      }
+    }
+    __syncthreads();
+    fft_lines<kMaxC, POW2>(buf, p.lsx, kc, p.fx, tw);
+    float2* dst = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * p.kz;
+    for (int i = threadIdx.x; i < nx * kc; i += blockDim.x) {
+      const int x = i / kc, k = i - x * kc;
+      const float2 v = buf[k * p.lsx + x];
+      dst[(size_t)x * p.kz + k0 + k] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+  }
+}
+
+__host__ __device__ inline size_t adjx_lds_bytes(const Params& p) {
+  return (size_t)p.kcx * p.lsx * sizeof(float2) + (size_t)p.nn[0] * sizeof(float2);
+}
+
+// y-pass adjoint: per (x, kz chunk, frame*E + e): Zbar(x, y, kz) for y < n_y
+// from Ybar over the kept m_y; writes zbuf.
+template <bool POW2>
+__global__ void __launch_bounds__(kWG) k_adj_y(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int x = blockIdx.x, ch = blockIdx.y;
+  const size_t fe = blockIdx.z;
+  const int k0 = ch * p.kcy;
+  const int kc = min(p.kcy, p.kz - k0);
+  const int ny = p.n[1], nny = p.nn[1];
+  float2* buf = reinterpret_cast<float2*>(smem);
+  float2* tw = buf + (size_t)p.kcy * p.lsy;
+  for (int i = threadIdx.x; i < nny; i += blockDim.x) tw[i] = b.twy[i];
+  for (int i = threadIdx.x; i < kc * nny; i += blockDim.x) {
+    const int k = i / nny, y = i - k * nny;
+    buf[k * p.lsy + y] = make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < p.ky * kc; i += blockDim.x) {
+    const int j = i / kc, k = i - j * kc;
+    const int iy = kept_to_index(j, p.ky, p.Ky, nny);
+    const float2 v = b.ybuf[((fe * p.ky + j) * p.n[0] + x) * p.kz + k0 + k];
+    buf[k * p.lsy + iy] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  fft_lines<kMaxC, POW2>(buf, p.lsy, kc, p.fy, tw);
+  float2* dst = b.zbuf + (fe * p.n[0] + x) * (size_t)ny * p.kz;
+  for (int i = threadIdx.x; i < ny * kc; i += blockDim.x) {
+    const int y = i / kc, k = i - y * kc;
+    const float2 v = buf[k * p.lsy + y];
+    dst[(size_t)y * p.kz + k0 + k] = make_float2(v.x, -v.y);
+  }
+}
+
+// LDS of k_adj_z: drho~ columns, FFT lines (the staged atom records alias
+// them after the transform), twiddles.
+__host__ __device__ inline size_t adjz_lds_bytes(const Params& p) { return spread_lds_bytes(p); }
+
+// z-pass adjoint + B-spline gradient gather: per (column tile, e, f).
+template <int P, bool POW2>
+__global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], nnz = p.nn[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  float* drho = reinterpret_cast<float*>(smem);
+  float2* buf = reinterpret_cast<float2*>(smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15)));
+  float4* recs = reinterpret_cast<float4*>(buf);
+  const size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
+  float2* tw = reinterpret_cast<float2*>(reinterpret_cast<unsigned char*>(buf) +
+                                          (buf_bytes > kSpreadRecs * sizeof(float4)
+                                               ? buf_bytes : kSpreadRecs * sizeof(float4)));
+  __shared__ int s_start[9], s_cnt[9], s_pref[10];
+  __shared__ double s_red[kWG / 64];
+  if (threadIdx.x < 9) {
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[threadIdx.x] = st;
+    s_cnt[threadIdx.x] = cnt;
+  }
+  for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
+  for (int i = threadIdx.x; i < nl * nnz; i += blockDim.x) {
+    const int l = i / nnz, z = i - l * nnz;
+    buf[l * p.lsz + z] = make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[k] = acc;
+      acc += s_cnt[k];
+    }
+    s_pref[9] = acc;
+  }
+  // S(k) = H_a(k) + i H_b(k), H(k) = (Zbar(k) + conj Zbar(-k)) / 2 with
+  // Zbar = 0 for k >= kz; stored conjugated (inverse through the forward
+  // transform).
+  const size_t fe = (size_t)f * p.E + e;
+  for (int i = threadIdx.x; i < nl * p.kz; i += blockDim.x) {
+    const int l = i / p.kz, k = i - l * p.kz;
+    const int c0 = 2 * l;
+    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+    float2 a = make_float2(0.0f, 0.0f), c = make_float2(0.0f, 0.0f);
+    if (lx < wx && ly < wy) {
+      const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
+      a = b.zbuf[row * p.kz + k];
+      if (ly + 1 < wy) c = b.zbuf[(row + 1) * p.kz + k];
+    }
+    float2* line = buf + l * p.lsz;
+    if (k == 0 || 2 * k == nnz) {
+      line[k] = make_float2(a.x, -c.x);
+    } else {
+      // S(k) = (a + i c)/2 ; S(N-k) = (conj a + i conj c)/2 ; store conj
+      line[k] = make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
+      line[nnz - k] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+    }
+  }
+  __syncthreads();
+  fft_lines<kMaxC, POW2>(buf, p.lsz, nl, p.fz, tw);
+  double part = 0.0;
+  for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) {
+    const int c = i / nz, z = i - c * nz;
+    const int l = c >> 1;
+    const int lx = c / p.ty, ly = c - lx * p.ty;
+    const float2 v = buf[l * p.lsz + z];
+    const float val = (lx < wx && ly < wy) ? ((c & 1) ? -v.y : v.x) : 0.0f;
+    drho[i] = val;
+    part += (double)val;
+  }
+  // block sum of drho~ over this tile -> padg[f][e]
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_red[w];
+    atomicAdd(&g.padg[fe], t);
+  }
+
+  // gather: every atom whose stencil reaches this tile accumulates
+  // sum drho~ grad(theta_x theta_y theta_z) over its in-tile stencil points
+  const int total = s_pref[9];
+  const float sx = (float)p.n[0] * p.oc[0], sy = (float)p.n[1] * p.oc[1],
+              sz = (float)p.n[2] * p.oc[2];
+  for (int base = 0; base < total; base += kSpreadRecs) {
+    const int chunk = min(kSpreadRecs, total - base);
+    __syncthreads();
+    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
+      const int a = base + i;
+      int sgm = 0;
+#pragma unroll
+      for (int k = 1; k < 9; ++k) sgm += (a >= s_pref[k]) ? 1 : 0;
+      recs[i] = b.sorted[s_start[sgm] + (a - s_pref[sgm])];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
+      const float4 rec = recs[i];
+      const int mx = cell_index(rec.x, p.n[0]);
+      const int my = cell_index(rec.y, p.n[1]);
+      const int mz = cell_index(rec.z, p.n[2]);
+      const float wxf = rec.x - (float)mx, wyf = rec.y - (float)my, wzf = rec.z - (float)mz;
+      float thx[P], thy[P], thz[P], dtx[P], dty[P], dtz[P];
+      bspline<P>(wxf, thx);
+      bspline<P>(wyf, thy);
+      bspline<P>(wzf, thz);
+      bspline_d<P>(wxf, dtx);
+      bspline_d<P>(wyf, dty);
+      bspline_d<P>(wzf, dtz);
+      int gz0 = mz - P;
+      if (gz0 < 0) gz0 += nz;
+      float gx = 0.0f, gy = 0.0f, gzs = 0.0f;
+      bool hit = false;
+#pragma unroll
+      for (int o = 0; o < P; ++o) {
+        int gxi = mx - P + o;
+        if (gxi < 0) gxi += p.n[0];
+        const int lx = gxi - X0;
+        if (lx < 0 || lx >= wx) continue;
+#pragma unroll
+        for (int o2 = 0; o2 < P; ++o2) {
+          int gyi = my - P + o2;
+          if (gyi < 0) gyi += p.n[1];
+          const int ly = gyi - Y0;
+          if (ly < 0 || ly >= wy) continue;
+          hit = true;
+          const float* colp = drho + (lx * p.ty + ly) * nz;
+          float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            int gz = gz0 + q;
+            if (gz >= nz) gz -= nz;
+            const float d = colp[gz];
+            s0 += d * thz[q];
+            s1 += d * dtz[q];
+          }
+          gx += dtx[o] * thy[o2] * s0;
+          gy += thx[o] * dty[o2] * s0;
+          gzs += thx[o] * thy[o2] * s1;
+        }
+      }
+      if (hit) {
+        const int idx = __float_as_int(rec.w);
+        float* gp = g.grad + ((size_t)f * p.N + idx) * 3;
+        atomicAdd(gp + 0, gx * sx);
+        atomicAdd(gp + 1, gy * sy);
+        atomicAdd(gp + 2, gzs * sz);
+      }
+    }
+  }
+}
+
+// Bulk-solvent mean term: dL/dx_j -= (padg_e / count) grad(border weight of
+// atom j), border weight = (sum th_x)(sum th_y)(sum over border z of th_z).
+template <int P>
+__global__ void __launch_bounds__(kWG) k_pad_grad(Params p, Buffers b, AdjBuffers g) {
+  const int f = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.N) return;
+  const int e = b.elem[i];
+  if (e < 0 || e >= p.E) return;
+  const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
+  float w[3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float r = cell_coord(x[a], p.oc[a], p.n[a]);
+    const int m = cell_index(r, p.n[a]);
+    w[a] = r - (float)m;
+    base[a] = wrap_base(m, P, p.n[a]);
+  }
+  float th[P], dth[P];
+  double Sx = 0, Sy = 0, Sz = 0, dSx = 0, dSy = 0, dSz = 0;
+  bspline<P>(w[0], th);
+  bspline_d<P>(w[0], dth);
+#pragma unroll
+  for (int o = 0; o < P; ++o) {
+    Sx += th[o];
+    dSx += dth[o];
+  }
+  bspline<P>(w[1], th);
+  bspline_d<P>(w[1], dth);
+#pragma unroll
+  for (int o = 0; o < P; ++o) {
+    Sy += th[o];
+    dSy += dth[o];
+  }
+  bspline<P>(w[2], th);
+  bspline_d<P>(w[2], dth);
+#pragma unroll
+  for (int o = 0; o < P; ++o) {
+    int z = base[2] + o;
+    if (z >= p.n[2]) z -= p.n[2];
+    if (z <= p.border || z >= p.n[2] - p.border) {
+      Sz += th[o];
+      dSz += dth[o];
+    }
+  }
+  const double c = g.padg[(size_t)f * p.E + e] * p.inv_count;
+  float* gp = g.grad + ((size_t)f * p.N + i) * 3;
+  gp[0] -= (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
+  gp[1] -= (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
+  gp[2] -= (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
+}
+
+}  // namespace saxs

@@ -14,6 +14,7 @@
 
 #include "saxs_engine.h"
 #include "saxs_kernels.hpp"
+#include "saxs_adjoint.hpp"
 
 namespace {
 
@@ -298,13 +299,19 @@ int check_call(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms
   return SAXS_OK;
 }
 
-// Enqueues the forward pipeline up to the per-block histograms.
-int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms,
-                        int n_frames, const float* oc_diag, hipStream_t st) {
+saxs::Params call_params(saxs_plan* pl, int n_atoms, int n_frames, const float* oc_diag) {
   saxs::Params p = pl->p;
   p.F = n_frames;
   p.N = n_atoms;
   for (int a = 0; a < 3; ++a) p.oc[a] = oc_diag[a];
+  return p;
+}
+
+// Enqueues the forward pipeline up to the per-block histograms (or, with
+// with_x false, up to the y-pass output for the adjoint).
+int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms,
+                        int n_frames, const float* oc_diag, hipStream_t st, bool with_x = true) {
+  saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
   const int nkeys = n_frames * p.E * p.nty * p.ntx;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
@@ -349,6 +356,10 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   else
     hipLaunchKernelGGL(saxs::k_yfft<false>, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
                        saxs::yfft_lds_bytes(p), st, p, b); }
+  if (!with_x) {
+    HIP_TRY(hipGetLastError());
+    return SAXS_OK;
+  }
   { Probe pr(pl, SAXS_K_XFFT, st);
   if (saxs::fft_is_pow2(p.fx))
     hipLaunchKernelGGL(saxs::k_xfft_epi<true>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
@@ -570,9 +581,60 @@ int saxs_backward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t 
                   int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
                   void* stream) {
   g_err.clear();
-  (void)pl; (void)xyz; (void)elem; (void)n_atoms; (void)n_frames; (void)oc_diag; (void)gH;
-  (void)grad; (void)stream;
-  return fail(SAXS_EUNSUP, "saxs_backward is not available in this build");
+  int rc = check_call(pl, xyz, elem, n_atoms, n_frames, oc_diag);
+  if (rc) return rc;
+  if (!gH || (n_atoms > 0 && !grad)) return fail(SAXS_EINVAL, "gH/grad pointer is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, false);
+  if (rc) return rc;
+  saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
+  saxs::Buffers b = make_buffers(pl, xyz, elem);
+  saxs::AdjBuffers g{gH, grad, pl->padg};
+  const size_t ng = (size_t)n_frames * n_atoms * 3;
+  const size_t nfill = std::max(ng, (size_t)n_frames * p.E);
+  hipLaunchKernelGGL(saxs::k_fill_zero, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, st,
+                     grad, ng, pl->padg, n_frames * p.E);
+  const bool px = saxs::fft_is_pow2(p.fx), py = saxs::fft_is_pow2(p.fy),
+             pz = saxs::fft_is_pow2(p.fz);
+  {
+    Probe pr(pl, SAXS_K_ADJ_X, st);
+    dim3 gx(p.ky, p.nchx, n_frames);
+    if (px)
+      hipLaunchKernelGGL(saxs::k_adj_x<true>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
+    else
+      hipLaunchKernelGGL(saxs::k_adj_x<false>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
+  }
+  {
+    Probe pr(pl, SAXS_K_ADJ_Y, st);
+    dim3 gy(p.n[0], p.nchy, n_frames * p.E);
+    if (py)
+      hipLaunchKernelGGL(saxs::k_adj_y<true>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b);
+    else
+      hipLaunchKernelGGL(saxs::k_adj_y<false>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b);
+  }
+  {
+    Probe pr(pl, SAXS_K_ADJ_Z, st);
+    dim3 gz(p.ntx * p.nty, p.E, n_frames);
+    const size_t lds = saxs::adjz_lds_bytes(p);
+    if (p.order == 4 && pz)
+      hipLaunchKernelGGL((saxs::k_adj_z<4, true>), gz, dim3(saxs::kWG), lds, st, p, b, g);
+    else if (p.order == 4)
+      hipLaunchKernelGGL((saxs::k_adj_z<4, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
+    else if (pz)
+      hipLaunchKernelGGL((saxs::k_adj_z<6, true>), gz, dim3(saxs::kWG), lds, st, p, b, g);
+    else
+      hipLaunchKernelGGL((saxs::k_adj_z<6, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
+  }
+  if (n_atoms > 0) {
+    Probe pr(pl, SAXS_K_GATHER, st);
+    dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
+    if (p.order == 4)
+      hipLaunchKernelGGL(saxs::k_pad_grad<4>, ga, dim3(saxs::kWG), 0, st, p, b, g);
+    else
+      hipLaunchKernelGGL(saxs::k_pad_grad<6>, ga, dim3(saxs::kWG), 0, st, p, b, g);
+  }
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
 }
 
 int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,

@@ -1,0 +1,103 @@
+"""GPU parity of the adjoint (saxs_backward) against the FP64 oracle gradient.
+
+The oracle's adjoint is itself pinned by central finite differences
+(tests/test_oracle_adjoint.py).  Bar: relative L2 error of the float32
+engine gradient <= 2e-4, and every component within 1e-3 of the largest
+gradient magnitude."""
+import numpy as np
+import pytest
+
+from oracle import saxs_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 2e-4
+
+
+def run_case(n, L, n_atoms, order=4, qcut=100.0, sigma=2.0, frames=1, seed=0, dq=0.0,
+             policy="reference"):
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    box = np.diag([L] * 3)
+    base, syms = O.synthetic_system(n_atoms, L, seed=seed)
+    fr = [O.frame_positions(base, seed, f, 0.3) for f in range(frames)]
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, sigma)
+    spec = SystemSpec(n3, nn, order, tuple(symbols), sigma, dq, qcut, policy)
+    eng = SaxsEngine(spec, box, max_atoms=n_atoms, max_frames=frames)
+    xyz = torch.from_numpy(np.stack(fr))
+    idt = torch.from_numpy(ids)
+    Hf, Cf = eng.forward_frames(xyz, idt)
+    rng = np.random.default_rng(seed + 100)
+    Hn = Hf.cpu().numpy()
+    gH = rng.normal(size=Hn.shape) / np.maximum(np.abs(Hn), 1.0)
+    grad = eng.backward(xyz, idt, torch.from_numpy(gH)).cpu().numpy()
+    eng.close()
+    plan = O.make_plan((n,) * 3, box, order, sigma, None, dq, qcut, policy)
+    worst = 0.0
+    for f in range(frames):
+        go = O.grad_linear_H(plan, fr[f], ids, symbols, box, gH[f])
+        rel = np.linalg.norm(grad[f] - go) / np.linalg.norm(go)
+        mx = np.abs(grad[f] - go).max() / np.abs(go).max()
+        print(f"[n={n} P={order} frame {f}] rel-L2 {rel:.2e}  max/|g|max {mx:.2e}", flush=True)
+        assert rel <= REL_L2 and mx <= 1e-3
+        worst = max(worst, rel)
+    return worst
+
+
+def test_adjoint_full_spectrum(gpu):
+    run_case(16, 20.0, 300, qcut=100.0)
+
+
+def test_adjoint_pruned(gpu):
+    run_case(32, 40.0, 3000, qcut=0.8, seed=1)
+
+
+def test_adjoint_order6(gpu):
+    run_case(32, 30.0, 1500, order=6, qcut=1.2, seed=2)
+
+
+def test_adjoint_mixed_radix(gpu):
+    run_case(30, 25.0, 1500, qcut=100.0, seed=3)
+
+
+def test_adjoint_batched_frames(gpu):
+    run_case(32, 40.0, 2000, qcut=0.9, frames=3, seed=4)
+
+
+def test_adjoint_exact_bins(gpu):
+    run_case(32, 40.0, 2000, qcut=0.6, dq=0.01, policy="exact", seed=5)
+
+
+def test_autograd_chi2_descends(gpu):
+    """chi^2 restraint through torch.autograd: the engine gradient equals the
+    chain rule through I_b = H_b/C_b, and a small step along -grad lowers
+    the loss."""
+    import torch
+    from pycusaxs_amd.autograd import chi2, saxs_profile
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    L, n = 40.0, 32
+    box = np.diag([L] * 3)
+    base, syms = O.synthetic_system(2000, L, seed=7)
+    target_pos = O.frame_positions(base, 7, 0, 1.0)
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 1.0), box, 2000, 1)
+    idt = torch.from_numpy(ids).cuda()
+    with torch.no_grad():
+        T, C = saxs_profile(torch.from_numpy(target_pos).cuda()[None], eng, idt)
+    sel = (C[0] > 0) & (torch.arange(C.shape[1], device=C.device) >= 1)
+    sigma = 0.01 * T[0].abs().clamp_min(1e-6)
+    x = torch.from_numpy(base).cuda()[None].clone().requires_grad_(True)
+    I, _ = saxs_profile(x, eng, idt)
+    loss = chi2(I[0], T[0], sigma, sel)
+    loss.backward()
+    g = x.grad.clone()
+    assert torch.isfinite(g).all() and g.abs().max() > 0
+    with torch.no_grad():
+        step = 1e-3 / g.abs().max()
+        I2, _ = saxs_profile(x - step * g, eng, idt)
+        loss2 = chi2(I2[0], T[0], sigma, sel)
+    print(f"chi2 {loss.item():.6e} -> {loss2.item():.6e}", flush=True)
+    assert loss2.item() < loss.item()
+    eng.close()
