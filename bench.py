@@ -74,6 +74,10 @@ def kernel_bytes(eng, frames_per_launch, n_atoms):
         "spread_z": F * (16.0 * n_atoms + zb),
         "yfft": F * (zb + yb),
         "xfft_epi": F * yb,
+        "adj_x": F * 2 * yb,
+        "adj_y": F * (yb + zb),
+        "adj_z": F * (zb + 16.0 * n_atoms + 12.0 * n_atoms),
+        "gather": F * n_atoms * (12 + 4 + 24),
     }
 
 
@@ -100,6 +104,7 @@ def main():
     ap.add_argument("--batch", type=int, default=100, help="frames per engine launch chain")
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--cpu-frames", type=int, default=10)
+    ap.add_argument("--plain-frames", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -162,10 +167,46 @@ def main():
         print(json.dumps(line))
         return
 
+    # forward + backward (per-frame histograms, then dL/dx for a fixed dL/dH)
+    gen = torch.Generator(device="cpu").manual_seed(SEED)
+    Hf, _ = eng.forward_frames(xyz, ids)
+    gH = (torch.randn(Hf.shape, generator=gen, dtype=torch.float64)
+          / Hf.abs().clamp_min(1.0).cpu()).to(dev)
+    for _ in range(max(1, args.warmup)):
+        eng.forward_frames(xyz, ids)
+        eng.backward(xyz, ids, gH)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.forward_frames(xyz, ids)
+        eng.backward(xyz, ids, gH)
+    torch.cuda.synchronize()
+    fb = frames_done / (time.perf_counter() - t0)
+    line["fwd_bwd_frames_per_s"] = round(fb, 2)
+    line["fwd_bwd_time_over_fwd"] = round(value / fb, 3)
+    print(f"# fwd+bwd: {fb:.1f} frames/s ({value / fb:.2f}x forward time)", file=sys.stderr,
+          flush=True)
+
+    # reference-structured per-frame path (per element, full-spectrum FFT)
+    n_plain = min(args.plain_frames, args.frames)
+    eng.forward_plain(xyz[:1], ids)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.forward_plain(xyz[:n_plain], ids)
+    torch.cuda.synchronize()
+    pl_fps = n_plain / (time.perf_counter() - t0)
+    line["plain_frames_per_s"] = round(pl_fps, 2)
+    line["batched_over_plain"] = round(value / pl_fps, 2)
+    print(f"# plain per-frame path: {pl_fps:.1f} frames/s; batched/plain = {value / pl_fps:.1f}x",
+          file=sys.stderr, flush=True)
+
     # per-kernel hipEvent times over the same number of steps
     eng.set_timing(True)
     for _ in range(args.steps):
         eng.forward(xyz, ids)
+    torch.cuda.synchronize()
+    eng.forward_frames(xyz, ids)
+    eng.backward(xyz, ids, gH)
     torch.cuda.synchronize()
     times = eng.kernel_times()
     eng.set_timing(False)

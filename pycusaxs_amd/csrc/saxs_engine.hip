@@ -15,6 +15,7 @@
 #include "saxs_engine.h"
 #include "saxs_kernels.hpp"
 #include "saxs_adjoint.hpp"
+#include "saxs_plain.hpp"
 
 namespace {
 
@@ -183,8 +184,12 @@ struct saxs_plan {
   double* hframe = nullptr;    // [max_frames][2*num_bins] scratch for gradients
   float* grid = nullptr;       // [E][n^3] adjoint density / plain-path grid
   float* sup = nullptr;        // [nn^3] plain-path embedded grid
-  float2* spec = nullptr;      // [nn_x][nn_y][h] plain-path spectrum
+  float2* spec = nullptr;      // [nn_x][nn_y][h] plain-path spectrum (z pass)
+  float2* yspec = nullptr;     // [nn_y][nn_x][h] plain-path spectrum (y pass)
   float2* acc = nullptr;       // [nn_x][nn_y][h] plain-path accumulator
+  double* padp = nullptr;      // [1] plain-path border sum
+  saxs::Params pp{};           // plain-path geometry (the whole FFT grid as "cell")
+  int plain_pairs = 0;
   double* padg = nullptr;      // [F*E] adjoint pad terms
 };
 
@@ -252,7 +257,7 @@ void free_plan(saxs_plan* pl) {
   void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
-                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg};
+                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete pl;
@@ -637,12 +642,126 @@ int saxs_backward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t 
   return SAXS_OK;
 }
 
-int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
-                       const float* oc_diag, double* H, double* C, void* stream) {
-  g_err.clear();
-  (void)pl; (void)xyz; (void)elem; (void)n_atoms; (void)oc_diag; (void)H; (void)C; (void)stream;
-  return fail(SAXS_EUNSUP, "saxs_forward_plain is not available in this build");
+}  // extern "C"
+
+namespace {
+
+// Full-spectrum geometry for the plain path and its buffers (allocated on
+// first use).
+int plain_setup(saxs_plan* pl) {
+  if (pl->acc) return SAXS_OK;
+  saxs::Params q = pl->p;
+  for (int a = 0; a < 3; ++a) q.n[a] = q.nn[a];
+  const int h = q.nn[2] / 2 + 1;
+  q.kz = h;
+  q.ky = q.nn[1];
+  q.Ky = q.nn[1] / 2;
+  q.kx = q.nn[0];
+  q.Kx = q.nn[0] / 2;
+  const size_t budget = 48 * 1024;
+  int kcy = std::min({h, fft_line_cap(q.fy), (int)(budget / (sizeof(float2) * q.lsy))});
+  int kcx = std::min({h, fft_line_cap(q.fx), (int)(budget / (sizeof(float2) * q.lsx))});
+  if (kcy < 1 || kcx < 1) return fail(SAXS_EUNSUP, "plain path: FFT lines do not fit LDS");
+  q.nchy = (h + kcy - 1) / kcy;
+  q.kcy = (h + q.nchy - 1) / q.nchy;
+  q.nchx = (h + kcx - 1) / kcx;
+  q.kcx = (h + q.nchx - 1) / q.nchx;
+  pl->plain_pairs = std::min(16, std::max(1, fft_line_cap(q.fz)));
+  while (pl->plain_pairs > 1 &&
+         (size_t)pl->plain_pairs * q.lsz * sizeof(float2) + q.nn[2] * sizeof(float2) > kLdsLimit)
+    pl->plain_pairs /= 2;
+  if (q.nn[1] % 2) return fail(SAXS_EUNSUP, "plain path needs an even nn_y");
+  pl->pp = q;
+  const size_t n3 = (size_t)pl->p.n[0] * pl->p.n[1] * pl->p.n[2];
+  const size_t nn3 = (size_t)q.nn[0] * q.nn[1] * q.nn[2];
+  const size_t nh = (size_t)q.nn[0] * q.nn[1] * h;
+  size_t& tot = pl->bytes;
+  int r;
+  if ((r = dev_alloc(&pl->grid, n3, tot)) || (r = dev_alloc(&pl->sup, nn3, tot)) ||
+      (r = dev_alloc(&pl->spec, nh, tot)) || (r = dev_alloc(&pl->yspec, nh, tot)) ||
+      (r = dev_alloc(&pl->padp, 1, tot)) || (r = dev_alloc(&pl->acc, nh, tot)))
+    return r;
+  return SAXS_OK;
 }
+
+}  // namespace
+
+extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t* elem,
+                                  int32_t n_atoms, const float* oc_diag, double* H, double* C,
+                                  void* stream) {
+  g_err.clear();
+  int rc = check_call(pl, xyz, elem, n_atoms, 1, oc_diag);
+  if (rc) return rc;
+  if (!H || !C) return fail(SAXS_EINVAL, "H/C pointer is NULL");
+  rc = plain_setup(pl);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  saxs::Params p = call_params(pl, n_atoms, 1, oc_diag);     // cell geometry
+  saxs::Params q = pl->pp;                                   // full grid geometry
+  q.F = 1;
+  q.N = n_atoms;
+  for (int a = 0; a < 3; ++a) q.oc[a] = oc_diag[a];
+  saxs::Buffers b = make_buffers(pl, xyz, elem);
+  saxs::Buffers bq = b;
+  bq.zbuf = pl->spec;
+  bq.ybuf = pl->yspec;
+  const int h = q.nn[2] / 2 + 1;
+  const size_t n3 = (size_t)p.n[0] * p.n[1] * p.n[2];
+  const size_t nn3 = (size_t)q.nn[0] * q.nn[1] * q.nn[2];
+  const size_t nh = (size_t)q.nn[0] * q.nn[1] * h;
+  const bool pz = saxs::fft_is_pow2(q.fz), py = saxs::fft_is_pow2(q.fy),
+             px = saxs::fft_is_pow2(q.fx);
+  Probe pr(pl, SAXS_K_PLAIN, st);
+  hipLaunchKernelGGL(saxs::k_plain_fill, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st,
+                     (float*)nullptr, (size_t)0, pl->acc, nh, (double*)nullptr);
+  for (int e = 0; e < p.E; ++e) {
+    hipLaunchKernelGGL(saxs::k_plain_fill, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                       pl->grid, n3, (float2*)nullptr, (size_t)0, pl->padp);
+    if (n_atoms > 0) {
+      dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG);
+      if (p.order == 4)
+        hipLaunchKernelGGL(saxs::k_plain_spread<4>, ga, dim3(saxs::kWG), 0, st, p, xyz, elem, e,
+                           pl->grid);
+      else
+        hipLaunchKernelGGL(saxs::k_plain_spread<6>, ga, dim3(saxs::kWG), 0, st, p, xyz, elem, e,
+                           pl->grid);
+    }
+    hipLaunchKernelGGL(saxs::k_plain_pad, dim3(256), dim3(saxs::kWG), 0, st, p, pl->grid, pl->padp);
+    hipLaunchKernelGGL(saxs::k_plain_embed, dim3((unsigned)((nn3 + 255) / 256)), dim3(256), 0, st,
+                       p, pl->grid, pl->sup, pl->padp);
+    const int np = pl->plain_pairs;
+    const size_t npairs = (size_t)q.nn[0] * (q.nn[1] / 2);
+    const size_t lds_z = (size_t)np * q.lsz * sizeof(float2) + q.nn[2] * sizeof(float2);
+    dim3 gz((unsigned)((npairs + np - 1) / np));
+    if (pz)
+      hipLaunchKernelGGL(saxs::k_plain_zfft<true>, gz, dim3(saxs::kWG), lds_z, st, q, b, pl->sup,
+                         pl->spec, np);
+    else
+      hipLaunchKernelGGL(saxs::k_plain_zfft<false>, gz, dim3(saxs::kWG), lds_z, st, q, b, pl->sup,
+                         pl->spec, np);
+    dim3 gy(q.nn[0], q.nchy, 1);
+    if (py)
+      hipLaunchKernelGGL(saxs::k_yfft<true>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(q), st, q, bq);
+    else
+      hipLaunchKernelGGL(saxs::k_yfft<false>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(q), st, q, bq);
+    dim3 gx(q.nn[1], q.nchx, 1);
+    const size_t lds_x = (size_t)q.kcx * q.lsx * sizeof(float2) + q.nn[0] * sizeof(float2);
+    if (px)
+      hipLaunchKernelGGL(saxs::k_plain_xfft_scatter<true>, gx, dim3(saxs::kWG), lds_x, st, q, b,
+                         pl->yspec, pl->acc, e);
+    else
+      hipLaunchKernelGGL(saxs::k_plain_xfft_scatter<false>, gx, dim3(saxs::kWG), lds_x, st, q, b,
+                         pl->yspec, pl->acc, e);
+  }
+  hipLaunchKernelGGL(saxs::k_plain_modulus, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, q,
+                     b, pl->acc);
+  hipLaunchKernelGGL(saxs::k_plain_hist, dim3(1024), dim3(saxs::kWG),
+                     2 * q.num_bins * sizeof(double), st, q, b, pl->acc, H, C);
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+extern "C" {
 
 int saxs_plan_set_timing(saxs_plan* pl, int32_t enable) {
   if (!pl) return fail(SAXS_EINVAL, "plan is NULL");

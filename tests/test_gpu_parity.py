@@ -118,3 +118,75 @@ def test_atoms_outside_box_wrap(gpu):
     H, C = engine_hist([shifted], syms, box, 32, 2.0, 0.0, 100.0)
     Ho, Co = oracle_hist([shifted], syms, box, 32, 2.0, 0.0, 100.0)
     compare(H, C, Ho, Co, label="wrapped")
+
+
+def plain_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4):
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, sigma)
+    eng = SaxsEngine(SystemSpec(n3, nn, order, tuple(symbols), sigma, dq, qcut), box,
+                     max_atoms=len(frames[0]), max_frames=1)
+    eng.forward_plain(torch.from_numpy(np.stack(frames)), torch.from_numpy(ids))
+    torch.cuda.synchronize()
+    H, C = eng.H.cpu().numpy(), eng.C.cpu().numpy()
+    eng.close()
+    return H, C
+
+
+def test_plain_path_c1(gpu):
+    box = np.diag([37.0] * 3)
+    pos, syms = O.synthetic_system(5000, 37.0, seed=1)
+    H, C = plain_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
+    Ho, Co = oracle_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
+    compare(H, C, Ho, Co, label="plain C1")
+
+
+@pytest.mark.parametrize("n,sigma,order", [(16, 2.0, 4), (30, 2.0, 4), (32, 2.0, 6)])
+def test_plain_path_full_spectrum(gpu, n, sigma, order):
+    box = np.diag([24.0] * 3)
+    base, syms = O.synthetic_system(800, 24.0, seed=n)
+    frames = [O.frame_positions(base, n, f, 0.4) for f in range(2)]
+    H, C = plain_hist(frames, syms, box, n, sigma, 0.0, 100.0, order)
+    Ho, Co = oracle_hist(frames, syms, box, n, sigma, 0.0, 100.0, order)
+    compare(H, C, Ho, Co, label=f"plain n={n} P={order}")
+
+
+def test_per_frame_histograms(gpu):
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    box = np.diag([30.0] * 3)
+    base, syms = O.synthetic_system(1000, 30.0, seed=12)
+    frames = [O.frame_positions(base, 12, f, 0.5) for f in range(3)]
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((32,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 1.0), box, 1000, 2)
+    Hf, Cf = eng.forward_frames(torch.from_numpy(np.stack(frames)), torch.from_numpy(ids))
+    Hf, Cf = Hf.cpu().numpy(), Cf.cpu().numpy()
+    plan = O.make_plan((32,) * 3, box, 4, 2.0, None, 0.0, 1.0)
+    for f in range(3):
+        Ho, Co = O.frame_histogram(plan, frames[f], ids, symbols, box)
+        compare(Hf[f], Cf[f], Ho, Co, label=f"frame {f}")
+    eng.close()
+
+
+def test_invalid_element_ids_are_ignored(gpu):
+    """Atoms whose element id is outside [0, E) are skipped by the kernels
+    (bounds safety), equal to leaving them out."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    box = np.diag([30.0] * 3)
+    pos, syms = O.synthetic_system(600, 30.0, seed=13)
+    symbols, ids = O.element_ids(syms)
+    bad = ids.copy()
+    bad[::5] = 99
+    n3, nn = resolve_grid((32,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 1.0), box, 600, 1)
+    eng.forward(torch.from_numpy(pos[None]), torch.from_numpy(bad))
+    torch.cuda.synchronize()
+    keep = np.ones(len(pos), bool)
+    keep[::5] = False
+    plan = O.make_plan((32,) * 3, box, 4, 2.0, None, 0.0, 1.0)
+    Ho, Co = O.frame_histogram(plan, pos[keep], ids[keep], symbols, box)
+    compare(eng.H.cpu().numpy(), eng.C.cpu().numpy(), Ho, Co, label="invalid ids")
+    eng.close()
