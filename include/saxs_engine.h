@@ -58,7 +58,13 @@ typedef struct saxs_plan saxs_plan;
  *  mod_x/y/z    [nn[a]] B-spline modulus tables lambda^2/|b|^2 (BSpmod.cpp:45-129).
  *  max_atoms, max_frames  capacity of the plan's workspace.
  *  tile_x, tile_y spreading column tile (0 = engine default).
+ *  flags        SAXS_FLAG_PER_ELEMENT_SPREAD: one spreading workgroup per
+ *               (tile, element) instead of one per tile for all elements;
+ *               SAXS_FLAG_NO_PRUNED_Z: full Stockham z transform even where
+ *               the output-pruned radix-16 one applies.
  */
+#define SAXS_FLAG_PER_ELEMENT_SPREAD 1
+#define SAXS_FLAG_NO_PRUNED_Z 2
 typedef struct saxs_desc {
   int32_t n[3];
   int32_t nn[3];
@@ -76,6 +82,7 @@ typedef struct saxs_desc {
   int32_t max_frames;
   int32_t tile_x;
   int32_t tile_y;
+  int32_t flags;
 } saxs_desc;
 
 /* Creates a plan: copies tables, builds FFT radix plans and twiddles,

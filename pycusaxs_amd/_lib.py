@@ -42,7 +42,12 @@ class SaxsDesc(C.Structure):
         ("max_frames", C.c_int32),
         ("tile_x", C.c_int32),
         ("tile_y", C.c_int32),
+        ("flags", C.c_int32),
     ]
+
+
+FLAG_PER_ELEMENT_SPREAD = 1
+FLAG_NO_PRUNED_Z = 2
 
 
 _lib = None

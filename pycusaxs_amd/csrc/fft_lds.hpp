@@ -236,6 +236,58 @@ __device__ void fft_lines_loaded(float2* buf, int ls, int nlines, const FftDesc&
   fft_lines<MAXC, POW2>(buf, ls, nlines, d, tw, 1, R);
 }
 
+// W16^m = exp(-2 pi i m / 16) for m = 0..15.
+__device__ __forceinline__ float2 w16(int m) {
+  constexpr float C[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                           0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
+                           -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
+                           0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+  return make_float2(C[m & 15], -C[(m + 12) & 15]);
+}
+
+// Radix-16 DFT in registers as 4 x 4: n = n1 + 4 n2, k = k2 + 4 k1.
+// HALF: inputs 8..15 are zero.
+template <bool HALF>
+__device__ __forceinline__ void dft16(float2* v) {
+  float2 u[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    float2 t[4] = {v[n1], v[n1 + 4], HALF ? make_float2(0.0f, 0.0f) : v[n1 + 8],
+                   HALF ? make_float2(0.0f, 0.0f) : v[n1 + 12]};
+    Radix<4>::dft(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) u[n1][k2] = (n1 * k2 == 0) ? t[k2] : cmul(t[k2], w16(n1 * k2));
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    float2 t[4] = {u[0][k2], u[1][k2], u[2][k2], u[3][k2]};
+    Radix<4>::dft(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = t[k1];
+  }
+}
+
+// Radix-32 DFT in registers as 16 x 2: n = n1 + 2 n2 (n1 < 2, n2 < 16),
+// k = k2 + 16 k1: X[k] = sum_n1 W32^{n1 k2} W2^{n1 k1} DFT16_{n2}(v[n1 + 2 n2])[k2].
+__device__ __forceinline__ void dft32(float2* v) {
+  float2 a[16], c[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 16; ++n2) {
+    a[n2] = v[2 * n2];
+    c[n2] = v[2 * n2 + 1];
+  }
+  dft16<false>(a);
+  dft16<false>(c);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) {
+    const double ang = -2.0 * 3.14159265358979323846 * k2 / 32.0;
+    const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+    const float2 t = (k2 == 0) ? c[k2] : cmul(c[k2], w);
+    v[k2] = cadd(a[k2], t);
+    v[k2 + 16] = csub(a[k2], t);
+  }
+}
+
 __host__ __device__ inline bool fft_is_pow2(const FftDesc& d) {
   for (int s = 0; s < d.nst; ++s)
     if (d.radix[s] != 2 && d.radix[s] != 4 && d.radix[s] != 8) return false;

@@ -156,6 +156,8 @@ struct KernelTimer {
 
 struct saxs_plan {
   KernelTimer timer;
+  bool spread_all = true;      // one workgroup per tile for all elements
+  bool pruned_z = true;        // output-pruned radix-16 z transform where it applies
   saxs::Params p{};            // geometry; F/N/oc set per call
   saxs_desc desc{};
   int max_frames = 0, max_atoms = 0, nkeys_max = 0;
@@ -346,7 +348,32 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const size_t lds_s = saxs::spread_lds_bytes(p);
   { Probe pr(pl, SAXS_K_SPREAD_Z, st);
   const bool pow2 = saxs::fft_is_pow2(p.fz);
-  if (p.order == 4 && pow2)
+  const size_t lds_all = saxs::spread_all_lds_bytes(p);
+  const size_t lds_p = saxs::spread_p_lds_bytes(p);
+  const int Nz = p.nn[2];
+  const bool pruned_ok = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
+                         2 * p.kz - 1 <= Nz / 4 && (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG &&
+                         lds_p <= kLdsLimit;
+  if (pruned_ok) {
+    dim3 ga(p.ntx * p.nty, n_frames);
+#define SAXS_ZP(PP, NN) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN>), ga, dim3(saxs::kWG), lds_p, st, p, b)
+    if (p.order == 4) {
+      if (Nz == 128) SAXS_ZP(4, 128); else if (Nz == 256) SAXS_ZP(4, 256); else SAXS_ZP(4, 512);
+    } else {
+      if (Nz == 128) SAXS_ZP(6, 128); else if (Nz == 256) SAXS_ZP(6, 256); else SAXS_ZP(6, 512);
+    }
+#undef SAXS_ZP
+  } else if (pl->spread_all && lds_all <= kLdsLimit) {
+    dim3 ga(p.ntx * p.nty, n_frames);
+    if (p.order == 4 && pow2)
+      hipLaunchKernelGGL((saxs::k_spread_zall<4, true>), ga, dim3(saxs::kWG), lds_all, st, p, b);
+    else if (p.order == 4)
+      hipLaunchKernelGGL((saxs::k_spread_zall<4, false>), ga, dim3(saxs::kWG), lds_all, st, p, b);
+    else if (pow2)
+      hipLaunchKernelGGL((saxs::k_spread_zall<6, true>), ga, dim3(saxs::kWG), lds_all, st, p, b);
+    else
+      hipLaunchKernelGGL((saxs::k_spread_zall<6, false>), ga, dim3(saxs::kWG), lds_all, st, p, b);
+  } else if (p.order == 4 && pow2)
     hipLaunchKernelGGL((saxs::k_spread_z<4, true>), gs, dim3(saxs::kWG), lds_s, st, p, b);
   else if (p.order == 4)
     hipLaunchKernelGGL((saxs::k_spread_z<4, false>), gs, dim3(saxs::kWG), lds_s, st, p, b);
@@ -439,6 +466,9 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   tx = std::min(tx, p.n[0]);
   ty = std::min(ty, p.n[1]);
   if (ty % 2) ty = std::max(2, ty - 1);
+  // tile + order - 1 <= n: the in-tile points of any stencil form one run
+  if (tx + d->order - 1 > p.n[0]) tx = p.n[0] - d->order + 1;
+  if (ty + d->order - 1 > p.n[1]) ty = (p.n[1] - d->order + 1) & ~1;
   for (;;) {
     p.tx = tx;
     p.ty = ty;
@@ -470,6 +500,8 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
     return rc;
   }
   pl->lut_cap = d->lut_len;
+  pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
+  pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   pl->max_frames = d->max_frames;
   pl->max_atoms = d->max_atoms;
   pl->nkeys_max = d->max_frames * p.E * p.nty * p.ntx;

@@ -373,6 +373,434 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   }
 }
 
+// ---- all elements of a column tile in one workgroup ----------------------
+
+// In-tile stencil range of one axis: stencil points base + o (mod n), o in
+// [0, P), that fall in [T0, T0 + w); returns [lo, hi) in o and the tile
+// offset of o = 0.
+// A tile that spans the whole axis (w == n) takes every o, with local
+// coordinate (base + o) mod n; otherwise the plan guarantees w + P - 1 <= n,
+// so the in-tile points form one run of o.
+template <int P>
+__device__ __forceinline__ void stencil_range(int m, int n, int T0, int w, int& lo, int& hi,
+                                              int& d) {
+  int base = m - P;
+  if (base < 0) base += n;
+  if (w == n) {
+    d = base;
+    lo = 0;
+    hi = P;
+    return;
+  }
+  d = base - T0;                      // tile-local coordinate of o = 0
+  if (d > n / 2) d -= n;
+  if (d < -n / 2) d += n;
+  lo = max(0, -d);
+  hi = min(P, w - d);
+}
+
+// Adds the in-tile stencil columns of `count` staged atoms to rho.
+template <int P>
+__device__ __forceinline__ void spread_staged(const float4* recs, int count, float* rho,
+                                              const Params& p, int X0, int Y0, int wx, int wy) {
+  const int nz = p.n[2];
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const float4 rec = recs[i];
+    const int mx = cell_index(rec.x, p.n[0]);
+    const int my = cell_index(rec.y, p.n[1]);
+    const int mz = cell_index(rec.z, p.n[2]);
+    int xlo, xhi, dx, ylo, yhi, dy;
+    stencil_range<P>(mx, p.n[0], X0, wx, xlo, xhi, dx);
+    stencil_range<P>(my, p.n[1], Y0, wy, ylo, yhi, dy);
+    if (xlo >= xhi || ylo >= yhi) continue;
+    float thx[P], thy[P], thz[P];
+    bspline<P>(rec.x - (float)mx, thx);
+    bspline<P>(rec.y - (float)my, thy);
+    bspline<P>(rec.z - (float)mz, thz);
+    int gz0 = mz - P;
+    if (gz0 < 0) gz0 += nz;
+    for (int o = xlo; o < xhi; ++o) {
+      float tx = thx[0];
+#pragma unroll
+      for (int t = 1; t < P; ++t) tx = (o == t) ? thx[t] : tx;
+      for (int o2 = ylo; o2 < yhi; ++o2) {
+        float tyv = thy[0];
+#pragma unroll
+        for (int t = 1; t < P; ++t) tyv = (o2 == t) ? thy[t] : tyv;
+        const float fxy = tx * tyv;
+        int lx = dx + o, ly = dy + o2;
+        if (lx >= p.n[0]) lx -= p.n[0];
+        if (ly >= p.n[1]) ly -= p.n[1];
+        float* colp = rho + (lx * p.ty + ly) * nz;
+        if (gz0 + P <= nz) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) atomicAdd(colp + gz0 + q, fxy * thz[q]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            int gz = gz0 + q;
+            if (gz >= nz) gz -= nz;
+            atomicAdd(colp + gz, fxy * thz[q]);
+          }
+        }
+      }
+    }
+  }
+}
+
+constexpr int kPrefetch = 2;                       // records per thread prefetched
+constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
+
+__host__ __device__ inline size_t spread_all_lds_bytes(const Params& p) {
+  const size_t rho = (((size_t)p.tx * p.ty * p.n[2] * sizeof(float)) + 15) & ~size_t(15);
+  size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t recs = (size_t)kAllRecs * sizeof(float4);
+  if (buf < recs) buf = recs;
+  const size_t tw = (size_t)p.nn[2] * sizeof(float2);
+  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  return rho + buf + tw + ((seg + 15) & ~size_t(15));
+}
+
+// Same result as k_spread_z, one workgroup per (column tile, frame) looping
+// over the elements: the bin segments of every element are fetched in one
+// round trip, and while element e is transformed the first records of
+// element e+1 are already in flight (loaded into registers, stored to LDS
+// after the transform).
+template <int P, bool POW2>
+__global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, f = blockIdx.y;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], nnz = p.nn[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int E = p.E;
+  float* rho = reinterpret_cast<float*>(smem);
+  unsigned char* after_rho = smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15));
+  float2* buf = reinterpret_cast<float2*>(after_rho);
+  float4* recs = reinterpret_cast<float4*>(after_rho);
+  size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
+  if (buf_bytes < kAllRecs * sizeof(float4)) buf_bytes = kAllRecs * sizeof(float4);
+  float2* tw = reinterpret_cast<float2*>(after_rho + buf_bytes);
+  int* s_start = reinterpret_cast<int*>(tw + nnz);   // [E][9]
+  int* s_cnt = s_start + 9 * E;                       // [E][9]
+  int* s_pref = s_cnt + 9 * E;                        // [E][10]
+
+  for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
+    const int e = t / 9, sg = t - e * 9;
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = sg % 3, iy = sg / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[t] = st;
+    s_cnt[t] = cnt;
+  }
+  for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[e * 10 + k] = acc;
+      acc += s_cnt[e * 9 + k];
+    }
+    s_pref[e * 10 + 9] = acc;
+  }
+  __syncthreads();
+
+  // record a of element e (a < total of e)
+  auto fetch = [&](int e, int a) -> float4 {
+    const int* pref = s_pref + e * 10;
+    int sgm = 0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
+    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+  };
+
+  // first round of element 0
+  {
+    const int tot0 = s_pref[9];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot0) recs[a] = fetch(0, a);
+    }
+  }
+  float4 pre[kPrefetch];
+  for (int e = 0; e < E; ++e) {
+    const int total = s_pref[e * 10 + 9];
+    for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
+    __syncthreads();
+    // first round (already staged), then any further rounds synchronously
+    spread_staged<P>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy);
+    for (int base = kAllRecs; base < total; base += kAllRecs) {
+      const int chunk = min(kAllRecs, total - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
+      __syncthreads();
+      spread_staged<P>(recs, chunk, rho, p, X0, Y0, wx, wy);
+    }
+    // prefetch the first round of element e+1 into registers
+    const int tot_next = (e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) pre[u] = fetch(e + 1, a);
+    }
+    __syncthreads();
+
+    const size_t fe = (size_t)f * E + e;
+    const double pad = b.pad_sum[fe] * p.inv_count;
+    const float pad_hi = (float)pad;
+    const float pad_lo = (float)(pad - (double)pad_hi);
+    const int ty = p.ty;
+    auto load = [&](unsigned l, unsigned z) -> float2 {
+      const int c0 = 2 * (int)l;
+      const int lx = c0 / ty, ly = c0 - lx * ty;
+      float re = 0.0f, im = 0.0f;
+      if ((int)z < nz && lx < wx) {
+        if (ly < wy) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
+        if (ly + 1 < wy) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
+      }
+      return make_float2(re, im);
+    };
+    fft_lines_loaded<kMaxC, POW2>(buf, p.lsz, nl, p.fz, tw, load);
+    for (int i = threadIdx.x; i < nl * p.kz; i += blockDim.x) {
+      const int l = i / p.kz, k = i - l * p.kz;
+      const int c0 = 2 * l;
+      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      if (lx >= wx || ly >= wy) continue;
+      const float2 Z = buf[l * p.lsz + k];
+      const float2 Zm = buf[l * p.lsz + (k == 0 ? 0 : nnz - k)];
+      const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+      const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+      const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
+      b.zbuf[row * p.kz + k] = Xa;
+      if (ly + 1 < wy) b.zbuf[(row + 1) * p.kz + k] = Xb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) recs[a] = pre[u];
+    }
+  }
+}
+
+// ---- output-pruned z transform (N = 16 * R2, powers of two) --------------
+//
+// Stage 0 (radix 16): butterfly j < R2 takes x[j + R2 r], r < 16, two real
+// columns packed per complex line, writes y[16 j + r].  Stage 1 (length R2):
+// butterfly j < 16 takes y[j + 16 r] W_N^{j r} and evaluates only the kept
+// outputs X(j + 16 r') with k = j + 16 r' in [0, kz) or (N - kz, N), by
+// direct sums over r.  Kept outputs go to a compact array:
+// slot(k) = k for k < kz, kz - 1 + (N - k) for the mirrored ones.
+
+__host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
+  const int N = p.nn[2];
+  const size_t rho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
+  const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
+  const size_t A = ((rho > y ? rho : y) + 15) & ~size_t(15);
+  const size_t xc = (size_t)(p.tx * p.ty / 2) * (2 * p.kz - 1) * sizeof(float2);
+  const size_t recs = (size_t)kAllRecs * sizeof(float4);
+  const size_t B = ((recs > xc ? recs : xc) + 15) & ~size_t(15);
+  const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j r} as [r][j]
+  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  return A + B + tw + seg;
+}
+
+template <int P, int N>
+__global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
+  constexpr int R2 = N / 16;
+  // y element e = 16 a + b is stored at 17 a + b (conflict-free stage-0
+  // writes); line stride LY
+  constexpr int LY = 17 * R2 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, f = blockIdx.y;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int E = p.E, kz = p.kz, nslot = 2 * kz - 1;
+  const size_t rho_b = (size_t)ncol * nz * sizeof(float);
+  const size_t y_b = (size_t)nl * LY * sizeof(float2);
+  const size_t A_b = ((rho_b > y_b ? rho_b : y_b) + 15) & ~size_t(15);
+  const size_t xc_b = (size_t)nl * nslot * sizeof(float2);
+  const size_t rc_b = (size_t)kAllRecs * sizeof(float4);
+  const size_t B_b = ((rc_b > xc_b ? rc_b : xc_b) + 15) & ~size_t(15);
+  float* rho = reinterpret_cast<float*>(smem);
+  float2* ybuf = reinterpret_cast<float2*>(smem);
+  float4* recs = reinterpret_cast<float4*>(smem + A_b);
+  float2* xc = reinterpret_cast<float2*>(smem + A_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b);
+  int* s_start = reinterpret_cast<int*>(tw + N);
+  int* s_cnt = s_start + 9 * E;
+  int* s_pref = s_cnt + 9 * E;
+
+  for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
+    const int e = t / 9, sg = t - e * 9;
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = sg % 3, iy = sg / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[t] = st;
+    s_cnt[t] = cnt;
+  }
+  // stage-1 input twiddles W_N^{j r}, stored [r][j] (j < 16, r < R2)
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int r = i >> 4, j = i & 15;
+    tw[i] = b.twz[j * r];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[e * 10 + k] = acc;
+      acc += s_cnt[e * 9 + k];
+    }
+    s_pref[e * 10 + 9] = acc;
+  }
+  __syncthreads();
+  auto fetch = [&](int e, int a) -> float4 {
+    const int* pref = s_pref + e * 10;
+    int sgm = 0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
+    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+  };
+  {
+    const int tot0 = s_pref[9];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot0) recs[a] = fetch(0, a);
+    }
+  }
+  const bool half = (2 * nz <= N);
+  float4 pre[kPrefetch];
+  for (int e = 0; e < E; ++e) {
+    const int total = s_pref[e * 10 + 9];
+    for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
+    __syncthreads();
+    spread_staged<P>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy);
+    for (int base = kAllRecs; base < total; base += kAllRecs) {
+      const int chunk = min(kAllRecs, total - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
+      __syncthreads();
+      spread_staged<P>(recs, chunk, rho, p, X0, Y0, wx, wy);
+    }
+    const int tot_next = (e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) pre[u] = fetch(e + 1, a);
+    }
+    __syncthreads();
+
+    const size_t fe = (size_t)f * E + e;
+    const double pad = b.pad_sum[fe] * p.inv_count;
+    const float pad_hi = (float)pad;
+    const float pad_lo = (float)(pad - (double)pad_hi);
+    // stage 0: read the columns into registers (y aliases rho), then DFT16
+    constexpr int IT0 = 1;                       // items per thread (nl * R2 <= kWG)
+    float2 v0[IT0][16];
+#pragma unroll
+    for (int it = 0; it < IT0; ++it) {
+      const int item = threadIdx.x + it * kWG;
+      if (item < nl * R2) {
+        const int l = item / R2, j = item - l * R2;
+        const int c0 = 2 * l;
+        const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+        const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int z = j + R2 * r;
+          float re = 0.0f, im = 0.0f;
+          if (z < nz) {
+            if (ok0) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
+            if (ok1) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
+          }
+          v0[it][r] = make_float2(re, im);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT0; ++it) {
+      const int item = threadIdx.x + it * kWG;
+      if (item < nl * R2) {
+        const int l = item / R2, j = item - l * R2;
+        if (half) dft16<true>(v0[it]);
+        else dft16<false>(v0[it]);
+        float2* dst = ybuf + l * LY + 17 * j;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) dst[r] = v0[it][r];
+      }
+    }
+    __syncthreads();
+    // stage 1: kept outputs only
+    for (int item = threadIdx.x; item < nl * 16; item += blockDim.x) {
+      const int l = item >> 4, j = item & 15;
+      float2 t[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const float2 y = ybuf[l * LY + 17 * r + j];
+        t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + j]);
+      }
+      if constexpr (R2 == 8) Radix<8>::dft(t);
+      else if constexpr (R2 == 16) dft16<false>(t);
+      else dft32(t);
+      // keep r' with j + 16 r' < kz or j + 16 r' > N - kz
+      const int lo_end = (kz - j + 15) >> 4;
+      const int hi_beg = (N - kz - j) / 16 + 1;
+#pragma unroll
+      for (int rr = 0; rr < R2; ++rr) {
+        if (rr < lo_end || rr >= hi_beg) {
+          const int k = j + 16 * rr;
+          const int slot = k < kz ? k : kz - 1 + (N - k);
+          xc[l * nslot + slot] = t[rr];
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nl * kz; i += blockDim.x) {
+      const int l = i / kz, k = i - l * kz;
+      const int c0 = 2 * l;
+      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      if (lx >= wx || ly >= wy) continue;
+      const float2 Z = xc[l * nslot + k];
+      const float2 Zm = xc[l * nslot + (k == 0 ? 0 : kz - 1 + k)];
+      const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+      const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+      const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
+      b.zbuf[row * kz + k] = Xa;
+      if (ly + 1 < wy) b.zbuf[(row + 1) * kz + k] = Xb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) recs[a] = pre[u];
+    }
+  }
+}
+
 __host__ __device__ inline size_t yfft_lds_bytes(const Params& p) {
   return (size_t)p.kcy * p.lsy * sizeof(float2) + (size_t)p.nn[1] * sizeof(float2);
 }

@@ -58,7 +58,8 @@ class SaxsEngine:
     d_histogram / d_nhist (saxsKernel.cu:248-254)."""
 
     def __init__(self, spec: SystemSpec, box, max_atoms: int, max_frames: int = 1,
-                 tile: Tuple[int, int] = (0, 0), device: Optional[torch.device] = None):
+                 tile: Tuple[int, int] = (0, 0), device: Optional[torch.device] = None,
+                 flags: int = 0):
         if not torch.cuda.is_available():
             raise RuntimeError("the MI355X SAXS engine needs a ROCm GPU (torch.cuda is unavailable)")
         self.lib = _lib.load()
@@ -91,6 +92,7 @@ class SaxsEngine:
         desc.max_atoms = self.max_atoms
         desc.max_frames = self.max_frames
         desc.tile_x, desc.tile_y = int(tile[0]), int(tile[1])
+        desc.flags = int(flags)
         plan = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.saxs_plan_create(C.byref(desc), C.byref(plan)), "saxs_plan_create")

@@ -17,7 +17,7 @@ REL_TOL = 1e-5
 
 
 def engine_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4, scaled=None,
-                policy="reference", max_frames=None, tile=(0, 0)):
+                policy="reference", max_frames=None, tile=(0, 0), flags=0):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
     symbols, ids = O.element_ids(syms)
@@ -25,7 +25,7 @@ def engine_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4, scal
     spec = SystemSpec(n3, nn, order, tuple(symbols), sigma, dq, qcut, policy)
     xyz = torch.from_numpy(np.stack(frames).astype(np.float32))
     eng = SaxsEngine(spec, box, max_atoms=xyz.shape[1], max_frames=max_frames or len(frames),
-                     tile=tile)
+                     tile=tile, flags=flags)
     eng.forward(xyz, torch.from_numpy(ids))
     torch.cuda.synchronize()
     H, C = eng.H.cpu().numpy(), eng.C.cpu().numpy()
@@ -190,3 +190,44 @@ def test_invalid_element_ids_are_ignored(gpu):
     Ho, Co = O.frame_histogram(plan, pos[keep], ids[keep], symbols, box)
     compare(eng.H.cpu().numpy(), eng.C.cpu().numpy(), Ho, Co, label="invalid ids")
     eng.close()
+
+
+@pytest.mark.parametrize("flags,tile,qcut", [(1, (4, 8), 1.5), (0, (4, 4), 1.5), (1, (2, 8), 1.5),
+                                            (0, (8, 4), 1.5), (2, (4, 8), 1.0), (0, (4, 8), 1.0),
+                                            (0, (4, 4), 0.6), (0, (2, 8), 0.6)])
+def test_spread_variants_and_tiles(gpu, flags, tile, qcut):
+    """Per-element / all-element spreading workgroups, the pruned radix-16 z
+    transform (small qcut) and the full one, several tiles."""
+    box = np.diag([40.0] * 3)
+    base, syms = O.synthetic_system(4000, 40.0, seed=17)
+    frames = [O.frame_positions(base, 17, f, 0.5) for f in range(2)]
+    H, C = engine_hist(frames, syms, box, 64, 2.0, 0.0, qcut, tile=tile, flags=flags)
+    Ho, Co = oracle_hist(frames, syms, box, 64, 2.0, 0.0, qcut)
+    compare(H, C, Ho, Co, label=f"flags={flags} tile={tile} qcut={qcut}")
+
+
+def test_pruned_z_n256(gpu):
+    box = np.diag([60.0] * 3)
+    pos, syms = O.synthetic_system(20000, 60.0, seed=19)
+    H, C = engine_hist([pos], syms, box, 128, 2.0, 0.0, 0.5)
+    Ho, Co = oracle_hist([pos], syms, box, 128, 2.0, 0.0, 0.5)
+    compare(H, C, Ho, Co, label="pruned 128->256")
+
+
+def test_pruned_z_n512_matches_full_transform(gpu):
+    """N = 512 (sigma 4 on a 128 cell): the pruned radix-16 transform against
+    the engine's full Stockham transform of the same frame."""
+    box = np.diag([80.0] * 3)
+    pos, syms = O.synthetic_system(20000, 80.0, seed=20)
+    H1, C1 = engine_hist([pos], syms, box, 128, 4.0, 0.0, 0.4, tile=(4, 4))
+    H2, C2 = engine_hist([pos], syms, box, 128, 4.0, 0.0, 0.4, tile=(4, 4), flags=2)
+    compare(H1, C1, H2, C2, tol=2e-6, label="pruned vs full, N=512")
+
+
+def test_dense_tiles_multiple_rounds(gpu):
+    """More atoms per tile than one staging round holds (512 records)."""
+    box = np.diag([20.0] * 3)
+    pos, syms = O.synthetic_system(60000, 20.0, seed=18)
+    H, C = engine_hist([pos], syms, box, 16, 2.0, 0.0, 100.0)
+    Ho, Co = oracle_hist([pos], syms, box, 16, 2.0, 0.0, 100.0)
+    compare(H, C, Ho, Co, label="dense")
