@@ -60,6 +60,130 @@ def make_inputs(n_atoms=N_ATOMS, frames=FRAMES):
     return xyz, symbols, ids, syms
 
 
+def smooth_at_least(x: float) -> int:
+    """Smallest 2,3,5,7-smooth integer >= x."""
+    v = max(1, int(np.ceil(x)))
+    while True:
+        m = v
+        for p_ in (2, 3, 5, 7):
+            while m % p_ == 0:
+                m //= p_
+        if m == 1:
+            return v
+        v += 1
+
+
+def mix_symbols(rng, n_atoms, mix):
+    counts = {s: int(round(n_atoms * f)) for s, f in mix}
+    counts[mix[0][0]] += n_atoms - sum(counts.values())
+    syms = []
+    for s_, c in counts.items():
+        syms += [s_] * c
+    syms = np.asarray(syms)[rng.permutation(n_atoms)]
+    symbols = sorted(set(syms.tolist()))
+    lut = {s_: i for i, s_ in enumerate(symbols)}
+    return symbols, np.array([lut[s_] for s_ in syms], dtype=np.int32)
+
+
+def batched_uneven_workload(count=32, frames=8, seed=3):
+    """32 systems, atom counts log-uniform in 10k-300k at 0.1 atoms/A^3,
+    cubic grids = smallest 7-smooth >= L/0.75 A clipped to [64, 192],
+    sigma 2, 8 frames each (0.5 A jitter), C1 element mix."""
+    out = []
+    rng = np.random.default_rng(seed)
+    for s_ in range(count):
+        n_atoms = int(round(np.exp(rng.uniform(np.log(1e4), np.log(3e5)))))
+        L = (n_atoms / 0.1) ** (1.0 / 3.0)
+        n = min(192, max(64, smooth_at_least(L / 0.75)))
+        r = np.random.default_rng([seed, s_])
+        base = r.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
+        symbols, ids = mix_symbols(r, n_atoms, MIX)
+        xyz = np.stack([base + np.random.default_rng([seed, s_, f]).normal(0.0, DISP, size=base.shape)
+                        .astype(np.float32) for f in range(frames)])
+        out.append(dict(n_atoms=n_atoms, L=float(np.float32(L)), n=n, symbols=symbols, ids=ids,
+                        xyz=xyz))
+    return out
+
+
+MIX9 = (("H", 0.48), ("C", 0.30), ("O", 0.15), ("N", 0.04), ("S", 0.01), ("P", 0.005),
+        ("Na", 0.005), ("Cl", 0.005), ("K", 0.005))
+
+
+def leg_batched_uneven(torch, steps):
+    from pycusaxs_amd.batch import BatchedSystems, SystemInput
+    from pycusaxs_amd.engine import SystemSpec, resolve_grid
+    from pycusaxs_amd.tuning import Autotuner
+    work = batched_uneven_workload()
+    tuner = Autotuner(frames=2, repeats=1)
+    systems = []
+    for w in work:
+        n3, nn = resolve_grid((w["n"],) * 3, SIGMA)
+        systems.append(SystemInput(SystemSpec(n3, nn, ORDER, tuple(w["symbols"]), SIGMA, DQ, 0.5),
+                                   np.diag([w["L"]] * 3), w["n_atoms"], w["xyz"].shape[0]))
+    t0 = time.perf_counter()
+    bs = BatchedSystems(systems, tuner=tuner)
+    tune_s = time.perf_counter() - t0
+    for i, w in enumerate(work):
+        bs.load(i, w["xyz"], w["ids"])
+    bs.capture()
+    bs.forward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bs.forward()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    frames = bs.frames_total()
+    atoms = sum(w["n_atoms"] for w in work)
+    grids = sorted(set(w["n"] for w in work))
+    bs.close()
+    return {"frames_per_s": round(frames / dt, 2), "ms_per_graph": round(dt * 1e3, 3),
+            "systems": len(work), "frames_per_graph": frames, "atoms_total": atoms,
+            "grids": grids, "setup_incl_autotune_s": round(tune_s, 1),
+            "workload": "32 systems, 10k-300k atoms at 0.1/A^3, grids 64-192 (sigma 2), "
+                        "8 frames each, one hipGraph replay"}
+
+
+def leg_large(torch, steps):
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    n_atoms, L, n, F = 1_000_000, 215.0, 256, 16
+    rng = np.random.default_rng(5)
+    base = rng.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
+    symbols, ids = mix_symbols(rng, n_atoms, MIX9)
+    xyz = torch.empty((F, n_atoms, 3), dtype=torch.float32, device="cuda")
+    for f in range(F):
+        fr = base + np.random.default_rng([5, f]).normal(0.0, DISP, size=base.shape).astype(np.float32)
+        xyz[f].copy_(torch.from_numpy(fr))
+    idt = torch.from_numpy(ids).cuda()
+    n3, nn = resolve_grid((n,) * 3, SIGMA)
+    eng = SaxsEngine(SystemSpec(n3, nn, ORDER, tuple(symbols), SIGMA, 0.0, 0.5),
+                     np.diag([L] * 3), max_atoms=n_atoms, max_frames=F, tile=(4, 4))
+    eng.forward(xyz, idt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.forward(xyz, idt)
+    torch.cuda.synchronize()
+    fwd = steps * F / (time.perf_counter() - t0)
+    Hf, _ = eng.forward_frames(xyz, idt)
+    gH = (torch.randn(Hf.shape, dtype=torch.float64, device="cuda") / Hf.abs().clamp_min(1.0))
+    eng.backward(xyz, idt, gH)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.forward_frames(xyz, idt)
+        eng.backward(xyz, idt, gH)
+    torch.cuda.synchronize()
+    fb = steps * F / (time.perf_counter() - t0)
+    out = {"fwd_frames_per_s": round(fwd, 2), "fwd_bwd_frames_per_s": round(fb, 2),
+           "fwd_bwd_time_over_fwd": round(fwd / fb, 3), "kept": list(eng.kept()),
+           "workspace_GiB": round(eng.workspace_bytes() / 2**30, 2),
+           "workload": "1,000,000 atoms, 215 A box, 9 elements, 256^3 -> 512^3 FFT, "
+                       "q <= 0.5, 16 frames per launch"}
+    eng.close()
+    return out
+
+
 def kernel_bytes(eng, frames_per_launch, n_atoms):
     """Algorithmic HBM bytes per launch of each kernel (float32 complex = 8 B)."""
     kx, ky, kz = eng.kept()
@@ -107,6 +231,7 @@ def main():
     ap.add_argument("--plain-frames", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and 1M-atom legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps only (for rocprofv3), print the line without "
                          "roofline/cpu legs")
@@ -199,6 +324,16 @@ def main():
     line["batched_over_plain"] = round(value / pl_fps, 2)
     print(f"# plain per-frame path: {pl_fps:.1f} frames/s; batched/plain = {value / pl_fps:.1f}x",
           file=sys.stderr, flush=True)
+
+    # batched uneven systems (one hipGraph) and the 1M-atom / 512^3 system
+    if not args.no_extra:
+        for name, fn in (("batched_uneven", leg_batched_uneven), ("large_1M_512", leg_large)):
+            try:
+                line[name] = fn(torch, max(1, min(args.steps, 3)))
+                print(f"# {name}: {line[name]}", file=sys.stderr, flush=True)
+            except Exception as exc:                      # report, keep the main line
+                line[name] = {"error": f"{type(exc).__name__}: {exc}"}
+                print(f"# {name} failed: {exc}", file=sys.stderr, flush=True)
 
     # per-kernel hipEvent times over the same number of steps
     eng.set_timing(True)

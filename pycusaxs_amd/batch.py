@@ -1,0 +1,101 @@
+"""Many independent systems (uneven atom counts, boxes and grids) computed
+together: one engine plan per system, inputs in static device buffers, and
+the launch chains of every system captured once into a single hipGraph that
+is replayed per batch.
+
+The reference has no batching: it computes one frame of one system at a
+time (RunSaxs.cu:154-203).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .engine import SaxsEngine, SystemSpec
+
+
+@dataclass
+class SystemInput:
+    spec: SystemSpec
+    box: np.ndarray            # 3x3, orthorhombic
+    n_atoms: int
+    frames: int
+
+
+class BatchedSystems:
+    """forward() replays one hipGraph covering every system's frames and
+    leaves per-system (H, C) in histograms()."""
+
+    def __init__(self, systems: Sequence[SystemInput], tuner=None):
+        self.systems = list(systems)
+        self.engines: List[SaxsEngine] = []
+        self.xyz: List[torch.Tensor] = []
+        self.elem: List[torch.Tensor] = []
+        self.shapes = []
+        for s in self.systems:
+            tile, flags = (0, 0), 0
+            if tuner is not None:
+                tile, flags = tuner.tune(s.spec, s.n_atoms, s.box)
+            eng = SaxsEngine(s.spec, s.box, max_atoms=s.n_atoms, max_frames=s.frames,
+                             tile=tile, flags=flags)
+            self.engines.append(eng)
+            self.shapes.append((tile, flags))
+            self.xyz.append(torch.zeros((s.frames, s.n_atoms, 3), dtype=torch.float32,
+                                        device=eng.device))
+            self.elem.append(torch.zeros(s.n_atoms, dtype=torch.int32, device=eng.device))
+        self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self._stream = torch.cuda.Stream()
+
+    def load(self, i: int, xyz, elem):
+        """Copies system i's frames [frames, atoms, 3] and element ids into
+        the static input buffers (the graph reads them in place)."""
+        x = torch.as_tensor(xyz, dtype=torch.float32)
+        e = torch.as_tensor(elem, dtype=torch.int32)
+        if tuple(x.shape) != tuple(self.xyz[i].shape) or tuple(e.shape) != tuple(self.elem[i].shape):
+            raise ValueError(f"system {i}: expected {tuple(self.xyz[i].shape)} positions")
+        self.xyz[i].copy_(x)
+        self.elem[i].copy_(e)
+
+    def _enqueue(self):
+        for eng, x, e in zip(self.engines, self.xyz, self.elem):
+            eng.H.zero_()
+            eng.C.zero_()
+            eng.forward(x, e)
+
+    def forward_eager(self):
+        self._enqueue()
+
+    def capture(self):
+        """Captures every system's launch chain into one graph (inputs and
+        accumulators are the static buffers above)."""
+        torch.cuda.synchronize()
+        self._stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self._stream):
+            self._enqueue()                  # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(self._stream)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self._stream):
+            self._enqueue()
+        torch.cuda.synchronize()
+        self.graph = g
+        return g
+
+    def forward(self):
+        if self.graph is None:
+            self.capture()
+        self.graph.replay()
+
+    def histograms(self):
+        return [(eng.H, eng.C) for eng in self.engines]
+
+    def frames_total(self) -> int:
+        return sum(s.frames for s in self.systems)
+
+    def close(self):
+        self.graph = None
+        for eng in self.engines:
+            eng.close()
