@@ -172,7 +172,7 @@ def leg_large(torch, steps):
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.forward_frames(xyz, idt)
-        eng.backward(xyz, idt, gH)
+        eng.backward(xyz, idt, gH, generation=eng.generation)
     torch.cuda.synchronize()
     fb = steps * F / (time.perf_counter() - t0)
     out = {"fwd_frames_per_s": round(fwd, 2), "fwd_bwd_frames_per_s": round(fb, 2),
@@ -297,14 +297,15 @@ def main():
     Hf, _ = eng.forward_frames(xyz, ids)
     gH = (torch.randn(Hf.shape, generator=gen, dtype=torch.float64)
           / Hf.abs().clamp_min(1.0).cpu()).to(dev)
+    # the backward reuses the spectra of the forward it follows (autograd use)
     for _ in range(max(1, args.warmup)):
         eng.forward_frames(xyz, ids)
-        eng.backward(xyz, ids, gH)
+        eng.backward(xyz, ids, gH, generation=eng.generation)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.forward_frames(xyz, ids)
-        eng.backward(xyz, ids, gH)
+        eng.backward(xyz, ids, gH, generation=eng.generation)
     torch.cuda.synchronize()
     fb = frames_done / (time.perf_counter() - t0)
     line["fwd_bwd_frames_per_s"] = round(fb, 2)
@@ -341,7 +342,7 @@ def main():
         eng.forward(xyz, ids)
     torch.cuda.synchronize()
     eng.forward_frames(xyz, ids)
-    eng.backward(xyz, ids, gH)
+    eng.backward(xyz, ids, gH, generation=eng.generation)
     torch.cuda.synchronize()
     times = eng.kernel_times()
     eng.set_timing(False)

@@ -125,6 +125,17 @@ int saxs_backward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_
                   int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
                   void* stream);
 
+/* saxs_backward with options.  SAXS_BWD_REUSE_FORWARD: skip the forward
+ * recomputation and use the sorted atoms and y-pass spectra left in the
+ * plan's workspace by the immediately preceding saxs_forward_frames call on
+ * this plan, which must have had the same positions, elements, n_atoms,
+ * n_frames and box (the caller guarantees it; nothing else may run on the
+ * plan in between). */
+#define SAXS_BWD_REUSE_FORWARD 1
+int saxs_backward_ex(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                     int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
+                     int32_t flags, void* stream);
+
 /* Reference-structured per-frame path: per element, spread into the n^3
  * grid with global atomics, border mean, explicit embed into nn^3, full
  * 3-D R2C FFT, form-factor accumulate, modulus, histogram -- the structure

@@ -18,7 +18,7 @@ EXPORTED = (
     "saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
     "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
     "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version", "saxs_plan_set_timing",
-    "saxs_plan_kernel_times",
+    "saxs_plan_kernel_times", "saxs_backward_ex",
 )
 KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
                 "adj_x", "adj_y", "adj_z", "gather", "plain")
@@ -48,6 +48,7 @@ class SaxsDesc(C.Structure):
 
 FLAG_PER_ELEMENT_SPREAD = 1
 FLAG_NO_PRUNED_Z = 2
+BWD_REUSE_FORWARD = 1
 
 
 _lib = None
@@ -73,6 +74,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_backward_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, vp]
     lib.saxs_forward_plain.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     lib.saxs_plan_workspace_bytes.argtypes = [vp]
     lib.saxs_plan_workspace_bytes.restype = i64
@@ -84,7 +86,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_abi_version.restype = i32
     for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
                  "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy",
-                 "saxs_plan_set_timing", "saxs_plan_kernel_times"):
+                 "saxs_plan_set_timing", "saxs_plan_kernel_times", "saxs_backward_ex"):
         getattr(lib, name).restype = C.c_int
     if lib.saxs_abi_version() != ABI_VERSION:
         raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")

@@ -19,6 +19,7 @@ class _SaxsHistogram(torch.autograd.Function):
         H, C = engine.forward_frames(xyz.detach(), elem)
         ctx.engine = engine
         ctx.elem = elem
+        ctx.generation = engine.generation
         ctx.save_for_backward(xyz.detach())
         ctx.mark_non_differentiable(C)
         return H, C
@@ -28,7 +29,7 @@ class _SaxsHistogram(torch.autograd.Function):
         (xyz,) = ctx.saved_tensors
         if gH is None:
             return None, None, None
-        grad = ctx.engine.backward(xyz, ctx.elem, gH.contiguous())
+        grad = ctx.engine.backward(xyz, ctx.elem, gH.contiguous(), generation=ctx.generation)
         return grad.to(xyz.dtype), None, None
 
 

@@ -156,6 +156,10 @@ struct KernelTimer {
 
 struct saxs_plan {
   KernelTimer timer;
+  // state of the last saxs_forward_frames (for SAXS_BWD_REUSE_FORWARD)
+  bool last_frames_valid = false;
+  const float* last_xyz = nullptr;
+  int last_N = 0, last_F = 0;
   bool spread_all = true;      // one workgroup per tile for all elements
   bool pruned_z = true;        // output-pruned radix-16 z transform where it applies
   saxs::Params p{};            // geometry; F/N/oc set per call
@@ -570,6 +574,7 @@ int saxs_plan_set_tables(saxs_plan* pl, int32_t lut_len, const int32_t* bin_lut,
     pl->p = saved;
     return fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
   }
+  pl->last_frames_valid = false;
   HIP_TRY(hipMemcpy(pl->lut_bin, bin_lut, sizeof(int32_t) * lut_len, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pl->lut_ff, ff_lut, sizeof(float) * lut_len * pl->p.E, hipMemcpyHostToDevice));
   return SAXS_OK;
@@ -582,6 +587,7 @@ int saxs_forward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n
   if (rc) return rc;
   if (!H || !C) return fail(SAXS_EINVAL, "H/C pointer is NULL");
   hipStream_t st = (hipStream_t)stream;
+  pl->last_frames_valid = false;
   rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
   if (rc) return rc;
   const int nblk = pl->p.ky * pl->p.nchx;
@@ -602,6 +608,7 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   if (rc) return rc;
   if (!Hf || !Cf) return fail(SAXS_EINVAL, "Hf/Cf pointer is NULL");
   hipStream_t st = (hipStream_t)stream;
+  pl->last_frames_valid = false;
   rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
   if (rc) return rc;
   const int nblk = pl->p.ky * pl->p.nchx;
@@ -610,6 +617,10 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, n_frames), dim3(saxs::kWG), 0,
                        st, pl->partial, nblk, 1, pl->p.num_bins, Hf, Cf, 1);
   }
+  pl->last_frames_valid = true;
+  pl->last_xyz = xyz;
+  pl->last_N = n_atoms;
+  pl->last_F = n_frames;
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
@@ -617,13 +628,26 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 int saxs_backward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
                   int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
                   void* stream) {
+  return saxs_backward_ex(pl, xyz, elem, n_atoms, n_frames, oc_diag, gH, grad, 0, stream);
+}
+
+int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                     int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
+                     int32_t flags, void* stream) {
   g_err.clear();
   int rc = check_call(pl, xyz, elem, n_atoms, n_frames, oc_diag);
   if (rc) return rc;
   if (!gH || (n_atoms > 0 && !grad)) return fail(SAXS_EINVAL, "gH/grad pointer is NULL");
   hipStream_t st = (hipStream_t)stream;
-  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, false);
-  if (rc) return rc;
+  if (flags & SAXS_BWD_REUSE_FORWARD) {
+    if (!pl->last_frames_valid || pl->last_xyz != xyz || pl->last_N != n_atoms ||
+        pl->last_F != n_frames)
+      return fail(SAXS_EINVAL, "no matching saxs_forward_frames state to reuse");
+  } else {
+    rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, false);
+    if (rc) return rc;
+  }
+  pl->last_frames_valid = false;           // the adjoint overwrites the spectra
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
   saxs::AdjBuffers g{gH, grad, pl->padg};
@@ -727,6 +751,7 @@ extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t
   if (!H || !C) return fail(SAXS_EINVAL, "H/C pointer is NULL");
   rc = plain_setup(pl);
   if (rc) return rc;
+  pl->last_frames_valid = false;
   hipStream_t st = (hipStream_t)stream;
   saxs::Params p = call_params(pl, n_atoms, 1, oc_diag);     // cell geometry
   saxs::Params q = pl->pp;                                   // full grid geometry

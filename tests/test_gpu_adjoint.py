@@ -101,3 +101,41 @@ def test_autograd_chi2_descends(gpu):
     print(f"chi2 {loss.item():.6e} -> {loss2.item():.6e}", flush=True)
     assert loss2.item() < loss.item()
     eng.close()
+
+
+def test_backward_reuse_matches_recompute(gpu):
+    """Reusing the preceding forward_frames state gives the same gradient as
+    recomputing; a forward in between or modified positions fall back."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    L, n = 40.0, 32
+    box = np.diag([L] * 3)
+    base, syms = O.synthetic_system(3000, L, seed=30)
+    fr = np.stack([O.frame_positions(base, 30, f, 0.3) for f in range(2)])
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.9), box, 3000, 2)
+    xyz = torch.from_numpy(fr).cuda()
+    idt = torch.from_numpy(ids).cuda()
+    Hf, _ = eng.forward_frames(xyz, idt)
+    gH = torch.randn(Hf.shape, dtype=torch.float64, device="cuda") / Hf.abs().clamp_min(1.0)
+    g_re = eng.backward(xyz, idt, gH)                       # recompute
+    eng.forward_frames(xyz, idt)
+    g_ru = eng.backward(xyz, idt, gH, generation=eng.generation)   # reuse
+    torch.cuda.synchronize()
+    assert torch.allclose(g_re, g_ru, rtol=1e-5, atol=1e-6 * g_re.abs().max().item())
+    # stale generation -> recompute path, same answer
+    gen = eng.generation
+    eng.forward_frames(xyz, idt)
+    eng.forward(xyz, idt)
+    g_st = eng.backward(xyz, idt, gH, generation=gen)
+    assert torch.allclose(g_re, g_st, rtol=1e-5, atol=1e-6 * g_re.abs().max().item())
+    # positions modified in place after the forward -> recompute on new positions
+    eng.forward_frames(xyz, idt)
+    gen = eng.generation
+    xyz.add_(0.05)
+    g_mod = eng.backward(xyz, idt, gH, generation=gen)
+    g_ref = eng.backward(xyz, idt, gH)
+    torch.cuda.synchronize()
+    assert torch.allclose(g_mod, g_ref, rtol=1e-5, atol=1e-6 * g_ref.abs().max().item())
+    eng.close()
