@@ -1,8 +1,9 @@
 """FP64 CPU oracle of the pyCuSaxs per-frame SAXS scattering path.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``,
-``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
-import it.  The product (``pycusaxs_amd``) never imports or calls it.
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``, the
+smoke check of the repository's build entry module and the CPU-baseline leg
+of ``bench.py`` may import it.  The product (``pycusaxs_amd``) never imports
+or calls it.
 
 It restates the reference's algorithm literally, in float64 where the
 reference computes in float32, keeping the reference's float32 steps only

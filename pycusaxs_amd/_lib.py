@@ -66,7 +66,7 @@ def load(path: Path = LIB_PATH):
     if not Path(path).exists():
         raise ImportError(
             f"MI355X SAXS engine library not built: {path} is missing "
-            "(run __graft_entry__.build() or python -m pycusaxs_amd._build)")
+            "(build it with: python -m pycusaxs_amd._build)")
     lib = C.CDLL(str(path))
     vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
     lib.saxs_plan_create.argtypes = [C.POINTER(SaxsDesc), C.POINTER(vp)]
