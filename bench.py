@@ -184,6 +184,72 @@ def leg_large(torch, steps):
     return out
 
 
+def restraint_workload():
+    """50,000 atoms in an 80 A box, 128^3 cell, sigma 2, order 6; target
+    profile from the structure displaced by N(0, 1 A^2); chi^2 over the
+    requested 100 bins from 0.005 to 0.5 A^-1 with 1 % errors."""
+    rng = np.random.default_rng(4)
+    n_atoms, L = 50_000, 80.0
+    base = rng.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
+    symbols, ids = mix_symbols(rng, n_atoms, MIX)
+    target = (base + np.random.default_rng([4, 1]).normal(0.0, 1.0, size=base.shape)
+              ).astype(np.float32)
+    return dict(base=base, target=target, symbols=symbols, ids=ids, L=L, n=128, order=6,
+                dq=0.005, qcut=0.5)
+
+
+def leg_restraint(torch, steps, keep):
+    from pycusaxs_amd.autograd import chi2, saxs_profile
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    w = restraint_workload()
+    n3, nn = resolve_grid((w["n"],) * 3, SIGMA)
+    box = np.diag([w["L"]] * 3)
+    eng = SaxsEngine(SystemSpec(n3, nn, w["order"], tuple(w["symbols"]), SIGMA, w["dq"],
+                                w["qcut"], "exact"), box, len(w["base"]), 1)
+    idt = torch.from_numpy(w["ids"]).cuda()
+    with torch.no_grad():
+        T, C = saxs_profile(torch.from_numpy(w["target"]).cuda()[None], eng, idt)
+    sel = (C[0] > 0) & (torch.arange(C.shape[1], device=C.device) >= 1)
+    s = 0.01 * T[0].abs() + 1e-30
+    x0 = torch.from_numpy(w["base"]).cuda()[None]
+
+    def step():
+        x = x0.clone().requires_grad_(True)
+        I, _ = saxs_profile(x, eng, idt)
+        loss = chi2(I[0], T[0], s, sel)
+        loss.backward()
+        return loss, x.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    keep.update(T=T[0].cpu().numpy(), s=s.cpu().numpy(), sel=sel.cpu().numpy(),
+                grad=g[0].cpu().numpy(), loss=loss.item(), w=w)
+    eng.close()
+    return {"fwd_bwd_ms": round(ms, 3), "chi2": loss.item(), "bins_used": int(sel.sum().item()),
+            "workload": "50,000 atoms, 80 A, 128^3 -> 256^3, order 6, 100 requested bins "
+                        "0.005-0.5, chi^2 vs displaced (1 A) structure, 1 % errors"}
+
+
+def restraint_grad_check(keep, threads):
+    """Engine chi^2 gradient vs the FP64 oracle gradient on the same target."""
+    from oracle import saxs_oracle as O
+    O.set_fft_workers(threads)
+    w = keep["w"]
+    box = np.diag([w["L"]] * 3)
+    plan = O.make_plan((w["n"],) * 3, box, w["order"], SIGMA, None, w["dq"], w["qcut"], "exact")
+    H, C = O.frame_histogram(plan, w["base"], w["ids"], w["symbols"], box, literal=False)
+    I = np.where(C > 0, H / np.where(C > 0, C, 1), 0.0)
+    r = np.where(keep["sel"], (I - keep["T"]) / keep["s"], 0.0)
+    gH = np.where(C > 0, np.where(keep["sel"], 2 * r / keep["s"], 0.0) / np.where(C > 0, C, 1), 0.0)
+    go = O.grad_linear_H(plan, w["base"], w["ids"], w["symbols"], box, gH)
+    return float(np.linalg.norm(keep["grad"] - go) / np.linalg.norm(go))
+
+
 def kernel_bytes(eng, frames_per_launch, n_atoms):
     """Algorithmic HBM bytes per launch of each kernel (float32 complex = 8 B)."""
     kx, ky, kz = eng.kept()
@@ -328,7 +394,10 @@ def main():
 
     # batched uneven systems (one hipGraph) and the 1M-atom / 512^3 system
     if not args.no_extra:
-        for name, fn in (("batched_uneven", leg_batched_uneven), ("large_1M_512", leg_large)):
+        restraint = {}
+        legs = (("batched_uneven", leg_batched_uneven), ("large_1M_512", leg_large),
+                ("restraint_50k_order6", lambda t, n: leg_restraint(t, n, restraint)))
+        for name, fn in legs:
             try:
                 line[name] = fn(torch, max(1, min(args.steps, 3)))
                 print(f"# {name}: {line[name]}", file=sys.stderr, flush=True)
@@ -378,6 +447,12 @@ def main():
         rel = np.abs(h0[sel] / c0[sel] - o_h[sel] / o_c[sel]) / np.abs(o_h[sel] / o_c[sel])
         line["iq_max_rel_err"] = float(rel.max())
         line["counts_exact"] = bool(np.array_equal(c0, o_c))
+        if not args.no_extra and "grad" in restraint:
+            try:
+                line["restraint_50k_order6"]["grad_rel_l2_vs_fp64"] = restraint_grad_check(
+                    restraint, threads)
+            except Exception as exc:
+                line["restraint_50k_order6"]["grad_check_error"] = f"{type(exc).__name__}: {exc}"
     print(json.dumps(line))
 
 

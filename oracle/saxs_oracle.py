@@ -53,17 +53,19 @@ F32 = np.float32
 # FFT used by the oracle: numpy's pocketfft by default; the CPU baseline in
 # bench.py may switch it to scipy.fft with several worker threads.
 _rfftn = np.fft.rfftn
+_ifftn = np.fft.ifftn
 
 
 def set_fft_workers(workers: int) -> int:
-    """Use scipy.fft.rfftn with this many threads (1 = numpy); returns the
-    number of threads in effect."""
-    global _rfftn
+    """Use scipy.fft with this many threads (1 = numpy); returns the number
+    of threads in effect."""
+    global _rfftn, _ifftn
     if workers <= 1:
-        _rfftn = np.fft.rfftn
+        _rfftn, _ifftn = np.fft.rfftn, np.fft.ifftn
         return 1
     import scipy.fft as sfft
     _rfftn = lambda a: sfft.rfftn(a, workers=workers)  # noqa: E731
+    _ifftn = lambda a: sfft.ifftn(a, workers=workers)  # noqa: E731
     return workers
 
 # Cromer-Mann coefficients {a1..a4, b1..b4, c} for the species the tests use,
@@ -481,7 +483,7 @@ def grad_linear_H(plan, pos, elem_ids, symbols, box, gH):
         # transpose of the half-spectrum map: Re sum_p Fbar(p) e^{+2 pi i p.g/nn}
         full = np.zeros(nn, dtype=np.complex128)
         full[:, :, : Fbar.shape[2]] = Fbar
-        rho_t = np.real(np.fft.ifftn(full) * (nn[0] * nn[1] * nn[2]))[: n[0], : n[1], : n[2]]
+        rho_t = np.real(_ifftn(full) * (nn[0] * nn[1] * nn[2]))[: n[0], : n[1], : n[2]]
         # pad-mean term: rho~ = rho - pad, pad = mean over border z-slabs
         d = plan.border
         mask = np.zeros(n[2], dtype=bool)
