@@ -355,6 +355,270 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
   }
 }
 
+// ---- pruned, all-element variant of the z adjoint (pow2 N = 16 * R2) ------
+//
+// x(z) = Re sum_{k < kz} Zbar(k) e^{+2 pi i k z / N} for two columns at once
+// as conj(FFT(conj S)) with S(k) = (Zbar_a(k) + conj Zbar_a(-k))/2 +
+// i (Zbar_b(k) + conj Zbar_b(-k))/2: stage 0 is a radix-16 DFT over the
+// strided sparse input k = j + R2 r (loaded straight from zbuf), stage 1 a
+// radix-R2 DFT giving z = j' + 16 r'; both in registers, one LDS transpose.
+
+__host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
+  const int N = p.nn[2];
+  const size_t drho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
+  const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
+  const size_t A = ((drho > y ? drho : y) + 15) & ~size_t(15);
+  const size_t B = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
+  const size_t tw = (size_t)N * sizeof(float2);
+  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  return A + B + tw + seg;
+}
+
+template <int P>
+__device__ __forceinline__ void gather_staged(const float4* recs, int count, const float* drho,
+                                              const Params& p, int X0, int Y0, int wx, int wy,
+                                              float* grad, int f) {
+  const int nz = p.n[2];
+  const float sx = (float)p.n[0] * p.oc[0], sy = (float)p.n[1] * p.oc[1],
+              sz = (float)p.n[2] * p.oc[2];
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const float4 rec = recs[i];
+    const int mx = cell_index(rec.x, p.n[0]);
+    const int my = cell_index(rec.y, p.n[1]);
+    const int mz = cell_index(rec.z, p.n[2]);
+    int xlo, xhi, dx, ylo, yhi, dy;
+    stencil_range<P>(mx, p.n[0], X0, wx, xlo, xhi, dx);
+    stencil_range<P>(my, p.n[1], Y0, wy, ylo, yhi, dy);
+    if (xlo >= xhi || ylo >= yhi) continue;
+    const float wxf = rec.x - (float)mx, wyf = rec.y - (float)my, wzf = rec.z - (float)mz;
+    float thx[P], thy[P], thz[P], dtx[P], dty[P], dtz[P];
+    bspline<P>(wxf, thx);
+    bspline<P>(wyf, thy);
+    bspline<P>(wzf, thz);
+    bspline_d<P>(wxf, dtx);
+    bspline_d<P>(wyf, dty);
+    bspline_d<P>(wzf, dtz);
+    int gz0 = mz - P;
+    if (gz0 < 0) gz0 += nz;
+    float gx = 0.0f, gy = 0.0f, gzs = 0.0f;
+    for (int o = xlo; o < xhi; ++o) {
+      float tx = thx[0], dxv = dtx[0];
+#pragma unroll
+      for (int t = 1; t < P; ++t) {
+        tx = (o == t) ? thx[t] : tx;
+        dxv = (o == t) ? dtx[t] : dxv;
+      }
+      for (int o2 = ylo; o2 < yhi; ++o2) {
+        float tyv = thy[0], dyv = dty[0];
+#pragma unroll
+        for (int t = 1; t < P; ++t) {
+          tyv = (o2 == t) ? thy[t] : tyv;
+          dyv = (o2 == t) ? dty[t] : dyv;
+        }
+        int lx = dx + o, ly = dy + o2;
+        if (lx >= p.n[0]) lx -= p.n[0];
+        if (ly >= p.n[1]) ly -= p.n[1];
+        const float* colp = drho + (lx * p.ty + ly) * nz;
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          int gz = gz0 + q;
+          if (gz >= nz) gz -= nz;
+          const float d = colp[gz];
+          s0 += d * thz[q];
+          s1 += d * dtz[q];
+        }
+        gx += dxv * tyv * s0;
+        gy += tx * dyv * s0;
+        gzs += tx * tyv * s1;
+      }
+    }
+    const int idx = __float_as_int(rec.w);
+    float* gp = grad + ((size_t)f * p.N + idx) * 3;
+    atomicAdd(gp + 0, gx * sx);
+    atomicAdd(gp + 1, gy * sy);
+    atomicAdd(gp + 2, gzs * sz);
+  }
+}
+
+template <int P, int N>
+__global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers g) {
+  constexpr int R2 = N / 16;
+  constexpr int LY = 17 * R2 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, f = blockIdx.y;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int E = p.E, kz = p.kz;
+  const size_t drho_b = (size_t)ncol * nz * sizeof(float);
+  const size_t y_b = (size_t)nl * LY * sizeof(float2);
+  const size_t A_b = ((drho_b > y_b ? drho_b : y_b) + 15) & ~size_t(15);
+  const size_t B_b = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
+  float* drho = reinterpret_cast<float*>(smem);
+  float2* ybuf = reinterpret_cast<float2*>(smem);
+  float4* recs = reinterpret_cast<float4*>(smem + A_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b);
+  int* s_start = reinterpret_cast<int*>(tw + N);
+  int* s_cnt = s_start + 9 * E;
+  int* s_pref = s_cnt + 9 * E;
+  __shared__ double s_red[kWG / 64];
+
+  for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
+    const int e = t / 9, sg = t - e * 9;
+    const int4 sxv = b.srcx[bx], syv = b.srcy[by];
+    const int ix = sg % 3, iy = sg / 3;
+    int st = 0, cnt = 0;
+    if (ix < sxv.x && iy < syv.x) {
+      const int kx = ix == 0 ? sxv.y : (ix == 1 ? sxv.z : sxv.w);
+      const int ky = iy == 0 ? syv.y : (iy == 1 ? syv.z : syv.w);
+      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[t] = st;
+    s_cnt[t] = cnt;
+  }
+  // stage-1 twiddles W_N^{j r} stored [r][j]
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int r = i >> 4, j = i & 15;
+    tw[i] = b.twz[j * r];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[e * 10 + k] = acc;
+      acc += s_cnt[e * 9 + k];
+    }
+    s_pref[e * 10 + 9] = acc;
+  }
+  __syncthreads();
+  auto fetch = [&](int e, int a) -> float4 {
+    const int* pref = s_pref + e * 10;
+    int sgm = 0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
+    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+  };
+
+  for (int e = 0; e < E; ++e) {
+    const size_t fe = (size_t)f * E + e;
+    const int total = s_pref[e * 10 + 9];
+    // records of this element: loads in flight during the transform
+    float4 pre[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < total) pre[u] = fetch(e, a);
+    }
+    // stage 0: sparse conj(S) straight from zbuf, radix-16 DFT
+    {
+      const int item = threadIdx.x;
+      if (item < nl * R2) {
+        const int l = item / R2, j = item - l * R2;
+        const int c0 = 2 * l;
+        const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+        const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+        const size_t row = ok0 ? (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly) : 0;
+        const float2* za = b.zbuf + row * kz;
+        const float2* zb = za + kz;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int k = j + R2 * r;
+          float2 S = make_float2(0.0f, 0.0f);
+          if (ok0) {
+            if (k == 0 || 2 * k == N) {
+              if (k < kz) {
+                const float2 a = za[k];
+                const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
+                S = make_float2(a.x, c.x);
+              }
+            } else {
+              if (k < kz) {
+                const float2 a = za[k];
+                const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
+                S.x += 0.5f * (a.x - c.y);
+                S.y += 0.5f * (a.y + c.x);
+              }
+              if (N - k < kz) {
+                const float2 a = za[N - k];
+                const float2 c = ok1 ? zb[N - k] : make_float2(0.0f, 0.0f);
+                S.x += 0.5f * (a.x + c.y);
+                S.y += 0.5f * (c.x - a.y);
+              }
+            }
+          }
+          v[r] = make_float2(S.x, -S.y);              // conj
+        }
+        dft16<false>(v);
+        float2* dst = ybuf + l * LY + 17 * j;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) dst[r] = v[r];
+      }
+    }
+    __syncthreads();
+    // stage 1: radix-R2 DFT; x = conj(out): column a = Re, column b = -Im
+    float2 t[R2];
+    const int item1 = threadIdx.x;
+    const bool act1 = item1 < nl * 16;
+    const int l1 = item1 >> 4, j1 = item1 & 15;
+    if (act1) {
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const float2 y = ybuf[l1 * LY + 17 * r + j1];
+        t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + j1]);
+      }
+      if constexpr (R2 == 8) Radix<8>::dft(t);
+      else if constexpr (R2 == 16) dft16<false>(t);
+      else dft32(t);
+    }
+    __syncthreads();                                  // drho aliases ybuf
+    double part = 0.0;
+    if (act1) {
+      const int c0 = 2 * l1;
+      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+#pragma unroll
+      for (int rr = 0; rr < R2; ++rr) {
+        const int z = j1 + 16 * rr;
+        if (z < nz) {
+          const float a = ok0 ? t[rr].x : 0.0f, c = ok1 ? -t[rr].y : 0.0f;
+          drho[c0 * nz + z] = a;
+          drho[(c0 + 1) * nz + z] = c;
+          part += (double)a + (double)c;
+        }
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < total) recs[a] = pre[u];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
+      atomicAdd(&g.padg[fe], sum);
+    }
+    gather_staged<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, g.grad, f);
+    for (int base = kAllRecs; base < total; base += kAllRecs) {
+      const int chunk = min(kAllRecs, total - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
+      __syncthreads();
+      gather_staged<P>(recs, chunk, drho, p, X0, Y0, wx, wy, g.grad, f);
+    }
+    __syncthreads();
+  }
+}
+
 // Bulk-solvent mean term: dL/dx_j -= (padg_e / count) grad(border weight of
 // atom j), border weight = (sum th_x)(sum th_y)(sum over border z of th_z).
 template <int P>

@@ -673,7 +673,22 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     else
       hipLaunchKernelGGL(saxs::k_adj_y<false>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b);
   }
-  {
+  const int Nz = p.nn[2];
+  const size_t lds_ap = saxs::adj_p_lds_bytes(p);
+  const bool adj_pruned = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
+                          2 * p.kz - 1 <= Nz / 4 &&
+                          (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_ap <= kLdsLimit;
+  if (adj_pruned) {
+    Probe pr(pl, SAXS_K_ADJ_Z, st);
+    dim3 ga(p.ntx * p.nty, n_frames);
+#define SAXS_AZP(PP, NN) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g)
+    if (p.order == 4) {
+      if (Nz == 128) SAXS_AZP(4, 128); else if (Nz == 256) SAXS_AZP(4, 256); else SAXS_AZP(4, 512);
+    } else {
+      if (Nz == 128) SAXS_AZP(6, 128); else if (Nz == 256) SAXS_AZP(6, 256); else SAXS_AZP(6, 512);
+    }
+#undef SAXS_AZP
+  } else {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 gz(p.ntx * p.nty, p.E, n_frames);
     const size_t lds = saxs::adjz_lds_bytes(p);

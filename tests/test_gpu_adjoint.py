@@ -15,7 +15,7 @@ REL_L2 = 2e-4
 
 
 def run_case(n, L, n_atoms, order=4, qcut=100.0, sigma=2.0, frames=1, seed=0, dq=0.0,
-             policy="reference"):
+             policy="reference", flags=0, tile=(0, 0)):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
     box = np.diag([L] * 3)
@@ -24,7 +24,7 @@ def run_case(n, L, n_atoms, order=4, qcut=100.0, sigma=2.0, frames=1, seed=0, dq
     symbols, ids = O.element_ids(syms)
     n3, nn = resolve_grid((n,) * 3, sigma)
     spec = SystemSpec(n3, nn, order, tuple(symbols), sigma, dq, qcut, policy)
-    eng = SaxsEngine(spec, box, max_atoms=n_atoms, max_frames=frames)
+    eng = SaxsEngine(spec, box, max_atoms=n_atoms, max_frames=frames, flags=flags, tile=tile)
     xyz = torch.from_numpy(np.stack(fr))
     idt = torch.from_numpy(ids)
     Hf, Cf = eng.forward_frames(xyz, idt)
@@ -63,6 +63,18 @@ def test_adjoint_mixed_radix(gpu):
 
 def test_adjoint_batched_frames(gpu):
     run_case(32, 40.0, 2000, qcut=0.9, frames=3, seed=4)
+
+
+@pytest.mark.parametrize("order,flags,tile", [(4, 0, (0, 0)), (6, 0, (0, 0)), (4, 2, (0, 0)),
+                                              (4, 0, (4, 4)), (6, 1, (0, 0))])
+def test_adjoint_pruned_pow2(gpu, order, flags, tile):
+    """FFT grid 128 with a small q range: the all-element pruned z adjoint
+    (flags 0) and the per-element / full-transform variants."""
+    run_case(64, 40.0, 4000, order=order, qcut=0.6, frames=2, seed=6, flags=flags, tile=tile)
+
+
+def test_adjoint_pruned_n256(gpu):
+    run_case(128, 60.0, 20000, qcut=0.5, seed=8)
 
 
 def test_adjoint_exact_bins(gpu):
