@@ -267,8 +267,16 @@ def kernel_bytes(eng, frames_per_launch, n_atoms):
         "adj_x": F * 2 * yb,
         "adj_y": F * (yb + zb),
         "adj_z": F * (zb + 16.0 * n_atoms + 12.0 * n_atoms),
-        "gather": F * n_atoms * (12 + 4 + 24),
+        "pad_grad": F * n_atoms * (12 + 12 + 12),   # xyz in, grad read-modify-write
     }
+
+
+def fixed_grad_H(torch, eng, xyz, ids, dev):
+    """A fixed, seeded dL/dH for the backward legs (scaled by 1/|H|)."""
+    gen = torch.Generator(device="cpu").manual_seed(SEED)
+    Hf, _ = eng.forward_frames(xyz, ids)
+    return (torch.randn(Hf.shape, generator=gen, dtype=torch.float64)
+            / Hf.abs().clamp_min(1.0).cpu()).to(dev)
 
 
 def cpu_baseline(xyz, symbols, ids, frames, threads):
@@ -298,9 +306,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and 1M-atom legs")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-frame plain leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps only (for rocprofv3), print the line without "
                          "roofline/cpu legs")
+    ap.add_argument("--profile-backward", action="store_true",
+                    help="with --profile-only: also run forward_frames + backward per step")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="with --profile-only: also run one elementwise copy of this many MiB "
+                         "(known HBM bytes, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this engine's "
@@ -355,14 +369,21 @@ def main():
                    "n_atoms": N_ATOMS, "grid": GRID, "fft_grid": nn[0]},
     }
     if args.profile_only:
+        if args.profile_backward:
+            gH = fixed_grad_H(torch, eng, xyz, ids, dev)
+            for _ in range(args.steps):
+                eng.forward_frames(xyz, ids)
+                eng.backward(xyz, ids, gH, generation=eng.generation)
+        if args.copy_calibration_mib > 0:
+            src = torch.ones(args.copy_calibration_mib * 2**18, dtype=torch.float32, device=dev)
+            dst = torch.empty_like(src)
+            torch.mul(src, 1.0, out=dst)        # reads and writes exactly 4 B per element
+        torch.cuda.synchronize()
         print(json.dumps(line))
         return
 
     # forward + backward (per-frame histograms, then dL/dx for a fixed dL/dH)
-    gen = torch.Generator(device="cpu").manual_seed(SEED)
-    Hf, _ = eng.forward_frames(xyz, ids)
-    gH = (torch.randn(Hf.shape, generator=gen, dtype=torch.float64)
-          / Hf.abs().clamp_min(1.0).cpu()).to(dev)
+    gH = fixed_grad_H(torch, eng, xyz, ids, dev)
     # the backward reuses the spectra of the forward it follows (autograd use)
     for _ in range(max(1, args.warmup)):
         eng.forward_frames(xyz, ids)
@@ -380,17 +401,18 @@ def main():
           flush=True)
 
     # reference-structured per-frame path (per element, full-spectrum FFT)
-    n_plain = min(args.plain_frames, args.frames)
-    eng.forward_plain(xyz[:1], ids)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.forward_plain(xyz[:n_plain], ids)
-    torch.cuda.synchronize()
-    pl_fps = n_plain / (time.perf_counter() - t0)
-    line["plain_frames_per_s"] = round(pl_fps, 2)
-    line["batched_over_plain"] = round(value / pl_fps, 2)
-    print(f"# plain per-frame path: {pl_fps:.1f} frames/s; batched/plain = {value / pl_fps:.1f}x",
-          file=sys.stderr, flush=True)
+    if not args.no_plain:
+        n_plain = min(args.plain_frames, args.frames)
+        eng.forward_plain(xyz[:1], ids)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.forward_plain(xyz[:n_plain], ids)
+        torch.cuda.synchronize()
+        pl_fps = n_plain / (time.perf_counter() - t0)
+        line["plain_frames_per_s"] = round(pl_fps, 2)
+        line["batched_over_plain"] = round(value / pl_fps, 2)
+        print(f"# plain per-frame path: {pl_fps:.1f} frames/s; batched/plain = "
+              f"{value / pl_fps:.1f}x", file=sys.stderr, flush=True)
 
     # batched uneven systems (one hipGraph) and the 1M-atom / 512^3 system
     if not args.no_extra:

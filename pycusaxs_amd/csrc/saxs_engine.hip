@@ -702,7 +702,7 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
       hipLaunchKernelGGL((saxs::k_adj_z<6, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
   }
   if (n_atoms > 0) {
-    Probe pr(pl, SAXS_K_GATHER, st);
+    Probe pr(pl, SAXS_K_PAD_GRAD, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
     if (p.order == 4)
       hipLaunchKernelGGL(saxs::k_pad_grad<4>, ga, dim3(saxs::kWG), 0, st, p, b, g);

@@ -448,6 +448,90 @@ __device__ __forceinline__ void spread_staged(const float4* recs, int count, flo
   }
 }
 
+// LDS column stride of the pruned spread: room for the z stencil to run
+// P - 1 points past nz (folded back onto z < P - 1 when the column is read),
+// rounded to 8 (mod 16) so that the two complex lines a half-wave reads in
+// the first transform stage sit 16 banks apart.
+template <int P>
+__host__ __device__ constexpr int spread_col_stride(int nz) {
+  return ((nz + P - 1 + 7) / 16) * 16 + 8;
+}
+
+// Fixed-point accumulation of the spread: 2^40 units per unit weight in an
+// unsigned 64-bit integer.  LDS float atomics run at a small fraction of the
+// integer rate on gfx950; integer sums are also exact and independent of
+// the order of the atomics.  Contributions are B-spline products in [0, 1]:
+// v * 2^40 is an exact integer for v >= 2^-17 (24-bit mantissa), smaller
+// ones are truncated (error < 2^-40).  A column sum overflows only past
+// 2^24 units of weight.
+constexpr double kFixScale = 1099511627776.0;      // 2^40
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  const float s = v * 256.0f;                      // exact
+  const float hi = floorf(s);
+  const float lo = (s - hi) * 4294967296.0f;       // exact, < 2^32
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
+}
+
+// Same contribution as spread_staged into columns of stride cs with a z
+// halo (no z wrap), accumulated in fixed point (to_fixed): the stencil is
+// unrolled and predicated on the in-tile ranges, plane and column offsets
+// are computed once per record, and LPR lanes share a record, lane s taking
+// the x planes o = s, s + LPR, ...
+template <int P, int LPR>
+__device__ __forceinline__ void spread_cols(const float4* recs, int count,
+                                            unsigned long long* rho, const Params& p, int X0,
+                                            int Y0, int wx, int wy, int cs) {
+  const int nz = p.n[2];
+  const int plane = p.ty * cs;
+  const int sub = threadIdx.x % LPR;
+  for (int i = threadIdx.x / LPR; i < count; i += blockDim.x / LPR) {
+    const float4 rec = recs[i];
+    const int mx = cell_index(rec.x, p.n[0]);
+    const int my = cell_index(rec.y, p.n[1]);
+    const int mz = cell_index(rec.z, p.n[2]);
+    int xlo, xhi, dx, ylo, yhi, dy;
+    stencil_range<P>(mx, p.n[0], X0, wx, xlo, xhi, dx);
+    stencil_range<P>(my, p.n[1], Y0, wy, ylo, yhi, dy);
+    if (xlo >= xhi || ylo >= yhi) continue;
+    float thx[P], thy[P], thz[P];
+    bspline<P>(rec.x - (float)mx, thx);
+    bspline<P>(rec.y - (float)my, thy);
+    bspline<P>(rec.z - (float)mz, thz);
+    int gz0 = mz - P;
+    if (gz0 < 0) gz0 += nz;
+    int yo[P];
+#pragma unroll
+    for (int o2 = 0; o2 < P; ++o2) {
+      int ly = dy + o2;
+      if (ly >= p.n[1]) ly -= p.n[1];
+      yo[o2] = ly * cs + gz0;
+    }
+#pragma unroll
+    for (int oo = 0; oo < P; oo += LPR) {
+      const int o = oo + sub;
+      if (o >= P || o < xlo || o >= xhi) continue;
+      int lx = dx + o;
+      if (lx >= p.n[0]) lx -= p.n[0];
+      unsigned long long* xp = rho + lx * plane;
+      const float fx = thx[o];
+#pragma unroll
+      for (int o2 = 0; o2 < P; ++o2) {
+        if (o2 < ylo || o2 >= yhi) continue;
+        unsigned long long* c = xp + yo[o2];
+        const float f = fx * thy[o2];
+#pragma unroll
+        for (int q = 0; q < P; ++q) atomicAdd(c + q, to_fixed(f * thz[q]));
+      }
+    }
+  }
+}
+
+#ifndef SAXS_SPREAD_LANES
+#define SAXS_SPREAD_LANES 1
+#endif
+constexpr int kSpreadLanes = SAXS_SPREAD_LANES;    // lanes per record in spread_cols
 constexpr int kPrefetch = 2;                       // records per thread prefetched
 constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
 
@@ -605,7 +689,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
 
 __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
-  const size_t rho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
+  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
+  const size_t rho = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
   const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
   const size_t A = ((rho > y ? rho : y) + 15) & ~size_t(15);
   const size_t xc = (size_t)(p.tx * p.ty / 2) * (2 * p.kz - 1) * sizeof(float2);
@@ -631,13 +716,14 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E, kz = p.kz, nslot = 2 * kz - 1;
-  const size_t rho_b = (size_t)ncol * nz * sizeof(float);
+  const int cs = spread_col_stride<P>(nz);
+  const size_t rho_b = (size_t)ncol * cs * sizeof(unsigned long long);
   const size_t y_b = (size_t)nl * LY * sizeof(float2);
   const size_t A_b = ((rho_b > y_b ? rho_b : y_b) + 15) & ~size_t(15);
   const size_t xc_b = (size_t)nl * nslot * sizeof(float2);
   const size_t rc_b = (size_t)kAllRecs * sizeof(float4);
   const size_t B_b = ((rc_b > xc_b ? rc_b : xc_b) + 15) & ~size_t(15);
-  float* rho = reinterpret_cast<float*>(smem);
+  unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
   float2* xc = reinterpret_cast<float2*>(smem + A_b);
@@ -695,15 +781,15 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   float4 pre[kPrefetch];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
-    for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
+    for (int i = threadIdx.x; i < ncol * cs; i += blockDim.x) rho[i] = 0ull;
     __syncthreads();
-    spread_staged<P>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy);
+    spread_cols<P, kSpreadLanes>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy, cs);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
       const int chunk = min(kAllRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
-      spread_staged<P>(recs, chunk, rho, p, X0, Y0, wx, wy);
+      spread_cols<P, kSpreadLanes>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
     }
     const int tot_next = (e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0;
 #pragma unroll
@@ -715,8 +801,6 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 
     const size_t fe = (size_t)f * E + e;
     const double pad = b.pad_sum[fe] * p.inv_count;
-    const float pad_hi = (float)pad;
-    const float pad_lo = (float)(pad - (double)pad_hi);
     // stage 0: read the columns into registers (y aliases rho), then DFT16
     constexpr int IT0 = 1;                       // items per thread (nl * R2 <= kWG)
     float2 v0[IT0][16];
@@ -733,8 +817,13 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
           const int z = j + R2 * r;
           float re = 0.0f, im = 0.0f;
           if (z < nz) {
-            if (ok0) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
-            if (ok1) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
+            unsigned long long r0 = rho[c0 * cs + z], r1 = rho[(c0 + 1) * cs + z];
+            if (z < P - 1) {                   // fold the z halo back
+              r0 += rho[c0 * cs + nz + z];
+              r1 += rho[(c0 + 1) * cs + nz + z];
+            }
+            if (ok0) re = (float)((double)r0 * kFixInv - pad);
+            if (ok1) im = (float)((double)r1 * kFixInv - pad);
           }
           v0[it][r] = make_float2(re, im);
         }
