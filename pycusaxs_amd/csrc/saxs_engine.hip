@@ -183,7 +183,7 @@ struct saxs_plan {
   int* offsets = nullptr;
   int2* keyslot = nullptr;
   float4* sorted = nullptr;
-  double* pad_sum = nullptr;
+  unsigned long long* pad_sum = nullptr;
   float2* zbuf = nullptr;
   float2* ybuf = nullptr;
   double* partial = nullptr;

@@ -53,7 +53,7 @@ struct Buffers {
   int* offsets;          // [F*E*nty*ntx + 1]
   int2* keyslot;         // [F*N]
   float4* sorted;        // [F*N]  (r_x, r_y, r_z, atom index bits)
-  double* pad_sum;       // [F*E]
+  unsigned long long* pad_sum;   // [F*E] border-slab weight, fixed point (to_fixed_d)
   const int4* srcx;      // [ntx] {count, bin0, bin1, bin2}
   const int4* srcy;      // [nty]
   float2* zbuf;
@@ -122,20 +122,40 @@ __device__ __forceinline__ int wrap_base(int m, int P, int n) {
   return b < 0 ? b + n : b;
 }
 
+// Fixed-point accumulation: 2^32 units per unit weight in an unsigned
+// 64-bit integer, used for the spread columns and the border-slab sums.
+// LDS float atomics run at a small fraction of the integer rate on gfx950;
+// integer sums are also exact and independent of the order of the atomics
+// (and of the column tiling).  Spread contributions are B-spline
+// products th_x th_y th_z <= (2/3)^3 < 1, so v * 2^32 fits 32 bits; it is an
+// exact integer for v >= 2^-9 (24-bit mantissa) and truncated below (error
+// < 2^-32, under a tenth of a float32 ulp of a typical column value).  A sum
+// overflows only past 2^32 units of weight.
+constexpr double kFixInv = 1.0 / 4294967296.0;     // 2^-32
+
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  return (unsigned long long)(unsigned)(v * 4294967296.0f);
+}
+
+// Same scale for a weight in [0, 1] held in double (border-slab shares).
+__device__ __forceinline__ unsigned long long to_fixed_d(double v) {
+  return (unsigned long long)(v * 4294967296.0);
+}
+
 // ------------------------------------------------------------------ kernels --
 
-__global__ void k_clear(int* counts, int nkeys, double* pad_sum, int npad) {
+__global__ void k_clear(int* counts, int nkeys, unsigned long long* pad_sum, int npad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nkeys) counts[i] = 0;
-  if (i < npad) pad_sum[i] = 0.0;
+  if (i < npad) pad_sum[i] = 0ull;
 }
 
 template <int P>
 __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
-  __shared__ double padacc[kMaxElements];
+  __shared__ unsigned long long padacc[kMaxElements];
   const int f = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0.0;
+  for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0ull;
   __syncthreads();
   const int e = i < p.N ? b.elem[i] : -1;
   if (i < p.N && (e < 0 || e >= p.E)) b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
@@ -170,12 +190,12 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
       if (z >= p.n[2]) z -= p.n[2];
       if (z <= p.border || z >= p.n[2] - p.border) sz += (double)th[o];
     }
-    atomicAdd(&padacc[e], sx * sy * sz);
+    atomicAdd(&padacc[e], to_fixed_d(sx * sy * sz));
   }
   __syncthreads();
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
-    const double v = padacc[e];
-    if (v != 0.0) atomicAdd(&b.pad_sum[f * p.E + e], v);
+    const unsigned long long v = padacc[e];
+    if (v != 0ull) atomicAdd(&b.pad_sum[f * p.E + e], v);
   }
 }
 
@@ -339,7 +359,7 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   // mean subtracted inside the cell (split in two floats so that its
   // rounding leaves no systematic box-shaped residue), zero padding beyond
   // n_z.  The first FFT stage reads the columns directly.
-  const double pad = b.pad_sum[f * p.E + e] * p.inv_count;
+  const double pad = (double)b.pad_sum[f * p.E + e] * kFixInv * p.inv_count;
   const float pad_hi = (float)pad;
   const float pad_lo = (float)(pad - (double)pad_hi);
   const int ty = p.ty;
@@ -457,22 +477,6 @@ __host__ __device__ constexpr int spread_col_stride(int nz) {
   return ((nz + P - 1 + 7) / 16) * 16 + 8;
 }
 
-// Fixed-point accumulation of the spread: 2^40 units per unit weight in an
-// unsigned 64-bit integer.  LDS float atomics run at a small fraction of the
-// integer rate on gfx950; integer sums are also exact and independent of
-// the order of the atomics.  Contributions are B-spline products in [0, 1]:
-// v * 2^40 is an exact integer for v >= 2^-17 (24-bit mantissa), smaller
-// ones are truncated (error < 2^-40).  A column sum overflows only past
-// 2^24 units of weight.
-constexpr double kFixScale = 1099511627776.0;      // 2^40
-constexpr double kFixInv = 1.0 / 1099511627776.0;
-
-__device__ __forceinline__ unsigned long long to_fixed(float v) {
-  const float s = v * 256.0f;                      // exact
-  const float hi = floorf(s);
-  const float lo = (s - hi) * 4294967296.0f;       // exact, < 2^32
-  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
-}
 
 // Same contribution as spread_staged into columns of stride cs with a z
 // halo (no z wrap), accumulated in fixed point (to_fixed): the stencil is
@@ -641,7 +645,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
     __syncthreads();
 
     const size_t fe = (size_t)f * E + e;
-    const double pad = b.pad_sum[fe] * p.inv_count;
+    const double pad = (double)b.pad_sum[fe] * kFixInv * p.inv_count;
     const float pad_hi = (float)pad;
     const float pad_lo = (float)(pad - (double)pad_hi);
     const int ty = p.ty;
@@ -781,7 +785,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   float4 pre[kPrefetch];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
-    for (int i = threadIdx.x; i < ncol * cs; i += blockDim.x) rho[i] = 0ull;
+    for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)   // cs is even
+      reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     spread_cols<P, kSpreadLanes>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy, cs);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
@@ -800,7 +805,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     __syncthreads();
 
     const size_t fe = (size_t)f * E + e;
-    const double pad = b.pad_sum[fe] * p.inv_count;
+    const double pad = (double)b.pad_sum[fe] * kFixInv * p.inv_count;
     // stage 0: read the columns into registers (y aliases rho), then DFT16
     constexpr int IT0 = 1;                       // items per thread (nl * R2 <= kWG)
     float2 v0[IT0][16];
