@@ -256,6 +256,8 @@ def kernel_bytes(eng, frames_per_launch, n_atoms):
     n = eng.spec.n
     E = len(eng.spec.symbols)
     F = frames_per_launch
+    P = eng.spec.order
+    vis = (4 + P - 1) / 4 * (8 + P - 1) / 8  # tile visits per atom, default 4 x 8 column tile
     zb = 8.0 * E * n[0] * n[1] * kz          # z-pass output per frame
     yb = 8.0 * E * n[0] * ky * kz            # y-pass output per frame
     return {
@@ -266,8 +268,10 @@ def kernel_bytes(eng, frames_per_launch, n_atoms):
         "xfft_epi": F * yb,
         "adj_x": F * 2 * yb,
         "adj_y": F * (yb + zb),
-        "adj_z": F * (zb + 16.0 * n_atoms + 12.0 * n_atoms),
-        "pad_grad": F * n_atoms * (12 + 12 + 12),   # xyz in, grad read-modify-write
+        # spectrum in, staged records in, one 12 B partial per tile visit
+        "adj_z": F * (zb + 16.0 * n_atoms + vis * 12.0 * n_atoms),
+        # xyz in, visited partials in, grad out
+        "grad_sum": F * n_atoms * (12 + vis * 12 + 12),
     }
 
 

@@ -157,7 +157,7 @@ int saxs_forward_plain(saxs_plan* plan, const float* xyz, const int32_t* elem,
 #define SAXS_K_ADJ_X 8
 #define SAXS_K_ADJ_Y 9
 #define SAXS_K_ADJ_Z 10
-#define SAXS_K_PAD_GRAD 11
+#define SAXS_K_GRAD_SUM 11
 #define SAXS_K_PLAIN 12
 #define SAXS_NKERNELS 13
 

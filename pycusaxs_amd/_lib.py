@@ -21,7 +21,7 @@ EXPORTED = (
     "saxs_plan_kernel_times", "saxs_backward_ex",
 )
 KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
-                "adj_x", "adj_y", "adj_z", "pad_grad", "plain")
+                "adj_x", "adj_y", "adj_z", "grad_sum", "plain")
 
 
 class SaxsDesc(C.Structure):

@@ -27,7 +27,24 @@ struct AdjBuffers {
   const double* gH;     // [F][num_bins]
   float* grad;          // [F][N][3]
   double* padg;         // [F][E] sum over the cell of drho~
+  float* gpart;         // [F][nsx*nsy][N][3] per-tile partial gradients
+  int nsx, nsy;         // tile offsets an atom's stencil reaches along x, y
 };
+
+// Slot of the partial gradient that tile (bx, by) writes for an atom in cell
+// (mx, my): the tile's offset from the atom's home tile (the tile holding
+// stencil point o = 0), ox + nsx oy.  Every (frame, slot, atom) has one
+// writer, so the gather stores instead of adding atomically and the sum
+// over slots (k_grad_finalize) is in a fixed order.
+template <int P>
+__device__ __forceinline__ int partial_slot(int mx, int my, int bx, int by, const Params& p,
+                                            const AdjBuffers& g) {
+  int ox = bx - wrap_base(mx, P, p.n[0]) / p.tx;
+  int oy = by - wrap_base(my, P, p.n[1]) / p.ty;
+  if (ox < 0) ox += p.ntx;
+  if (oy < 0) oy += p.nty;
+  return ox + g.nsx * oy;
+}
 
 __device__ __forceinline__ int kept_to_index(int j, int k, int K, int N) {
   return (k == N) ? j : (j <= K ? j : N - (k - j));
@@ -346,10 +363,11 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
       }
       if (hit) {
         const int idx = __float_as_int(rec.w);
-        float* gp = g.grad + ((size_t)f * p.N + idx) * 3;
-        atomicAdd(gp + 0, gx * sx);
-        atomicAdd(gp + 1, gy * sy);
-        atomicAdd(gp + 2, gzs * sz);
+        const int slot = partial_slot<P>(mx, my, bx, by, p, g);
+        float* gp = g.gpart + (((size_t)f * g.nsx * g.nsy + slot) * p.N + idx) * 3;
+        gp[0] = gx * sx;
+        gp[1] = gy * sy;
+        gp[2] = gzs * sz;
       }
     }
   }
@@ -365,7 +383,8 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
 
 __host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
-  const size_t drho = (size_t)p.tx * p.ty * p.n[2] * sizeof(float);
+  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
+  const size_t drho = (size_t)p.tx * p.ty * cs * sizeof(float);
   const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
   const size_t A = ((drho > y ? drho : y) + 15) & ~size_t(15);
   const size_t B = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
@@ -374,11 +393,18 @@ __host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
   return A + B + tw + seg;
 }
 
+// Adjoint of spread_cols: for each staged atom, the in-tile part of
+// sum_stencil drho~ grad(th_x th_y th_z), with drho~ in columns of stride cs
+// whose z halo repeats z < P - 1 after nz.  The stencil is unrolled and
+// predicated on the in-tile ranges; the partial gradient goes to the atom's
+// slot for this tile (partial_slot).
 template <int P>
-__device__ __forceinline__ void gather_staged(const float4* recs, int count, const float* drho,
-                                              const Params& p, int X0, int Y0, int wx, int wy,
-                                              float* grad, int f) {
+__device__ __forceinline__ void gather_cols(const float4* recs, int count, const float* drho,
+                                            const Params& p, int X0, int Y0, int wx, int wy,
+                                            int cs, const AdjBuffers& g, int f, int bx, int by) {
   const int nz = p.n[2];
+  const int plane = p.ty * cs;
+  const int nslot = g.nsx * g.nsy;
   const float sx = (float)p.n[0] * p.oc[0], sy = (float)p.n[1] * p.oc[1],
               sz = (float)p.n[2] * p.oc[2];
   for (int i = threadIdx.x; i < count; i += blockDim.x) {
@@ -400,44 +426,46 @@ __device__ __forceinline__ void gather_staged(const float4* recs, int count, con
     bspline_d<P>(wzf, dtz);
     int gz0 = mz - P;
     if (gz0 < 0) gz0 += nz;
+    int yo[P];
+#pragma unroll
+    for (int o2 = 0; o2 < P; ++o2) {
+      int ly = dy + o2;
+      if (ly >= p.n[1]) ly -= p.n[1];
+      yo[o2] = ly * cs + gz0;
+    }
     float gx = 0.0f, gy = 0.0f, gzs = 0.0f;
-    for (int o = xlo; o < xhi; ++o) {
-      float tx = thx[0], dxv = dtx[0];
 #pragma unroll
-      for (int t = 1; t < P; ++t) {
-        tx = (o == t) ? thx[t] : tx;
-        dxv = (o == t) ? dtx[t] : dxv;
-      }
-      for (int o2 = ylo; o2 < yhi; ++o2) {
-        float tyv = thy[0], dyv = dty[0];
+    for (int o = 0; o < P; ++o) {
+      if (o < xlo || o >= xhi) continue;
+      int lx = dx + o;
+      if (lx >= p.n[0]) lx -= p.n[0];
+      const float* xp = drho + lx * plane;
+      float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
-        for (int t = 1; t < P; ++t) {
-          tyv = (o2 == t) ? thy[t] : tyv;
-          dyv = (o2 == t) ? dty[t] : dyv;
-        }
-        int lx = dx + o, ly = dy + o2;
-        if (lx >= p.n[0]) lx -= p.n[0];
-        if (ly >= p.n[1]) ly -= p.n[1];
-        const float* colp = drho + (lx * p.ty + ly) * nz;
+      for (int o2 = 0; o2 < P; ++o2) {
+        if (o2 < ylo || o2 >= yhi) continue;
+        const float* c = xp + yo[o2];
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-          int gz = gz0 + q;
-          if (gz >= nz) gz -= nz;
-          const float d = colp[gz];
+          const float d = c[q];
           s0 += d * thz[q];
           s1 += d * dtz[q];
         }
-        gx += dxv * tyv * s0;
-        gy += tx * dyv * s0;
-        gzs += tx * tyv * s1;
+        ax += thy[o2] * s0;
+        ay += dty[o2] * s0;
+        az += thy[o2] * s1;
       }
+      gx += dtx[o] * ax;
+      gy += thx[o] * ay;
+      gzs += thx[o] * az;
     }
     const int idx = __float_as_int(rec.w);
-    float* gp = grad + ((size_t)f * p.N + idx) * 3;
-    atomicAdd(gp + 0, gx * sx);
-    atomicAdd(gp + 1, gy * sy);
-    atomicAdd(gp + 2, gzs * sz);
+    const int slot = partial_slot<P>(mx, my, bx, by, p, g);
+    float* gp = g.gpart + (((size_t)f * nslot + slot) * p.N + idx) * 3;
+    gp[0] = gx * sx;
+    gp[1] = gy * sy;
+    gp[2] = gzs * sz;
   }
 }
 
@@ -454,7 +482,8 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E, kz = p.kz;
-  const size_t drho_b = (size_t)ncol * nz * sizeof(float);
+  const int cs = spread_col_stride<P>(nz);
+  const size_t drho_b = (size_t)ncol * cs * sizeof(float);
   const size_t y_b = (size_t)nl * LY * sizeof(float2);
   const size_t A_b = ((drho_b > y_b ? drho_b : y_b) + 15) & ~size_t(15);
   const size_t B_b = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
@@ -588,8 +617,12 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
         const int z = j1 + 16 * rr;
         if (z < nz) {
           const float a = ok0 ? t[rr].x : 0.0f, c = ok1 ? -t[rr].y : 0.0f;
-          drho[c0 * nz + z] = a;
-          drho[(c0 + 1) * nz + z] = c;
+          drho[c0 * cs + z] = a;
+          drho[(c0 + 1) * cs + z] = c;
+          if (z < P - 1) {                            // z halo
+            drho[c0 * cs + nz + z] = a;
+            drho[(c0 + 1) * cs + nz + z] = c;
+          }
           part += (double)a + (double)c;
         }
       }
@@ -607,27 +640,36 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
       atomicAdd(&g.padg[fe], sum);
     }
-    gather_staged<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, g.grad, f);
+    gather_cols<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
       const int chunk = min(kAllRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
-      gather_staged<P>(recs, chunk, drho, p, X0, Y0, wx, wy, g.grad, f);
+      gather_cols<P>(recs, chunk, drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
     }
     __syncthreads();
   }
 }
 
-// Bulk-solvent mean term: dL/dx_j -= (padg_e / count) grad(border weight of
-// atom j), border weight = (sum th_x)(sum th_y)(sum over border z of th_z).
+// Per atom: the sum of the partial gradients of the tiles its stencil
+// reaches (slots recomputed from the same cell indices as the gather), minus
+// the bulk-solvent mean term (padg_e / count) grad(border weight), border
+// weight = (sum th_x)(sum th_y)(sum over border z of th_z).  Writes every
+// entry of grad (zero for atoms with an invalid element id).
 template <int P>
-__global__ void __launch_bounds__(kWG) k_pad_grad(Params p, Buffers b, AdjBuffers g) {
+__global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjBuffers g) {
   const int f = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.N) return;
+  float* gp = g.grad + ((size_t)f * p.N + i) * 3;
   const int e = b.elem[i];
-  if (e < 0 || e >= p.E) return;
+  if (e < 0 || e >= p.E) {
+    gp[0] = 0.0f;
+    gp[1] = 0.0f;
+    gp[2] = 0.0f;
+    return;
+  }
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
   float w[3];
   int base[3];
@@ -637,6 +679,33 @@ __global__ void __launch_bounds__(kWG) k_pad_grad(Params p, Buffers b, AdjBuffer
     const int m = cell_index(r, p.n[a]);
     w[a] = r - (float)m;
     base[a] = wrap_base(m, P, p.n[a]);
+  }
+  // tile offsets (from the home tile) that the stencil reaches along x, y
+  unsigned vx = 0u, vy = 0u;
+  const int hx = base[0] / p.tx, hy = base[1] / p.ty;
+#pragma unroll
+  for (int o = 0; o < P; ++o) {
+    int cx = base[0] + o, cy = base[1] + o;
+    if (cx >= p.n[0]) cx -= p.n[0];
+    if (cy >= p.n[1]) cy -= p.n[1];
+    int ox = cx / p.tx - hx, oy = cy / p.ty - hy;
+    if (ox < 0) ox += p.ntx;
+    if (oy < 0) oy += p.nty;
+    vx |= 1u << ox;
+    vy |= 1u << oy;
+  }
+  const int nslot = g.nsx * g.nsy;
+  const float* part = g.gpart + ((size_t)f * nslot * p.N + i) * 3;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  for (int oy = 0; oy < g.nsy; ++oy) {
+    if (!((vy >> oy) & 1u)) continue;
+    for (int ox = 0; ox < g.nsx; ++ox) {
+      if (!((vx >> ox) & 1u)) continue;
+      const float* q = part + (size_t)(ox + g.nsx * oy) * p.N * 3;
+      s0 += q[0];
+      s1 += q[1];
+      s2 += q[2];
+    }
   }
   float th[P], dth[P];
   double Sx = 0, Sy = 0, Sz = 0, dSx = 0, dSy = 0, dSz = 0;
@@ -666,10 +735,9 @@ __global__ void __launch_bounds__(kWG) k_pad_grad(Params p, Buffers b, AdjBuffer
     }
   }
   const double c = g.padg[(size_t)f * p.E + e] * p.inv_count;
-  float* gp = g.grad + ((size_t)f * p.N + i) * 3;
-  gp[0] -= (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
-  gp[1] -= (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
-  gp[2] -= (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
+  gp[0] = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
+  gp[1] = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
+  gp[2] = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
 }
 
 }  // namespace saxs

@@ -116,6 +116,20 @@ bool tile_sources(int n, int t, int P, std::vector<int4>& out) {
   return true;
 }
 
+// Number of tile offsets (tile - home tile, mod the tile count) that one
+// stencil base + [0, P) reaches along an axis of n cells in tiles of t; the
+// home tile holds the stencil's first point.
+int slot_count(int n, int t, int P) {
+  const int nt = (n + t - 1) / t;
+  int m = 1;
+  for (int base = 0; base < n; ++base)
+    for (int o = 0; o < P; ++o) {
+      const int T = ((base + o) % n) / t;
+      m = std::max(m, (T - base / t + nt) % nt + 1);
+    }
+  return m;
+}
+
 template <typename T>
 int dev_alloc(T** p, size_t count, size_t& total) {
   *p = nullptr;
@@ -197,6 +211,8 @@ struct saxs_plan {
   saxs::Params pp{};           // plain-path geometry (the whole FFT grid as "cell")
   int plain_pairs = 0;
   double* padg = nullptr;      // [F*E] adjoint pad terms
+  float* gpart = nullptr;      // [F][nsx*nsy][N][3] adjoint partials (first backward)
+  int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
 };
 
 namespace {
@@ -263,7 +279,8 @@ void free_plan(saxs_plan* pl) {
   void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
-                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp};
+                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp,
+                  pl->gpart};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete pl;
@@ -498,6 +515,8 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
     free_plan(pl);
     return fail(SAXS_EUNSUP, "spreading tile %dx%d too narrow for order %d", p.tx, p.ty, p.order);
   }
+  pl->nsx = slot_count(p.n[0], p.tx, p.order);
+  pl->nsy = slot_count(p.n[1], p.ty, p.order);
   int rc = derive_ranges(pl, d->lut_len);
   if (rc) {
     free_plan(pl);
@@ -639,6 +658,11 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   if (rc) return rc;
   if (!gH || (n_atoms > 0 && !grad)) return fail(SAXS_EINVAL, "gH/grad pointer is NULL");
   hipStream_t st = (hipStream_t)stream;
+  if (!pl->gpart && pl->max_atoms > 0) {   // partial-gradient slots, on first use
+    rc = dev_alloc(&pl->gpart, (size_t)pl->max_frames * pl->nsx * pl->nsy * pl->max_atoms * 3,
+                   pl->bytes);
+    if (rc) return rc;
+  }
   if (flags & SAXS_BWD_REUSE_FORWARD) {
     if (!pl->last_frames_valid || pl->last_xyz != xyz || pl->last_N != n_atoms ||
         pl->last_F != n_frames)
@@ -650,11 +674,10 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   pl->last_frames_valid = false;           // the adjoint overwrites the spectra
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
-  saxs::AdjBuffers g{gH, grad, pl->padg};
-  const size_t ng = (size_t)n_frames * n_atoms * 3;
-  const size_t nfill = std::max(ng, (size_t)n_frames * p.E);
-  hipLaunchKernelGGL(saxs::k_fill_zero, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, st,
-                     grad, ng, pl->padg, n_frames * p.E);
+  saxs::AdjBuffers g{gH, grad, pl->padg, pl->gpart, pl->nsx, pl->nsy};
+  const int npadg = n_frames * p.E;
+  hipLaunchKernelGGL(saxs::k_fill_zero, dim3((unsigned)((npadg + 255) / 256)), dim3(256), 0, st,
+                     grad, (size_t)0, pl->padg, npadg);
   const bool px = saxs::fft_is_pow2(p.fx), py = saxs::fft_is_pow2(p.fy),
              pz = saxs::fft_is_pow2(p.fz);
   {
@@ -702,12 +725,12 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
       hipLaunchKernelGGL((saxs::k_adj_z<6, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
   }
   if (n_atoms > 0) {
-    Probe pr(pl, SAXS_K_PAD_GRAD, st);
+    Probe pr(pl, SAXS_K_GRAD_SUM, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
     if (p.order == 4)
-      hipLaunchKernelGGL(saxs::k_pad_grad<4>, ga, dim3(saxs::kWG), 0, st, p, b, g);
+      hipLaunchKernelGGL(saxs::k_grad_finalize<4>, ga, dim3(saxs::kWG), 0, st, p, b, g);
     else
-      hipLaunchKernelGGL(saxs::k_pad_grad<6>, ga, dim3(saxs::kWG), 0, st, p, b, g);
+      hipLaunchKernelGGL(saxs::k_grad_finalize<6>, ga, dim3(saxs::kWG), 0, st, p, b, g);
   }
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
