@@ -60,8 +60,8 @@ typedef struct saxs_plan saxs_plan;
  *  tile_x, tile_y spreading column tile (0 = engine default).
  *  flags        SAXS_FLAG_PER_ELEMENT_SPREAD: one spreading workgroup per
  *               (tile, element) instead of one per tile for all elements;
- *               SAXS_FLAG_NO_PRUNED_Z: full Stockham z transform even where
- *               the output-pruned radix-16 one applies.
+ *               SAXS_FLAG_NO_PRUNED_Z: full Stockham z and y transforms even
+ *               where the output-pruned radix-16 ones apply.
  */
 #define SAXS_FLAG_PER_ELEMENT_SPREAD 1
 #define SAXS_FLAG_NO_PRUNED_Z 2

@@ -203,6 +203,74 @@ __global__ void __launch_bounds__(kWG) k_adj_y(Params p, Buffers b) {
   }
 }
 
+// Pruned y-pass adjoint for N = nn_y = 16 R2, the transpose of k_yfft_p:
+// U(y) = sum over kept m of conj(Ybar(m)) W_N^{m y}, Zbar(y) = conj(U(y)) for
+// y < n_y.  With m = s + 16 r' and y = j + R2 r, W^{m y} = W_N^{s j}
+// W_16^{s r} W_R2^{r' j}: stage A item (line, s) loads the kept r' and forms
+// the radix-R2 DFT over r' times W_N^{s j}; one LDS exchange (T_s[j] at
+// 17 j + s); stage B item (line, j) forms the radix-16 DFT over s and writes
+// the y < n_y outputs.  Lines and workgroups as in k_yfft_p.
+template <int N>
+__global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
+  constexpr int R2 = N / 16;
+  constexpr int LY = 17 * R2 + 1;
+  constexpr int LW = yfft_p_lines<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tbuf = reinterpret_cast<float2*>(smem);
+  float2* tw = tbuf + (size_t)LW * LY;                 // W_N^{j s} stored [j][s]
+  const size_t fe = blockIdx.y;
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const int g0 = blockIdx.x * LW;
+  const int Ky = p.Ky, ky = p.ky;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[(i >> 4) * (i & 15)];
+  __syncthreads();
+  for (int item = threadIdx.x; item < LW * 16; item += blockDim.x) {
+    const int l = item % LW, s = item / LW;
+    const int g = g0 + l;
+    float2 t[R2];
+#pragma unroll
+    for (int rr = 0; rr < R2; ++rr) t[rr] = make_float2(0.0f, 0.0f);
+    if (g < nlines) {
+      const int x = g / kz, k = g - x * kz;
+#pragma unroll
+      for (int rr = 0; rr < R2; ++rr) {
+        const int m = s + 16 * rr;
+        if (m <= Ky || m >= N - Ky) {
+          const int jy = m <= Ky ? m : ky - (N - m);
+          const float2 v = b.ybuf[((fe * ky + jy) * nx + x) * kz + k];
+          t[rr] = make_float2(v.x, -v.y);
+        }
+      }
+    }
+    if constexpr (R2 == 8) Radix<8>::dft(t);
+    else if constexpr (R2 == 16) dft16<false>(t);
+    else dft32(t);
+    float2* dst = tbuf + l * LY + s;
+#pragma unroll
+    for (int j = 0; j < R2; ++j) dst[17 * j] = (j == 0) ? t[0] : cmul(t[j], tw[j * 16 + s]);
+  }
+  __syncthreads();
+  {
+    const int l = threadIdx.x % LW, j = threadIdx.x / LW;     // j < R2
+    const int g = g0 + l;
+    float2 v[16];
+    const float2* srcl = tbuf + l * LY + 17 * j;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = srcl[q];
+    dft16<false>(v);
+    if (g < nlines) {
+      const int x = g / kz, k = g - x * kz;
+      float2* dst = b.zbuf + (fe * nx + x) * (size_t)ny * kz + k;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int y = j + R2 * r;
+        if (y < ny) dst[(size_t)y * kz] = make_float2(v[r].x, -v[r].y);
+      }
+    }
+  }
+}
+
 // LDS of k_adj_z: drho~ columns, FFT lines (the staged atom records alias
 // them after the transform), twiddles.
 __host__ __device__ inline size_t adjz_lds_bytes(const Params& p) { return spread_lds_bytes(p); }

@@ -403,11 +403,21 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   else
     hipLaunchKernelGGL((saxs::k_spread_z<6, false>), gs, dim3(saxs::kWG), lds_s, st, p, b); }
   { Probe pr(pl, SAXS_K_YFFT, st);
-  if (saxs::fft_is_pow2(p.fy))
-    hipLaunchKernelGGL(saxs::k_yfft<true>, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
+  const int Ny = p.nn[1];
+  const unsigned nfe = (unsigned)(n_frames * p.E);
+  if (pl->pruned_z && (Ny == 128 || Ny == 256 || Ny == 512)) {
+#define SAXS_YP(NN)                                                                          \
+  hipLaunchKernelGGL((saxs::k_yfft_p<NN>),                                                   \
+                     dim3((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), \
+                          nfe),                                                              \
+                     dim3(saxs::kWG), saxs::yfft_p_lds_bytes<NN>(), st, p, b)
+    if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
+#undef SAXS_YP
+  } else if (saxs::fft_is_pow2(p.fy))
+    hipLaunchKernelGGL(saxs::k_yfft<true>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG),
                        saxs::yfft_lds_bytes(p), st, p, b);
   else
-    hipLaunchKernelGGL(saxs::k_yfft<false>, dim3(p.n[0], p.nchy, n_frames * p.E), dim3(saxs::kWG),
+    hipLaunchKernelGGL(saxs::k_yfft<false>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG),
                        saxs::yfft_lds_bytes(p), st, p, b); }
   if (!with_x) {
     HIP_TRY(hipGetLastError());
@@ -688,7 +698,18 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     else
       hipLaunchKernelGGL(saxs::k_adj_x<false>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
   }
-  {
+  const int Ny = p.nn[1];
+  if (pl->pruned_z && (Ny == 128 || Ny == 256 || Ny == 512)) {
+    Probe pr(pl, SAXS_K_ADJ_Y, st);
+    const unsigned nfe = (unsigned)(n_frames * p.E);
+#define SAXS_AYP(NN)                                                                         \
+  hipLaunchKernelGGL((saxs::k_adj_y_p<NN>),                                                  \
+                     dim3((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), \
+                          nfe),                                                              \
+                     dim3(saxs::kWG), saxs::yfft_p_lds_bytes<NN>(), st, p, b)
+    if (Ny == 128) SAXS_AYP(128); else if (Ny == 256) SAXS_AYP(256); else SAXS_AYP(512);
+#undef SAXS_AYP
+  } else {
     Probe pr(pl, SAXS_K_ADJ_Y, st);
     dim3 gy(p.n[0], p.nchy, n_frames * p.E);
     if (py)

@@ -928,6 +928,84 @@ __global__ void __launch_bounds__(kWG) k_yfft(Params p, Buffers b) {
   }
 }
 
+// Pruned y pass for N = nn_y = 16 R2 (R2 = 8, 16, 32): the n_x kz lines of
+// one (frame, element) are taken 256 / R2 per workgroup; each line's
+// y < n_y inputs (zero-padded to N) go straight from zbuf into registers,
+// y = j + R2 r feeding the radix-16 DFT of item (line, j); one LDS exchange
+// (line stride 17 R2 + 1, Y_j[s] at 17 j + s); stage 1 item (line, s) forms
+// the radix-R2 DFT over j of Y_j[s] W_N^{j s} and writes only the kept
+// outputs m = s + 16 r' with m <= Ky or m >= N - Ky.
+template <int N>
+__host__ __device__ constexpr int yfft_p_lines() { return kWG / (N / 16); }
+
+template <int N>
+__host__ __device__ inline size_t yfft_p_lds_bytes() {
+  return (size_t)yfft_p_lines<N>() * (17 * (N / 16) + 1) * sizeof(float2) + N * sizeof(float2);
+}
+
+template <int N>
+__global__ void __launch_bounds__(kWG) k_yfft_p(Params p, Buffers b) {
+  constexpr int R2 = N / 16;
+  constexpr int LY = 17 * R2 + 1;
+  constexpr int LW = yfft_p_lines<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* ybuf = reinterpret_cast<float2*>(smem);
+  float2* tw = ybuf + (size_t)LW * LY;                 // W_N^{j s} stored [j][s]
+  const size_t fe = blockIdx.y;
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const int g0 = blockIdx.x * LW;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[(i >> 4) * (i & 15)];
+  const float2* src = b.zbuf + fe * (size_t)nx * ny * kz;
+  {
+    const int l = threadIdx.x % LW, j = threadIdx.x / LW;      // j < R2
+    const int g = g0 + l;
+    float2 v[16];
+    if (g < nlines) {
+      const int x = g / kz, k = g - x * kz;
+      const float2* col = src + (size_t)x * ny * kz + k;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int y = j + R2 * r;
+        v[r] = y < ny ? col[(size_t)y * kz] : make_float2(0.0f, 0.0f);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+    }
+    if (2 * ny <= N) dft16<true>(v);
+    else dft16<false>(v);
+    float2* dst = ybuf + l * LY + 17 * j;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) dst[r] = v[r];
+  }
+  __syncthreads();
+  const int Ky = p.Ky, ky = p.ky;
+  for (int item = threadIdx.x; item < LW * 16; item += blockDim.x) {
+    const int l = item % LW, s = item / LW;
+    const int g = g0 + l;
+    if (g >= nlines) continue;
+    float2 t[R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      const float2 y = ybuf[l * LY + 17 * r + s];
+      t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + s]);
+    }
+    if constexpr (R2 == 8) Radix<8>::dft(t);
+    else if constexpr (R2 == 16) dft16<false>(t);
+    else dft32(t);
+    const int x = g / kz, k = g - x * kz;
+#pragma unroll
+    for (int rr = 0; rr < R2; ++rr) {
+      const int m = s + 16 * rr;
+      if (m <= Ky || m >= N - Ky) {
+        const int jy = m <= Ky ? m : ky - (N - m);
+        b.ybuf[((fe * ky + jy) * nx + x) * kz + k] = t[rr];
+      }
+    }
+  }
+}
+
 __host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
   return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) + (size_t)p.kcx * p.lsx * sizeof(float2) +
          (size_t)p.nn[0] * sizeof(float2);
