@@ -212,6 +212,8 @@ struct saxs_plan {
   int plain_pairs = 0;
   double* padg = nullptr;      // [F*E] adjoint pad terms
   float* gpart = nullptr;      // [F][nsx*nsy][N][3] adjoint partials (first backward)
+  int* segoff = nullptr;       // [(512 + max_frames) * keys per frame] segmented sort
+  bool seg_sort = false;       // one frame's keys fit the LDS histogram of k_prep_seg
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
 };
 
@@ -280,7 +282,7 @@ void free_plan(saxs_plan* pl) {
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
                   pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp,
-                  pl->gpart};
+                  pl->gpart, pl->segoff};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete pl;
@@ -308,6 +310,9 @@ saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem)
   b.lut_bin = pl->lut_bin;
   b.lut_ff = pl->lut_ff;
   b.partial = pl->partial;
+  b.segoff = pl->segoff;
+  b.nseg = 0;
+  b.seglen = 1;
   return b;
 }
 
@@ -341,22 +346,42 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
                         int n_frames, const float* oc_diag, hipStream_t st, bool with_x = true) {
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
-  const int nkeys = n_frames * p.E * p.nty * p.ntx;
+  const int kframe = p.E * p.nty * p.ntx;               // keys per frame
+  const int nkeys = n_frames * kframe;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
+  const bool seg = pl->seg_sort && n_atoms > 0;
+  if (seg) {  // ~512 workgroups, >= 4096 atoms per segment, <= 65536 (key, segment) runs
+             // per frame for k_scan_seg; nseg * F <= 512 + F - 1
+    b.nseg = std::max(1, std::min({(512 + n_frames - 1) / n_frames, (n_atoms + 4095) / 4096,
+                                   65536 / kframe}));
+    b.seglen = (n_atoms + b.nseg - 1) / b.nseg;
+  }
   { Probe pr(pl, SAXS_K_CLEAR, st);
   hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, pl->counts,
                      nkeys + 1, pl->pad_sum, n_frames * p.E); }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_PREP, st);
-    dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
-    if (p.order == 4)
-      hipLaunchKernelGGL(saxs::k_prep<4>, ga, dim3(saxs::kWG), 0, st, p, b);
-    else
-      hipLaunchKernelGGL(saxs::k_prep<6>, ga, dim3(saxs::kWG), 0, st, p, b);
+    if (seg) {
+      dim3 gq(b.nseg, n_frames);
+      const size_t lq = saxs::prep_seg_lds_bytes(kframe, p.E);
+      if (p.order == 4)
+        hipLaunchKernelGGL(saxs::k_prep_seg<4>, gq, dim3(saxs::kSortWG), lq, st, p, b);
+      else
+        hipLaunchKernelGGL(saxs::k_prep_seg<6>, gq, dim3(saxs::kSortWG), lq, st, p, b);
+    } else {
+      dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
+      if (p.order == 4)
+        hipLaunchKernelGGL(saxs::k_prep<4>, ga, dim3(saxs::kWG), 0, st, p, b);
+      else
+        hipLaunchKernelGGL(saxs::k_prep<6>, ga, dim3(saxs::kWG), 0, st, p, b);
+    }
   }
   { Probe pr(pl, SAXS_K_SCAN, st);
-  hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
-                     p.E * p.nty * p.ntx, n_atoms); }
+  if (seg)
+    hipLaunchKernelGGL(saxs::k_scan_seg, dim3(n_frames), dim3(1024), 0, st, p, b);
+  else
+    hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
+                       kframe, n_atoms); }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_SCATTER, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
@@ -373,11 +398,16 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const size_t lds_p = saxs::spread_p_lds_bytes(p);
   const int Nz = p.nn[2];
   const bool pruned_ok = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
-                         2 * p.kz - 1 <= Nz / 4 && (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG &&
-                         lds_p <= kLdsLimit;
+                         2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
+                         (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_p <= kLdsLimit;
   if (pruned_ok) {
     dim3 ga(p.ntx * p.nty, n_frames);
-#define SAXS_ZP(PP, NN) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN>), ga, dim3(saxs::kWG), lds_p, st, p, b)
+    const bool zhalf = 2 * p.n[2] <= Nz;
+#define SAXS_ZP(PP, NN)                                                                         \
+  do {                                                                                          \
+    if (zhalf) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_p, st, p, b); \
+    else hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, false>), ga, dim3(saxs::kWG), lds_p, st, p, b);    \
+  } while (0)
     if (p.order == 4) {
       if (Nz == 128) SAXS_ZP(4, 128); else if (Nz == 256) SAXS_ZP(4, 256); else SAXS_ZP(4, 512);
     } else {
@@ -563,6 +593,11 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->counts, (size_t)pl->nkeys_max + 1);
   ALLOC(pl->offsets, (size_t)pl->nkeys_max + 1);
   ALLOC(pl->keyslot, F * N);
+  {
+    const int kframe = E * p.nty * p.ntx;
+    pl->seg_sort = saxs::prep_seg_lds_bytes(kframe, E) <= 48 * 1024;
+    if (pl->seg_sort) ALLOC(pl->segoff, (size_t)(512 + F) * kframe);
+  }
   ALLOC(pl->sorted, F * N);
   ALLOC(pl->pad_sum, F * E);
   ALLOC(pl->padg, F * E);

@@ -67,6 +67,8 @@ struct Buffers {
   const int* lut_bin;
   const float* lut_ff;   // [E][lut_len]
   double* partial;       // [F][ky][nchx][2*num_bins]
+  int* segoff;           // [F][nseg][keys per frame] segmented sort (nseg > 0)
+  int nseg, seglen;      // segments per frame and atoms per segment
 };
 
 // ----------------------------------------------------------- atom helpers --
@@ -126,20 +128,34 @@ __device__ __forceinline__ int wrap_base(int m, int P, int n) {
 // 64-bit integer, used for the spread columns and the border-slab sums.
 // LDS float atomics run at a small fraction of the integer rate on gfx950;
 // integer sums are also exact and independent of the order of the atomics
-// (and of the column tiling).  Spread contributions are B-spline
-// products th_x th_y th_z <= (2/3)^3 < 1, so v * 2^32 fits 32 bits; it is an
-// exact integer for v >= 2^-9 (24-bit mantissa) and truncated below (error
-// < 2^-32, under a tenth of a float32 ulp of a typical column value).  A sum
-// overflows only past 2^32 units of weight.
+// (and of the column tiling).  A spread contribution is a B-spline product
+// v = th_x th_y th_z <= (2/3)^3 < 1 taken as (unsigned)(v * 2^32): it fits
+// 32 bits, is an exact integer for v >= 2^-9 (24-bit mantissa) and is
+// truncated below (error < 2^-32, under a tenth of a float32 ulp of a
+// typical column value).  A sum overflows only past 2^32 units of weight.
 constexpr double kFixInv = 1.0 / 4294967296.0;     // 2^-32
 
-__device__ __forceinline__ unsigned long long to_fixed(float v) {
-  return (unsigned long long)(unsigned)(v * 4294967296.0f);
-}
-
-// Same scale for a weight in [0, 1] held in double (border-slab shares).
+// The same scale for a weight in [0, 1] held in double (border-slab shares).
 __device__ __forceinline__ unsigned long long to_fixed_d(double v) {
   return (unsigned long long)(v * 4294967296.0);
+}
+
+// q = a / d for 0 <= a < 2^22, 1 <= d < 2^22 via a float reciprocal and one
+// correction each way (exact; cheaper than integer division by a runtime d).
+__device__ __forceinline__ int div_small(int a, int d, float inv_d) {
+  int q = (int)((float)a * inv_d);
+  if (q * d > a) --q;
+  if ((q + 1) * d <= a) ++q;
+  return q;
+}
+
+// Column value (fixed point minus pad, both 2^-32 units) as float: the
+// difference is exact in 64 bits and is rounded once, through double.
+__device__ __forceinline__ float fixed_minus(unsigned long long v, unsigned long long pad) {
+  const unsigned long long d = v - pad;
+  const double hi = (double)(int)(unsigned)(d >> 32);
+  const double lo = (double)(unsigned)d;
+  return (float)(hi + lo * kFixInv);
 }
 
 // ------------------------------------------------------------------ kernels --
@@ -150,6 +166,44 @@ __global__ void k_clear(int* counts, int nkeys, unsigned long long* pad_sum, int
   if (i < npad) pad_sum[i] = 0ull;
 }
 
+// Per atom of frame f: its column-tile key within the frame,
+// (e ny_tile + ty) ntx + tx, and its border-slab share of unit weight in
+// fixed point: the sum over the stencil of th_x th_y th_z restricted to
+// z <= d or z >= n_z - d.
+template <int P>
+__device__ __forceinline__ void atom_key_border(const Params& p, const Buffers& b, int f, int i,
+                                                int e, int& key, unsigned long long& border) {
+  const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
+  int base[3];
+  float w[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float r = cell_coord(x[a], p.oc[a], p.n[a]);
+    const int m = cell_index(r, p.n[a]);
+    w[a] = r - (float)m;
+    base[a] = wrap_base(m, P, p.n[a]);
+  }
+  key = (e * p.nty + base[1] / p.ty) * p.ntx + base[0] / p.tx;
+  float th[P];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  bspline<P>(w[0], th);
+#pragma unroll
+  for (int o = 0; o < P; ++o) sx += (double)th[o];
+  bspline<P>(w[1], th);
+#pragma unroll
+  for (int o = 0; o < P; ++o) sy += (double)th[o];
+  bspline<P>(w[2], th);
+#pragma unroll
+  for (int o = 0; o < P; ++o) {
+    int z = base[2] + o;
+    if (z >= p.n[2]) z -= p.n[2];
+    if (z <= p.border || z >= p.n[2] - p.border) sz += (double)th[o];
+  }
+  border = to_fixed_d(sx * sy * sz);
+}
+
+// Counting sort with one global counter per key (any number of keys): the
+// slot of an atom in its (frame, key) bin is an atomic increment.
 template <int P>
 __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
   __shared__ unsigned long long padacc[kMaxElements];
@@ -160,42 +214,100 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
   const int e = i < p.N ? b.elem[i] : -1;
   if (i < p.N && (e < 0 || e >= p.E)) b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
   if (i < p.N && e >= 0 && e < p.E) {
-    const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
-    int base[3];
-    float w[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float r = cell_coord(x[a], p.oc[a], p.n[a]);
-      const int m = cell_index(r, p.n[a]);
-      w[a] = r - (float)m;
-      base[a] = wrap_base(m, P, p.n[a]);
-    }
-    const int key = ((f * p.E + e) * p.nty + base[1] / p.ty) * p.ntx + base[0] / p.tx;
+    int key;
+    unsigned long long border;
+    atom_key_border<P>(p, b, f, i, e, key, border);
+    key += f * p.E * p.nty * p.ntx;
     const int slot = atomicAdd(&b.counts[key], 1);
     b.keyslot[(size_t)f * p.N + i] = make_int2(key, slot);
-    // Border-slab share of this atom's unit weight: sum over the stencil of
-    // th_x th_y th_z restricted to z <= d or z >= n_z - d.
-    float th[P];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    bspline<P>(w[0], th);
-#pragma unroll
-    for (int o = 0; o < P; ++o) sx += (double)th[o];
-    bspline<P>(w[1], th);
-#pragma unroll
-    for (int o = 0; o < P; ++o) sy += (double)th[o];
-    bspline<P>(w[2], th);
-#pragma unroll
-    for (int o = 0; o < P; ++o) {
-      int z = base[2] + o;
-      if (z >= p.n[2]) z -= p.n[2];
-      if (z <= p.border || z >= p.n[2] - p.border) sz += (double)th[o];
-    }
-    atomicAdd(&padacc[e], to_fixed_d(sx * sy * sz));
+    atomicAdd(&padacc[e], border);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
     const unsigned long long v = padacc[e];
     if (v != 0ull) atomicAdd(&b.pad_sum[f * p.E + e], v);
+  }
+}
+
+// ---- segmented counting sort (one frame's keys fit in LDS) ---------------
+//
+// k_prep_seg: workgroup (segment s, frame f) takes atoms [s L, (s+1) L) and
+// counts them per key in an LDS histogram (the slot within (key, segment)
+// is the LDS atomic's old value); the histogram goes to segoff[f][s][k].
+// k_scan_seg turns those counts into the starting position of each
+// (key, segment) run and writes the per-key counts and offsets the spread
+// kernels read.  No global atomics on the counters.
+constexpr int kSortWG = 1024;
+
+__host__ __device__ inline size_t prep_seg_lds_bytes(int K, int E) {
+  return (((size_t)K * sizeof(int) + 15) & ~size_t(15)) + (size_t)E * sizeof(unsigned long long);
+}
+
+template <int P>
+__global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int K = p.E * p.nty * p.ntx;
+  int* hist = reinterpret_cast<int*>(smem);
+  unsigned long long* padacc =
+      reinterpret_cast<unsigned long long*>(smem + (((size_t)K * sizeof(int) + 15) & ~size_t(15)));
+  const int s = blockIdx.x, f = blockIdx.y;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
+  for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0ull;
+  __syncthreads();
+  const int i0 = s * b.seglen, i1 = min(p.N, i0 + b.seglen);
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const int e = b.elem[i];
+    if (e < 0 || e >= p.E) {
+      b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
+      continue;
+    }
+    int key;
+    unsigned long long border;
+    atom_key_border<P>(p, b, f, i, e, key, border);
+    const int slot = atomicAdd(&hist[key], 1);
+    b.keyslot[(size_t)f * p.N + i] = make_int2(f * K + key, slot);
+    atomicAdd(&padacc[e], border);
+  }
+  __syncthreads();
+  int* out = b.segoff + ((size_t)f * b.nseg + s) * K;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) out[k] = hist[k];
+  for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
+    const unsigned long long v = padacc[e];
+    if (v != 0ull) atomicAdd(&b.pad_sum[f * p.E + e], v);
+  }
+}
+
+// Per frame (blockIdx.x): key totals over segments, exclusive scan over
+// keys from f N, then each (key, segment) run's start in place of its count.
+__global__ void __launch_bounds__(1024) k_scan_seg(Params p, Buffers b) {
+  __shared__ int sh[1024];
+  const int f = blockIdx.x, t = threadIdx.x;
+  const int K = p.E * p.nty * p.ntx, S = b.nseg;
+  int* seg = b.segoff + (size_t)f * S * K;
+  const int chunk = (K + 1023) / 1024;
+  const int beg = min(K, t * chunk), end = min(K, beg + chunk);
+  int sum = 0;
+  for (int k = beg; k < end; ++k)
+    for (int q = 0; q < S; ++q) sum += seg[(size_t)q * K + k];
+  sh[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = f * p.N + sh[t] - sum;
+  for (int k = beg; k < end; ++k) {
+    b.offsets[(size_t)f * K + k] = run;
+    int tot = 0;
+    for (int q = 0; q < S; ++q) {
+      const int c = seg[(size_t)q * K + k];
+      seg[(size_t)q * K + k] = run + tot;
+      tot += c;
+    }
+    b.counts[(size_t)f * K + k] = tot;
+    run += tot;
   }
 }
 
@@ -239,7 +351,11 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   rec.y = cell_coord(x[1], p.oc[1], p.n[1]);
   rec.z = cell_coord(x[2], p.oc[2], p.n[2]);
   rec.w = __int_as_float(i);
-  b.sorted[b.offsets[ks.x] + ks.y] = rec;
+  const int pos = b.nseg > 0
+                      ? b.segoff[((size_t)f * b.nseg + i / b.seglen) * (p.E * p.nty * p.ntx) +
+                                 (ks.x - f * p.E * p.nty * p.ntx)] + ks.y
+                      : b.offsets[ks.x] + ks.y;
+  b.sorted[pos] = rec;
 }
 
 // Atom records staged per round in k_spread_z.
@@ -479,7 +595,7 @@ __host__ __device__ constexpr int spread_col_stride(int nz) {
 
 
 // Same contribution as spread_staged into columns of stride cs with a z
-// halo (no z wrap), accumulated in fixed point (to_fixed): the stencil is
+// halo (no z wrap), accumulated in fixed point (kFixInv): the stencil is
 // unrolled and predicated on the in-tile ranges, plane and column offsets
 // are computed once per record, and LPR lanes share a record, lane s taking
 // the x planes o = s, s + LPR, ...
@@ -503,6 +619,8 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
     bspline<P>(rec.x - (float)mx, thx);
     bspline<P>(rec.y - (float)my, thy);
     bspline<P>(rec.z - (float)mz, thz);
+#pragma unroll
+    for (int q = 0; q < P; ++q) thz[q] *= 4294967296.0f;   // 2^32 (exact): to_fixed's scale
     int gz0 = mz - P;
     if (gz0 < 0) gz0 += nz;
     int yo[P];
@@ -526,7 +644,7 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
         unsigned long long* c = xp + yo[o2];
         const float f = fx * thy[o2];
 #pragma unroll
-        for (int q = 0; q < P; ++q) atomicAdd(c + q, to_fixed(f * thz[q]));
+        for (int q = 0; q < P; ++q) atomicAdd(c + q, (unsigned long long)(unsigned)(f * thz[q]));
       }
     }
   }
@@ -685,32 +803,40 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
 // ---- output-pruned z transform (N = 16 * R2, powers of two) --------------
 //
 // Stage 0 (radix 16): butterfly j < R2 takes x[j + R2 r], r < 16, two real
-// columns packed per complex line, writes y[16 j + r].  Stage 1 (length R2):
-// butterfly j < 16 takes y[j + 16 r] W_N^{j r} and evaluates only the kept
-// outputs X(j + 16 r') with k = j + 16 r' in [0, kz) or (N - kz, N), by
-// direct sums over r.  Kept outputs go to a compact array:
-// slot(k) = k for k < kz, kz - 1 + (N - k) for the mirrored ones.
+// columns packed per complex line, writes Y_j[s] at 17 j + s of the line.
+// Stage 1 (radix R2): butterfly s < 16 takes Y_j[s] W_N^{j s} over j and
+// keeps only the outputs X(s + 16 r') with k = s + 16 r' in [0, kz) or
+// (N - kz, N), stored compactly: slot(k) = k for k < kz, kz - 1 + (N - k)
+// for the mirrored ones.
+//
+// LDS: the spread columns (u64, stride cs) and the stage-0 lines (float2,
+// stride 2 cs) share one region so that line l occupies exactly the bytes of
+// columns 2l and 2l + 1.  The threads that read a column pair in stage 0 are
+// the ones that write its line, and those that read the line in stage 1 are
+// the ones that clear it for the next element -- all in one wavefront, whose
+// LDS operations execute in order -- so these steps need no workgroup
+// barrier.
+
+__host__ __device__ inline int zp_col_stride(int nz, int P, int N) {
+  const int a = nz + P - 1, b = 17 * (N / 16) / 2;   // z halo; 17 R2 line entries
+  return ((a > b ? a : b) + 1) & ~1;
+}
 
 __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
-  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
-  const size_t rho = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
-  const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
-  const size_t A = ((rho > y ? rho : y) + 15) & ~size_t(15);
-  const size_t xc = (size_t)(p.tx * p.ty / 2) * (2 * p.kz - 1) * sizeof(float2);
-  const size_t recs = (size_t)kAllRecs * sizeof(float4);
-  const size_t B = ((recs > xc ? recs : xc) + 15) & ~size_t(15);
-  const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j r} as [r][j]
+  const int cs = zp_col_stride(p.n[2], p.order, N);
+  const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
+  const size_t B = (size_t)kAllRecs * sizeof(float4);
+  const size_t X = ((size_t)(p.tx * p.ty / 2) * (2 * p.kz - 1) * sizeof(float2) + 15) & ~size_t(15);
+  const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j s} as [j][s]
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
-  return A + B + tw + seg;
+  return A + B + X + tw + seg;
 }
 
-template <int P, int N>
+template <int P, int N, bool HALF>
 __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   constexpr int R2 = N / 16;
-  // y element e = 16 a + b is stored at 17 a + b (conflict-free stage-0
-  // writes); line stride LY
-  constexpr int LY = 17 * R2 + 1;
+  constexpr int RIN = HALF ? 8 : 16;                   // stage-0 inputs that can be nonzero
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, f = blockIdx.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
@@ -720,18 +846,16 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E, kz = p.kz, nslot = 2 * kz - 1;
-  const int cs = spread_col_stride<P>(nz);
-  const size_t rho_b = (size_t)ncol * cs * sizeof(unsigned long long);
-  const size_t y_b = (size_t)nl * LY * sizeof(float2);
-  const size_t A_b = ((rho_b > y_b ? rho_b : y_b) + 15) & ~size_t(15);
-  const size_t xc_b = (size_t)nl * nslot * sizeof(float2);
-  const size_t rc_b = (size_t)kAllRecs * sizeof(float4);
-  const size_t B_b = ((rc_b > xc_b ? rc_b : xc_b) + 15) & ~size_t(15);
+  const int cs = zp_col_stride(nz, P, N);
+  const int LY = 2 * cs;                               // line stride (float2)
+  const size_t A_b = (size_t)ncol * cs * sizeof(unsigned long long);
+  const size_t B_b = (size_t)kAllRecs * sizeof(float4);
+  const size_t X_b = ((size_t)nl * nslot * sizeof(float2) + 15) & ~size_t(15);
   unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
-  float2* xc = reinterpret_cast<float2*>(smem + A_b);
-  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b);
+  float2* xc = reinterpret_cast<float2*>(smem + A_b + B_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b + X_b);
   int* s_start = reinterpret_cast<int*>(tw + N);
   int* s_cnt = s_start + 9 * E;
   int* s_pref = s_cnt + 9 * E;
@@ -751,11 +875,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     s_start[t] = st;
     s_cnt[t] = cnt;
   }
-  // stage-1 input twiddles W_N^{j r}, stored [r][j] (j < 16, r < R2)
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const int r = i >> 4, j = i & 15;
-    tw[i] = b.twz[j * r];
-  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[(i >> 4) * (i & 15)];
+  for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
+    reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
   __syncthreads();
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     int acc = 0;
@@ -781,13 +903,11 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       if (a < tot0) recs[a] = fetch(0, a);
     }
   }
-  const bool half = (2 * nz <= N);
+  const float inv_kz = 1.0f / (float)kz, inv_ty = 1.0f / (float)p.ty;
   float4 pre[kPrefetch];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
-    for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)   // cs is even
-      reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
+    __syncthreads();                                 // records staged, columns clear
     spread_cols<P, kSpreadLanes>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy, cs);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
       const int chunk = min(kAllRecs, total - base);
@@ -802,53 +922,45 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       const int a = threadIdx.x + u * blockDim.x;
       if (a < tot_next) pre[u] = fetch(e + 1, a);
     }
-    __syncthreads();
+    __syncthreads();                                 // columns complete
 
     const size_t fe = (size_t)f * E + e;
-    const double pad = (double)b.pad_sum[fe] * kFixInv * p.inv_count;
-    // stage 0: read the columns into registers (y aliases rho), then DFT16
-    constexpr int IT0 = 1;                       // items per thread (nl * R2 <= kWG)
-    float2 v0[IT0][16];
+    // bulk-solvent mean in the columns' fixed-point units (a column outside
+    // the cell keeps its zeros)
+    const unsigned long long pad =
+        (unsigned long long)llrint((double)b.pad_sum[fe] * p.inv_count);
+    // stage 0 (item = line l, butterfly j): columns 2l, 2l+1 -> line l in place
+    if (threadIdx.x < nl * R2) {
+      const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;
+      const int c0 = 2 * l;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+      const unsigned long long pad0 = (lx < wx && ly < wy) ? pad : 0ull;
+      const unsigned long long pad1 = (lx < wx && ly + 1 < wy) ? pad : 0ull;
+      const unsigned long long* col0 = rho + c0 * cs;
+      const unsigned long long* col1 = col0 + cs;
+      float2 v[16];
 #pragma unroll
-    for (int it = 0; it < IT0; ++it) {
-      const int item = threadIdx.x + it * kWG;
-      if (item < nl * R2) {
-        const int l = item / R2, j = item - l * R2;
-        const int c0 = 2 * l;
-        const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
-        const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int z = j + R2 * r;
-          float re = 0.0f, im = 0.0f;
-          if (z < nz) {
-            unsigned long long r0 = rho[c0 * cs + z], r1 = rho[(c0 + 1) * cs + z];
-            if (z < P - 1) {                   // fold the z halo back
-              r0 += rho[c0 * cs + nz + z];
-              r1 += rho[(c0 + 1) * cs + nz + z];
-            }
-            if (ok0) re = (float)((double)r0 * kFixInv - pad);
-            if (ok1) im = (float)((double)r1 * kFixInv - pad);
+      for (int r = 0; r < RIN; ++r) {
+        const int z = j + R2 * r;
+        if (z < nz) {
+          unsigned long long r0 = col0[z], r1 = col1[z];
+          if (r == 0 && z < P - 1) {                 // fold the z halo back
+            r0 += col0[nz + z];
+            r1 += col1[nz + z];
           }
-          v0[it][r] = make_float2(re, im);
+          v[r] = make_float2(fixed_minus(r0, pad0), fixed_minus(r1, pad1));
         }
       }
-    }
-    __syncthreads();
+      dft16<HALF>(v);
+      __builtin_amdgcn_wave_barrier();
+      float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll
-    for (int it = 0; it < IT0; ++it) {
-      const int item = threadIdx.x + it * kWG;
-      if (item < nl * R2) {
-        const int l = item / R2, j = item - l * R2;
-        if (half) dft16<true>(v0[it]);
-        else dft16<false>(v0[it]);
-        float2* dst = ybuf + l * LY + 17 * j;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) dst[r] = v0[it][r];
-      }
+      for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
-    __syncthreads();
-    // stage 1: kept outputs only
+    __syncthreads();                                 // lines complete
+    // stage 1 (item = line l, butterfly s): kept outputs, then clear line l
     for (int item = threadIdx.x; item < nl * 16; item += blockDim.x) {
       const int l = item >> 4, j = item & 15;
       float2 t[R2];
@@ -860,33 +972,35 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       if constexpr (R2 == 8) Radix<8>::dft(t);
       else if constexpr (R2 == 16) dft16<false>(t);
       else dft32(t);
-      // keep r' with j + 16 r' < kz or j + 16 r' > N - kz
+      // kz <= N / 8 (plan check): at most KL = N / 128 kept outputs at each end
+      constexpr int KL = N / 128;
       const int lo_end = (kz - j + 15) >> 4;
       const int hi_beg = (N - kz - j) / 16 + 1;
 #pragma unroll
-      for (int rr = 0; rr < R2; ++rr) {
-        if (rr < lo_end || rr >= hi_beg) {
-          const int k = j + 16 * rr;
-          const int slot = k < kz ? k : kz - 1 + (N - k);
-          xc[l * nslot + slot] = t[rr];
-        }
-      }
+      for (int rr = 0; rr < KL; ++rr)
+        if (rr < lo_end) xc[l * nslot + j + 16 * rr] = t[rr];
+#pragma unroll
+      for (int rr = R2 - KL; rr < R2; ++rr)
+        if (rr >= hi_beg) xc[l * nslot + kz - 1 + (N - j - 16 * rr)] = t[rr];
+      __builtin_amdgcn_wave_barrier();
+      uint4* line = reinterpret_cast<uint4*>(ybuf + l * LY);   // LY / 2 uint4
+      for (int i = j; i < LY / 2; i += 16) line[i] = make_uint4(0u, 0u, 0u, 0u);
     }
-    __syncthreads();
+    __syncthreads();                                 // kept outputs complete
+    float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
     for (int i = threadIdx.x; i < nl * kz; i += blockDim.x) {
-      const int l = i / kz, k = i - l * kz;
+      const int l = div_small(i, kz, inv_kz), k = i - l * kz;
       const int c0 = 2 * l;
-      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
       if (lx >= wx || ly >= wy) continue;
       const float2 Z = xc[l * nslot + k];
       const float2 Zm = xc[l * nslot + (k == 0 ? 0 : kz - 1 + k)];
       const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
       const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
-      const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
-      b.zbuf[row * kz + k] = Xa;
-      if (ly + 1 < wy) b.zbuf[(row + 1) * kz + k] = Xb;
+      float2* dst = zt + (lx * p.n[1] + ly) * kz + k;
+      dst[0] = Xa;
+      if (ly + 1 < wy) dst[kz] = Xb;
     }
-    __syncthreads();
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
