@@ -222,11 +222,13 @@ def leg_restraint(torch, steps, keep):
 
     step()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    times = []
+    for _ in range(max(steps, 20)):        # one step is ~1 ms: median of synchronized steps
+        t0 = time.perf_counter()
         loss, g = step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
     keep.update(T=T[0].cpu().numpy(), s=s.cpu().numpy(), sel=sel.cpu().numpy(),
                 grad=g[0].cpu().numpy(), loss=loss.item(), w=w)
     eng.close()

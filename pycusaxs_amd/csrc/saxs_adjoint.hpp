@@ -449,12 +449,19 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
 // strided sparse input k = j + R2 r (loaded straight from zbuf), stage 1 a
 // radix-R2 DFT giving z = j' + 16 r'; both in registers, one LDS transpose.
 
+// Line stride (float2) of the pruned adjoint's stage-0 lines, equal to the
+// column stride (float) of drho~: float columns 2l and 2l + 1 occupy exactly
+// the bytes of line l.  Room for 17 R2 line entries and for a column with
+// its z halo.
+__host__ __device__ inline int adj_line_stride(int nz, int P, int N) {
+  const int a = 17 * (N / 16), c = nz + P - 1;
+  return a > c ? a : c;
+}
+
 __host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
-  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
-  const size_t drho = (size_t)p.tx * p.ty * cs * sizeof(float);
-  const size_t y = (size_t)(p.tx * p.ty / 2) * (17 * (N / 16) + 1) * sizeof(float2);
-  const size_t A = ((drho > y ? drho : y) + 15) & ~size_t(15);
+  const int LY = adj_line_stride(p.n[2], p.order, N);
+  const size_t A = ((size_t)(p.tx * p.ty / 2) * LY * sizeof(float2) + 15) & ~size_t(15);
   const size_t B = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
   const size_t tw = (size_t)N * sizeof(float2);
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
@@ -537,10 +544,17 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
   }
 }
 
+// Pruned z-pass adjoint + gradient gather, all elements of one column tile
+// per workgroup.  Stage 0: the kept spectrum of a column pair (kz <= N / 8,
+// so only inputs r = 0, 1 of the low and r = 14, 15 of the mirrored
+// frequencies can be nonzero) unpacked into conj(S), radix-16 DFT, line l
+// in LDS.  Stage 1: radix-R2 DFT; its outputs overwrite the same line as
+// the drho~ columns 2l and 2l + 1 (line stride = column stride, the threads
+// of a line share one wavefront), z halo repeated after nz.  Then the
+// gather (gather_cols) into the atoms' partial-gradient slots.
 template <int P, int N>
-__global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers g) {
+__global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffers g) {
   constexpr int R2 = N / 16;
-  constexpr int LY = 17 * R2 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, f = blockIdx.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
@@ -550,10 +564,8 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E, kz = p.kz;
-  const int cs = spread_col_stride<P>(nz);
-  const size_t drho_b = (size_t)ncol * cs * sizeof(float);
-  const size_t y_b = (size_t)nl * LY * sizeof(float2);
-  const size_t A_b = ((drho_b > y_b ? drho_b : y_b) + 15) & ~size_t(15);
+  const int LY = adj_line_stride(nz, P, N);            // line stride = drho~ column stride
+  const size_t A_b = ((size_t)nl * LY * sizeof(float2) + 15) & ~size_t(15);
   const size_t B_b = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
   float* drho = reinterpret_cast<float*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
@@ -580,10 +592,7 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
     s_cnt[t] = cnt;
   }
   // stage-1 twiddles W_N^{j r} stored [r][j]
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const int r = i >> 4, j = i & 15;
-    tw[i] = b.twz[j * r];
-  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[(i >> 4) * (i & 15)];
   __syncthreads();
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     int acc = 0;
@@ -612,59 +621,50 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
       const int a = threadIdx.x + u * blockDim.x;
       if (a < total) pre[u] = fetch(e, a);
     }
-    // stage 0: sparse conj(S) straight from zbuf, radix-16 DFT
-    {
-      const int item = threadIdx.x;
-      if (item < nl * R2) {
-        const int l = item / R2, j = item - l * R2;
-        const int c0 = 2 * l;
-        const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
-        const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
-        const size_t row = ok0 ? (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly) : 0;
-        const float2* za = b.zbuf + row * kz;
+    // stage 0 (item = line l, butterfly j): sparse conj(S) from zbuf, DFT16
+    if (threadIdx.x < nl * R2) {
+      const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;
+      const int c0 = 2 * l;
+      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+      if (ok0) {
+        const float2* za = b.zbuf + ((fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly)) * kz;
         const float2* zb = za + kz;
-        float2 v[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int r = 0; r < 2; ++r) {                 // k = j + R2 r < kz
           const int k = j + R2 * r;
-          float2 S = make_float2(0.0f, 0.0f);
-          if (ok0) {
-            if (k == 0 || 2 * k == N) {
-              if (k < kz) {
-                const float2 a = za[k];
-                const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
-                S = make_float2(a.x, c.x);
-              }
-            } else {
-              if (k < kz) {
-                const float2 a = za[k];
-                const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
-                S.x += 0.5f * (a.x - c.y);
-                S.y += 0.5f * (a.y + c.x);
-              }
-              if (N - k < kz) {
-                const float2 a = za[N - k];
-                const float2 c = ok1 ? zb[N - k] : make_float2(0.0f, 0.0f);
-                S.x += 0.5f * (a.x + c.y);
-                S.y += 0.5f * (c.x - a.y);
-              }
-            }
+          if (k < kz) {
+            const float2 a = za[k];
+            const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
+            v[r] = k == 0 ? make_float2(a.x, -c.x)
+                          : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
           }
-          v[r] = make_float2(S.x, -S.y);              // conj
         }
-        dft16<false>(v);
-        float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) dst[r] = v[r];
+        for (int r = 14; r < 16; ++r) {               // N - k < kz
+          const int m = N - (j + R2 * r);
+          if (m < kz) {
+            const float2 a = za[m];
+            const float2 c = ok1 ? zb[m] : make_float2(0.0f, 0.0f);
+            v[r] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+          }
+        }
       }
+      dft16<false>(v);
+      float2* dst = ybuf + l * LY + 17 * j;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
-    __syncthreads();
-    // stage 1: radix-R2 DFT; x = conj(out): column a = Re, column b = -Im
-    float2 t[R2];
-    const int item1 = threadIdx.x;
-    const bool act1 = item1 < nl * 16;
-    const int l1 = item1 >> 4, j1 = item1 & 15;
-    if (act1) {
+    __syncthreads();                                  // lines complete
+    // stage 1 (item = line l, butterfly s): radix-R2 DFT; x = conj(out):
+    // column 2l = Re, column 2l+1 = -Im, written over line l
+    double part = 0.0;
+    for (int item = threadIdx.x; item < nl * 16; item += blockDim.x) {
+      const int l1 = item >> 4, j1 = item & 15;
+      float2 t[R2];
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
         const float2 y = ybuf[l1 * LY + 17 * r + j1];
@@ -673,23 +673,22 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
       if constexpr (R2 == 8) Radix<8>::dft(t);
       else if constexpr (R2 == 16) dft16<false>(t);
       else dft32(t);
-    }
-    __syncthreads();                                  // drho aliases ybuf
-    double part = 0.0;
-    if (act1) {
       const int c0 = 2 * l1;
       const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
       const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+      float* col0 = drho + c0 * LY;
+      float* col1 = col0 + LY;
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int rr = 0; rr < R2; ++rr) {
         const int z = j1 + 16 * rr;
         if (z < nz) {
           const float a = ok0 ? t[rr].x : 0.0f, c = ok1 ? -t[rr].y : 0.0f;
-          drho[c0 * cs + z] = a;
-          drho[(c0 + 1) * cs + z] = c;
-          if (z < P - 1) {                            // z halo
-            drho[c0 * cs + nz + z] = a;
-            drho[(c0 + 1) * cs + nz + z] = c;
+          col0[z] = a;
+          col1[z] = c;
+          if (rr == 0 && z < P - 1) {                 // z halo
+            col0[nz + z] = a;
+            col1[nz + z] = c;
           }
           part += (double)a + (double)c;
         }
@@ -702,21 +701,21 @@ __global__ void __launch_bounds__(kWG) k_adj_zp(Params p, Buffers b, AdjBuffers 
       const int a = threadIdx.x + u * blockDim.x;
       if (a < total) recs[a] = pre[u];
     }
-    __syncthreads();
+    __syncthreads();                                  // drho~ and records ready
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
       atomicAdd(&g.padg[fe], sum);
     }
-    gather_cols<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
+    gather_cols<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, LY, g, f, bx, by);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
       const int chunk = min(kAllRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
-      gather_cols<P>(recs, chunk, drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
+      gather_cols<P>(recs, chunk, drho, p, X0, Y0, wx, wy, LY, g, f, bx, by);
     }
-    __syncthreads();
+    __syncthreads();                                  // gather done: lines reused
   }
 }
 

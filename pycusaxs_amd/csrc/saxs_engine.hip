@@ -755,7 +755,7 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   const int Nz = p.nn[2];
   const size_t lds_ap = saxs::adj_p_lds_bytes(p);
   const bool adj_pruned = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
-                          2 * p.kz - 1 <= Nz / 4 &&
+                          2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
                           (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_ap <= kLdsLimit;
   if (adj_pruned) {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
