@@ -41,6 +41,16 @@ MIX = (("H", 0.50), ("C", 0.30), ("O", 0.15), ("N", 0.04), ("S", 0.01))
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per launch of each kernel for the default workload (100 frames
+# per launch), from rocprofv3 counters on MI355X: reads = 128 B x
+# TCC_EA0_RDREQ_128B + 64 B x TCC_EA0_RDREQ_64B, writes = WRITE_SIZE; the same
+# counters give a 512 MiB elementwise copy exactly.  Commands and the
+# per-kernel breakdown: profiles/hbm_traffic_trajectory_100k_128.md.
+MEASURED_TRAFFIC = {"prep": 211.7e6, "scatter": 569.4e6, "spread_z": 1675.8e6, "yfft": 2889.8e6,
+                    "xfft_epi": 801.7e6, "adj_x": 1278.6e6, "adj_y": 1710.2e6, "adj_z": 2711.3e6,
+                    "grad_sum": 755.9e6}
+
+
 def make_inputs(n_atoms=N_ATOMS, frames=FRAMES):
     rng = np.random.default_rng(SEED)
     base = rng.uniform(0.0, BOX, size=(n_atoms, 3)).astype(np.float32)
@@ -452,9 +462,11 @@ def main():
                          "alg_bytes": kb.get(k), "GBps": round(gbs, 1) if gbs else None}
     dom = max((k for k in per_kernel if k in kb), key=lambda k: times[k][0])
     d = per_kernel[dom]
+    default_workload = args.frames == FRAMES and batch == FRAMES
     line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"],
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": None}
+                        "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+                        "traffic": MEASURED_TRAFFIC.get(dom) if default_workload else None}
     line["kernels"] = per_kernel
     print("# kernels: " + json.dumps(per_kernel), file=sys.stderr, flush=True)
 
