@@ -149,13 +149,12 @@ __device__ __forceinline__ int div_small(int a, int d, float inv_d) {
   return q;
 }
 
-// Column value (fixed point minus pad, both 2^-32 units) as float: the
-// difference is exact in 64 bits and is rounded once, through double.
-__device__ __forceinline__ float fixed_minus(unsigned long long v, unsigned long long pad) {
+// Column value minus pad, both in fixed-point units (2^-32 of a unit
+// weight), as a float in the same units: the difference is exact in 64 bits
+// and is rounded once, through double.
+__device__ __forceinline__ float fixed_minus_units(unsigned long long v, unsigned long long pad) {
   const unsigned long long d = v - pad;
-  const double hi = (double)(int)(unsigned)(d >> 32);
-  const double lo = (double)(unsigned)d;
-  return (float)(hi + lo * kFixInv);
+  return (float)((double)(int)(unsigned)(d >> 32) * 4294967296.0 + (double)(unsigned)d);
 }
 
 // ------------------------------------------------------------------ kernels --
@@ -827,10 +826,9 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const int cs = zp_col_stride(p.n[2], p.order, N);
   const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
   const size_t B = (size_t)kAllRecs * sizeof(float4);
-  const size_t X = ((size_t)(p.tx * p.ty / 2) * (2 * p.kz - 1) * sizeof(float2) + 15) & ~size_t(15);
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j s} as [j][s]
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
-  return A + B + X + tw + seg;
+  return A + B + tw + seg;
 }
 
 template <int P, int N, bool HALF>
@@ -845,17 +843,15 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   const int nz = p.n[2];
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
-  const int E = p.E, kz = p.kz, nslot = 2 * kz - 1;
+  const int E = p.E, kz = p.kz;
   const int cs = zp_col_stride(nz, P, N);
   const int LY = 2 * cs;                               // line stride (float2)
   const size_t A_b = (size_t)ncol * cs * sizeof(unsigned long long);
   const size_t B_b = (size_t)kAllRecs * sizeof(float4);
-  const size_t X_b = ((size_t)nl * nslot * sizeof(float2) + 15) & ~size_t(15);
   unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
-  float2* xc = reinterpret_cast<float2*>(smem + A_b + B_b);
-  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b + X_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b);
   int* s_start = reinterpret_cast<int*>(tw + N);
   int* s_cnt = s_start + 9 * E;
   int* s_pref = s_cnt + 9 * E;
@@ -903,7 +899,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       if (a < tot0) recs[a] = fetch(0, a);
     }
   }
-  const float inv_kz = 1.0f / (float)kz, inv_ty = 1.0f / (float)p.ty;
+  const float inv_ty = 1.0f / (float)p.ty;
   float4 pre[kPrefetch];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
@@ -925,6 +921,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     __syncthreads();                                 // columns complete
 
     const size_t fe = (size_t)f * E + e;
+    float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
     // bulk-solvent mean in the columns' fixed-point units (a column outside
     // the cell keeps its zeros)
     const unsigned long long pad =
@@ -938,20 +935,38 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       const unsigned long long pad1 = (lx < wx && ly + 1 < wy) ? pad : 0ull;
       const unsigned long long* col0 = rho + c0 * cs;
       const unsigned long long* col1 = col0 + cs;
-      float2 v[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+      // inputs in fixed-point units; the 2^-32 scale is applied to the kept
+      // outputs (a power of two commutes exactly with the transform)
+      unsigned long long a0[RIN], a1[RIN];
+      unsigned big = (unsigned)(pad >> 31);
 #pragma unroll
       for (int r = 0; r < RIN; ++r) {
         const int z = j + R2 * r;
+        a0[r] = pad0;
+        a1[r] = pad1;
         if (z < nz) {
-          unsigned long long r0 = col0[z], r1 = col1[z];
+          a0[r] = col0[z];
+          a1[r] = col1[z];
           if (r == 0 && z < P - 1) {                 // fold the z halo back
-            r0 += col0[nz + z];
-            r1 += col1[nz + z];
+            a0[r] += col0[nz + z];
+            a1[r] += col1[nz + z];
           }
-          v[r] = make_float2(fixed_minus(r0, pad0), fixed_minus(r1, pad1));
         }
+        big |= (unsigned)((a0[r] | a1[r]) >> 31);
+      }
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+      if (!__any(big != 0u)) {
+        // every value and the pad below 2^31 units: exact 32-bit difference
+#pragma unroll
+        for (int r = 0; r < RIN; ++r)
+          v[r] = make_float2((float)(int)((unsigned)a0[r] - (unsigned)pad0),
+                             (float)(int)((unsigned)a1[r] - (unsigned)pad1));
+      } else {
+#pragma unroll
+        for (int r = 0; r < RIN; ++r)
+          v[r] = make_float2(fixed_minus_units(a0[r], pad0), fixed_minus_units(a1[r], pad1));
       }
       dft16<HALF>(v);
       __builtin_amdgcn_wave_barrier();
@@ -960,7 +975,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
     __syncthreads();                                 // lines complete
-    // stage 1 (item = line l, butterfly s): kept outputs, then clear line l
+    // stage 1 (item = line l, butterfly s): kept outputs Z(k), k = s + 16 rr;
+    // the two real columns of the line are separated with Z(N - k), which
+    // lane (16 - s) mod 16 of the same line holds; then clear line l
     for (int item = threadIdx.x; item < nl * 16; item += blockDim.x) {
       const int l = item >> 4, j = item & 15;
       float2 t[R2];
@@ -974,32 +991,38 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       else dft32(t);
       // kz <= N / 8 (plan check): at most KL = N / 128 kept outputs at each end
       constexpr int KL = N / 128;
-      const int lo_end = (kz - j + 15) >> 4;
-      const int hi_beg = (N - kz - j) / 16 + 1;
+      // Z(N - k) = t[R2 - 1 - rr] of lane (16 - j) mod 16; for j = 0 it is
+      // t[0] (k = 0) and the lane's own t[R2 - rr]
+      const int src = (threadIdx.x & ~15) | ((16 - j) & 15);
+      float2 zm[KL];
 #pragma unroll
-      for (int rr = 0; rr < KL; ++rr)
-        if (rr < lo_end) xc[l * nslot + j + 16 * rr] = t[rr];
+      for (int rr = 0; rr < KL; ++rr) {
+        const float2 m = t[R2 - 1 - rr];
+        zm[rr] = make_float2(__shfl(m.x, src, 64), __shfl(m.y, src, 64));
+      }
+      if (j == 0) {
 #pragma unroll
-      for (int rr = R2 - KL; rr < R2; ++rr)
-        if (rr >= hi_beg) xc[l * nslot + kz - 1 + (N - j - 16 * rr)] = t[rr];
+        for (int rr = KL - 1; rr >= 1; --rr) zm[rr] = zm[rr - 1];
+        zm[0] = t[0];
+      }
+      const int c0 = 2 * l;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+      if (lx < wx && ly < wy) {
+        float2* dst = zt + (lx * p.n[1] + ly) * kz;
+        constexpr float h = 0.5f / 4294967296.0f;   // 1/2 and the 2^-32 unit
+#pragma unroll
+        for (int rr = 0; rr < KL; ++rr) {
+          const int k = j + 16 * rr;
+          if (k < kz) {
+            const float2 Z = t[rr], Zm = zm[rr];
+            dst[k] = make_float2(h * (Z.x + Zm.x), h * (Z.y - Zm.y));
+            if (ly + 1 < wy) dst[kz + k] = make_float2(h * (Z.y + Zm.y), -h * (Z.x - Zm.x));
+          }
+        }
+      }
       __builtin_amdgcn_wave_barrier();
       uint4* line = reinterpret_cast<uint4*>(ybuf + l * LY);   // LY / 2 uint4
       for (int i = j; i < LY / 2; i += 16) line[i] = make_uint4(0u, 0u, 0u, 0u);
-    }
-    __syncthreads();                                 // kept outputs complete
-    float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
-    for (int i = threadIdx.x; i < nl * kz; i += blockDim.x) {
-      const int l = div_small(i, kz, inv_kz), k = i - l * kz;
-      const int c0 = 2 * l;
-      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
-      if (lx >= wx || ly >= wy) continue;
-      const float2 Z = xc[l * nslot + k];
-      const float2 Zm = xc[l * nslot + (k == 0 ? 0 : kz - 1 + k)];
-      const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
-      const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
-      float2* dst = zt + (lx * p.n[1] + ly) * kz + k;
-      dst[0] = Xa;
-      if (ly + 1 < wy) dst[kz] = Xb;
     }
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
