@@ -552,9 +552,10 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
 // the drho~ columns 2l and 2l + 1 (line stride = column stride, the threads
 // of a line share one wavefront), z halo repeated after nz.  Then the
 // gather (gather_cols) into the atoms' partial-gradient slots.
-template <int P, int N>
+template <int P, int N, bool HALF>
 __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffers g) {
   constexpr int R2 = N / 16;
+  constexpr int ROUT = HALF ? R2 / 2 : R2;             // stage-1 outputs that can be z < nz
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, f = blockIdx.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
@@ -675,22 +676,25 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
       else dft32(t);
       const int c0 = 2 * l1;
       const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
-      const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
       float* col0 = drho + c0 * LY;
       float* col1 = col0 + LY;
       __builtin_amdgcn_wave_barrier();
+      // columns outside the cell are never gathered and add nothing
+      if (lx < wx && ly < wy) {
+        const bool ok1 = ly + 1 < wy;
 #pragma unroll
-      for (int rr = 0; rr < R2; ++rr) {
-        const int z = j1 + 16 * rr;
-        if (z < nz) {
-          const float a = ok0 ? t[rr].x : 0.0f, c = ok1 ? -t[rr].y : 0.0f;
-          col0[z] = a;
-          col1[z] = c;
-          if (rr == 0 && z < P - 1) {                 // z halo
-            col0[nz + z] = a;
-            col1[nz + z] = c;
+        for (int rr = 0; rr < ROUT; ++rr) {
+          const int z = j1 + 16 * rr;
+          if (z < nz) {
+            const float a = t[rr].x, c = ok1 ? -t[rr].y : 0.0f;
+            col0[z] = a;
+            col1[z] = c;
+            if (rr == 0 && z < P - 1) {               // z halo
+              col0[nz + z] = a;
+              col1[nz + z] = c;
+            }
+            part += (double)a + (double)c;
           }
-          part += (double)a + (double)c;
         }
       }
     }

@@ -760,7 +760,12 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   if (adj_pruned) {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 ga(p.ntx * p.nty, n_frames);
-#define SAXS_AZP(PP, NN) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g)
+    const bool zhalf = 2 * p.n[2] <= Nz;
+#define SAXS_AZP(PP, NN)                                                                       \
+  do {                                                                                         \
+    if (zhalf) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g); \
+    else hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, false>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g);    \
+  } while (0)
     if (p.order == 4) {
       if (Nz == 128) SAXS_AZP(4, 128); else if (Nz == 256) SAXS_AZP(4, 256); else SAXS_AZP(4, 512);
     } else {
