@@ -437,10 +437,19 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const unsigned nfe = (unsigned)(n_frames * p.E);
   if (pl->pruned_z && (Ny == 128 || Ny == 256 || Ny == 512)) {
 #define SAXS_YP(NN)                                                                          \
-  hipLaunchKernelGGL((saxs::k_yfft_p<NN>),                                                   \
-                     dim3((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), \
-                          nfe),                                                              \
-                     dim3(saxs::kWG), saxs::yfft_p_lds_bytes<NN>(), st, p, b)
+  do {                                                                                       \
+    const dim3 gyp((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), nfe); \
+    const bool ends = p.Ky < NN / 8 && p.ky < NN, yhalf = 2 * p.n[1] <= NN;                  \
+    const size_t lq = saxs::yfft_p_lds_bytes<NN>();                                          \
+    if (ends && yhalf)                                                                       \
+      hipLaunchKernelGGL((saxs::k_yfft_p<NN, true, true>), gyp, dim3(saxs::kWG), lq, st, p, b); \
+    else if (ends)                                                                           \
+      hipLaunchKernelGGL((saxs::k_yfft_p<NN, true, false>), gyp, dim3(saxs::kWG), lq, st, p, b); \
+    else if (yhalf)                                                                          \
+      hipLaunchKernelGGL((saxs::k_yfft_p<NN, false, true>), gyp, dim3(saxs::kWG), lq, st, p, b); \
+    else                                                                                     \
+      hipLaunchKernelGGL((saxs::k_yfft_p<NN, false, false>), gyp, dim3(saxs::kWG), lq, st, p, b); \
+  } while (0)
     if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
 #undef SAXS_YP
   } else if (saxs::fft_is_pow2(p.fy))

@@ -1080,7 +1080,10 @@ __host__ __device__ inline size_t yfft_p_lds_bytes() {
   return (size_t)yfft_p_lines<N>() * (17 * (N / 16) + 1) * sizeof(float2) + N * sizeof(float2);
 }
 
-template <int N>
+// ENDS: Ky < N / 8, so the kept outputs are r' < KL and r' >= R2 - KL with
+// KL = N / 128 (the others are never formed).  HALF: 2 n_y <= N, so inputs
+// r >= 8 are zero.
+template <int N, bool ENDS, bool HALF>
 __global__ void __launch_bounds__(kWG) k_yfft_p(Params p, Buffers b) {
   constexpr int R2 = N / 16;
   constexpr int LY = 17 * R2 + 1;
@@ -1098,20 +1101,18 @@ __global__ void __launch_bounds__(kWG) k_yfft_p(Params p, Buffers b) {
     const int l = threadIdx.x % LW, j = threadIdx.x / LW;      // j < R2
     const int g = g0 + l;
     float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
     if (g < nlines) {
       const int x = g / kz, k = g - x * kz;
       const float2* col = src + (size_t)x * ny * kz + k;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < (HALF ? 8 : 16); ++r) {
         const int y = j + R2 * r;
-        v[r] = y < ny ? col[(size_t)y * kz] : make_float2(0.0f, 0.0f);
+        if (y < ny) v[r] = col[(size_t)y * kz];
       }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
     }
-    if (2 * ny <= N) dft16<true>(v);
-    else dft16<false>(v);
+    dft16<HALF>(v);
     float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll
     for (int r = 0; r < 16; ++r) dst[r] = v[r];
@@ -1132,12 +1133,27 @@ __global__ void __launch_bounds__(kWG) k_yfft_p(Params p, Buffers b) {
     else if constexpr (R2 == 16) dft16<false>(t);
     else dft32(t);
     const int x = g / kz, k = g - x * kz;
+    float2* out = b.ybuf + (fe * ky * nx + x) * kz + k;  // + jy nx kz
+    if constexpr (ENDS) {
+      constexpr int KL = N / 128;
 #pragma unroll
-    for (int rr = 0; rr < R2; ++rr) {
-      const int m = s + 16 * rr;
-      if (m <= Ky || m >= N - Ky) {
-        const int jy = m <= Ky ? m : ky - (N - m);
-        b.ybuf[((fe * ky + jy) * nx + x) * kz + k] = t[rr];
+      for (int rr = 0; rr < KL; ++rr) {
+        const int m = s + 16 * rr;
+        if (m <= Ky) out[(size_t)m * nx * kz] = t[rr];
+      }
+#pragma unroll
+      for (int rr = R2 - KL; rr < R2; ++rr) {
+        const int m = s + 16 * rr;
+        if (m >= N - Ky) out[(size_t)(ky - (N - m)) * nx * kz] = t[rr];
+      }
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < R2; ++rr) {
+        const int m = s + 16 * rr;
+        if (m <= Ky || m >= N - Ky) {
+          const int jy = m <= Ky ? m : ky - (N - m);
+          out[(size_t)jy * nx * kz] = t[rr];
+        }
       }
     }
   }
