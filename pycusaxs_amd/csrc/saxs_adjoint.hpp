@@ -165,6 +165,170 @@ __global__ void __launch_bounds__(kWG) k_adj_x(Params p, Buffers b, AdjBuffers g
   }
 }
 
+// Pruned x-pass adjoint for N = nn_x = 16 R2, same workgroups, lines and
+// items as k_xfft_p (kept outputs at the ends, Kx < N / 8).  Phase 1
+// recomputes A = sum_e f_e F_e at the kept m_x from Y; Abar = 2 mod w gH_b A;
+// phase 2, per element: c = conj(f_e Abar) at the kept m = s + 16 r',
+// U(x) = sum_m c(m) W_N^{m x} by the transposed stages (x = j + R2 r,
+// W^{m x} = W_N^{s j} W_16^{s r} W_R2^{r' j}: radix-R2 DFT over r' times
+// W_N^{s j}, LDS exchange, radix-16 DFT over s), Ybar(x) = conj(U(x)) for
+// x < n_x written over Y_e.
+template <int N>
+__host__ __device__ inline size_t adjx_p_lds_bytes(const Params& p) {
+  return (size_t)p.kcx * (17 * (N / 16) + 1) * sizeof(float2) + N * sizeof(float2);
+}
+
+template <int N, bool HALF>
+__global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers g) {
+  constexpr int R2 = N / 16;
+  constexpr int LY = 17 * R2 + 1;
+  constexpr int KL = N / 128;
+  constexpr int NO = 2 * KL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], nb = p.num_bins, Kx = p.Kx;
+  float2* lines = reinterpret_cast<float2*>(smem);
+  float2* tw = lines + (size_t)p.kcx * LY;            // W_N^{j s} stored [j][s]
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[(i >> 4) * (i & 15)];
+  const int iy = kept_to_index(jy, p.ky, p.Ky, p.nn[1]);
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  float2 A[kXItems][NO];
+  int nsq_o[kXItems][NO];
+#pragma unroll
+  for (int u = 0; u < kXItems; ++u)
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      A[u][o] = make_float2(0.0f, 0.0f);
+      nsq_o[u][o] = 1 << 30;
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < kc * 16) {
+        const int s = item / kc, k = item - s * kc;
+        const int rr = o < KL ? o : R2 - NO + o;
+        const int m = s + 16 * rr;
+        if (m <= Kx || m >= N - Kx) {
+          const int mx = m < p.nf[0] ? m : m - N;
+          nsq_o[u][o] = mx * mx + my * my + (k0 + k) * (k0 + k);
+        }
+      }
+    }
+  __syncthreads();
+  // phase 1: A at the kept outputs
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = item / kc, k = item - j * kc;
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int r = 0; r < (HALF ? 8 : 16); ++r) {
+        const int x = j + R2 * r;
+        if (x < nx) v[r] = src[(size_t)x * kz + k];
+      }
+      dft16<HALF>(v);
+      float2* dst = lines + k * LY + 17 * j;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) dst[r] = v[r];
+    }
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < kc * 16) {
+        const int s = item / kc, k = item - s * kc;
+        float2 t[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+          const float2 y = lines[k * LY + 17 * r + s];
+          t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + s]);
+        }
+        if constexpr (R2 == 8) Radix<8>::dft(t);
+        else if constexpr (R2 == 16) dft16<false>(t);
+        else dft32(t);
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+          const int rr = o < KL ? o : R2 - NO + o;
+          if (nsq_o[u][o] < p.lut_len) {
+            const float fv = ff[nsq_o[u][o]];
+            A[u][o].x += fv * t[rr].x;
+            A[u][o].y += fv * t[rr].y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Abar = 2 * mod * w * gH_b * A at in-range points, 0 elsewhere
+#pragma unroll
+  for (int u = 0; u < kXItems; ++u) {
+    const int item = threadIdx.x + u * blockDim.x;
+    const int s = item / max(kc, 1), k = item - s * kc;
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      float2 ab = make_float2(0.0f, 0.0f);
+      const int nsq = nsq_o[u][o];
+      if (item < kc * 16 && nsq < p.lut_len) {
+        const int bin = b.lut_bin[nsq];
+        if (bin >= 0 && bin < nb) {
+          const int rr = o < KL ? o : R2 - NO + o;
+          const int m = s + 16 * rr, mz = k0 + k;
+          const float mod = b.modx[m] * b.mody[iy] * b.modz[mz];
+          const bool origin = (m == 0) && (iy == 0) && (mz == 0);
+          const float sc = 2.0f * mod * (origin ? 1.0f : 2.0f) * (float)g.gH[(size_t)f * nb + bin];
+          ab = make_float2(sc * A[u][o].x, sc * A[u][o].y);
+        }
+      }
+      A[u][o] = ab;
+    }
+  }
+  // phase 2: per element, Ybar_e over Y_e
+  for (int e = 0; e < p.E; ++e) {
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < kc * 16) {
+        const int s = item / kc, k = item - s * kc;
+        float2 t[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) t[r] = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+          const int rr = o < KL ? o : R2 - NO + o;
+          if (nsq_o[u][o] < p.lut_len) {
+            const float fv = ff[nsq_o[u][o]];
+            t[rr] = make_float2(fv * A[u][o].x, -fv * A[u][o].y);   // conj(f Abar)
+          }
+        }
+        if constexpr (R2 == 8) Radix<8>::dft(t);
+        else if constexpr (R2 == 16) dft16<false>(t);
+        else dft32(t);
+        float2* dst = lines + k * LY + s;
+#pragma unroll
+        for (int j = 0; j < R2; ++j) dst[17 * j] = (j == 0) ? t[0] : cmul(t[j], tw[j * 16 + s]);
+      }
+    }
+    __syncthreads();
+    float2* ydst = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = item / kc, k = item - j * kc;
+      float2 v[16];
+      const float2* srcl = lines + k * LY + 17 * j;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = srcl[q];
+      dft16<false>(v);
+#pragma unroll
+      for (int r = 0; r < (HALF ? 8 : 16); ++r) {
+        const int x = j + R2 * r;
+        if (x < nx) ydst[(size_t)x * kz + k] = make_float2(v[r].x, -v[r].y);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __host__ __device__ inline size_t adjx_lds_bytes(const Params& p) {
   return (size_t)p.kcx * p.lsx * sizeof(float2) + (size_t)p.nn[0] * sizeof(float2);
 }

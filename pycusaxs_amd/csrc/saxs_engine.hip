@@ -214,6 +214,7 @@ struct saxs_plan {
   float* gpart = nullptr;      // [F][nsx*nsy][N][3] adjoint partials (first backward)
   int* segoff = nullptr;       // [(512 + max_frames) * keys per frame] segmented sort
   bool seg_sort = false;       // one frame's keys fit the LDS histogram of k_prep_seg
+  bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
 };
 
@@ -249,7 +250,23 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
   p.kcy = (p.kz + p.nchy - 1) / p.nchy;
   p.nchx = (p.kz + kcx - 1) / kcx;
   p.kcx = (p.kz + p.nchx - 1) / p.nchx;
-  if (saxs::yfft_lds_bytes(p) > kLdsLimit || saxs::xfft_lds_bytes(p) > kLdsLimit)
+  // pruned x pass: kept m_x at the ends (Kx < N/8), one thread per stage-0
+  // item (kc * N/16 <= 512), LDS for kc lines, twiddles and the histogram
+  const int Nx = p.nn[0];
+  pl->x_pruned = false;
+  if (pl->pruned_z && (Nx == 128 || Nx == 256 || Nx == 512) && p.Kx < Nx / 8 && p.kx < Nx) {
+    const int ly = 17 * (Nx / 16) + 1;
+    const long long room = (long long)kLdsLimit - (((long long)p.num_bins * 12 + 15) & ~15LL) -
+                           (long long)Nx * (long long)sizeof(float2);
+    int kc = std::min({p.kz, 512 / (Nx / 16), (int)(room / (ly * (long long)sizeof(float2)))});
+    if (kc < p.kz && kc >= 8) kc = kc / 8 * 8;
+    if (kc >= 1) {
+      pl->x_pruned = true;
+      p.nchx = (p.kz + kc - 1) / kc;
+      p.kcx = (p.kz + p.nchx - 1) / p.nchx;
+    }
+  }
+  if (saxs::yfft_lds_bytes(p) > kLdsLimit || (!pl->x_pruned && saxs::xfft_lds_bytes(p) > kLdsLimit))
     return fail(SAXS_EUNSUP, "pass LDS exceeds 64 KiB (num_bins %d too large?)", p.num_bins);
   return SAXS_OK;
 }
@@ -463,7 +480,19 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     return SAXS_OK;
   }
   { Probe pr(pl, SAXS_K_XFFT, st);
-  if (saxs::fft_is_pow2(p.fx))
+  if (pl->x_pruned) {
+    const int Nx = p.nn[0];
+    const bool xhalf = 2 * p.n[0] <= Nx;
+    const dim3 gx(p.ky, p.nchx, n_frames);
+    const dim3 bx((unsigned)(((p.kcx * (Nx / 16) + 63) / 64) * 64));
+#define SAXS_XP(NN)                                                                          \
+  do {                                                                                       \
+    if (xhalf) hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, true>), gx, bx, saxs::xfft_p_lds_bytes<NN>(p), st, p, b); \
+    else hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, false>), gx, bx, saxs::xfft_p_lds_bytes<NN>(p), st, p, b); \
+  } while (0)
+    if (Nx == 128) SAXS_XP(128); else if (Nx == 256) SAXS_XP(256); else SAXS_XP(512);
+#undef SAXS_XP
+  } else if (saxs::fft_is_pow2(p.fx))
     hipLaunchKernelGGL(saxs::k_xfft_epi<true>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
                        saxs::xfft_lds_bytes(p), st, p, b);
   else
@@ -566,14 +595,14 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   }
   pl->nsx = slot_count(p.n[0], p.tx, p.order);
   pl->nsy = slot_count(p.n[1], p.ty, p.order);
+  pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
+  pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   int rc = derive_ranges(pl, d->lut_len);
   if (rc) {
     free_plan(pl);
     return rc;
   }
   pl->lut_cap = d->lut_len;
-  pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
-  pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   pl->max_frames = d->max_frames;
   pl->max_atoms = d->max_atoms;
   pl->nkeys_max = d->max_frames * p.E * p.nty * p.ntx;
@@ -737,7 +766,18 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   {
     Probe pr(pl, SAXS_K_ADJ_X, st);
     dim3 gx(p.ky, p.nchx, n_frames);
-    if (px)
+    if (pl->x_pruned) {
+      const int Nx = p.nn[0];
+      const bool xhalf = 2 * p.n[0] <= Nx;
+      const dim3 bx((unsigned)(((p.kcx * (Nx / 16) + 63) / 64) * 64));
+#define SAXS_AXP(NN)                                                                         \
+  do {                                                                                       \
+    if (xhalf) hipLaunchKernelGGL((saxs::k_adj_x_p<NN, true>), gx, bx, saxs::adjx_p_lds_bytes<NN>(p), st, p, b, g); \
+    else hipLaunchKernelGGL((saxs::k_adj_x_p<NN, false>), gx, bx, saxs::adjx_p_lds_bytes<NN>(p), st, p, b, g); \
+  } while (0)
+      if (Nx == 128) SAXS_AXP(128); else if (Nx == 256) SAXS_AXP(256); else SAXS_AXP(512);
+#undef SAXS_AXP
+    } else if (px)
       hipLaunchKernelGGL(saxs::k_adj_x<true>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
     else
       hipLaunchKernelGGL(saxs::k_adj_x<false>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);

@@ -1249,6 +1249,137 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
   }
 }
 
+// Pruned x pass + epilogue for N = nn_x = 16 R2 (R2 = 8, 16, 32): workgroup
+// (kept m_y, kz chunk, frame) with kc = p.kcx lines (one per kz), looping
+// over the elements.  Stage 0 item (line k, j): inputs x = j + R2 r straight
+// from ybuf (k fastest: a wavefront reads contiguous rows), radix-16 DFT
+// (HALF: 2 n_x <= N, inputs r >= 8 zero), one LDS exchange.  Stage 1 item
+// (line k, s): radix-R2 DFT over j of Y_j[s] W_N^{j s}; only the kept outputs
+// m = s + 16 r' are formed (ENDS: Kx < N / 8, r' < KL or >= R2 - KL with
+// KL = N / 128) and each thread accumulates A = sum_e f_e(q) F_e for its
+// outputs in registers.  Then |A|^2 x moduli into the LDS histogram.
+template <int N>
+__host__ __device__ inline size_t xfft_p_lds_bytes(const Params& p) {
+  return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) +
+         (size_t)p.kcx * (17 * (N / 16) + 1) * sizeof(float2) + N * sizeof(float2);
+}
+
+constexpr int kXItems = 2;       // stage-1 items per thread (kc * 16 <= 512 threads * 2)
+
+template <int N, bool ENDS, bool HALF>
+__global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
+  constexpr int R2 = N / 16;
+  constexpr int LY = 17 * R2 + 1;
+  constexpr int NO = ENDS ? 2 * (N / 128) : R2;     // kept outputs per item, at most
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], nb = p.num_bins, Kx = p.Kx;
+  double* Hs = reinterpret_cast<double*>(smem);
+  unsigned int* Cs = reinterpret_cast<unsigned int*>(Hs + nb);
+  float2* lines = reinterpret_cast<float2*>(smem + (((size_t)nb * 12 + 15) & ~size_t(15)));
+  float2* tw = lines + (size_t)p.kcx * LY;            // W_N^{j s} stored [j][s]
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    Hs[i] = 0.0;
+    Cs[i] = 0u;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[(i >> 4) * (i & 15)];
+  const int iy = (p.ky == p.nn[1]) ? jy : (jy <= p.Ky ? jy : p.nn[1] - (p.ky - jy));
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  // this thread's stage-1 items and their kept outputs (same for every element)
+  float2 A[kXItems][NO];
+#pragma unroll
+  for (int u = 0; u < kXItems; ++u)
+#pragma unroll
+    for (int o = 0; o < NO; ++o) A[u][o] = make_float2(0.0f, 0.0f);
+  __syncthreads();
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = item / kc, k = item - j * kc;
+      float2 v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int r = 0; r < (HALF ? 8 : 16); ++r) {
+        const int x = j + R2 * r;
+        if (x < nx) v[r] = src[(size_t)x * kz + k];
+      }
+      dft16<HALF>(v);
+      float2* dst = lines + k * LY + 17 * j;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) dst[r] = v[r];
+    }
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < kc * 16) {
+        const int s = item / kc, k = item - s * kc;
+        float2 t[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+          const float2 y = lines[k * LY + 17 * r + s];
+          t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + s]);
+        }
+        if constexpr (R2 == 8) Radix<8>::dft(t);
+        else if constexpr (R2 == 16) dft16<false>(t);
+        else dft32(t);
+        const int mz = k0 + k;
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+          const int rr = ENDS ? (o < NO / 2 ? o : R2 - NO + o) : o;
+          const int m = s + 16 * rr;
+          if (m <= Kx || m >= N - Kx) {
+            const int mx = m < p.nf[0] ? m : m - N;
+            const int nsq = mx * mx + my * my + mz * mz;
+            if (nsq < p.lut_len) {
+              const float fv = ff[nsq];
+              A[u][o].x += fv * t[rr].x;
+              A[u][o].y += fv * t[rr].y;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kXItems; ++u) {
+    const int item = threadIdx.x + u * blockDim.x;
+    if (item < kc * 16) {
+      const int s = item / kc, k = item - s * kc;
+      const int mz = k0 + k;
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        const int rr = ENDS ? (o < NO / 2 ? o : R2 - NO + o) : o;
+        const int m = s + 16 * rr;
+        if (m <= Kx || m >= N - Kx) {
+          const int mx = m < p.nf[0] ? m : m - N;
+          const int nsq = mx * mx + my * my + mz * mz;
+          if (nsq < p.lut_len) {
+            const int bin = b.lut_bin[nsq];
+            if (bin >= 0 && bin < nb) {
+              const float mod = b.modx[m] * b.mody[iy] * b.modz[mz];
+              const float I = (A[u][o].x * A[u][o].x + A[u][o].y * A[u][o].y) * mod;
+              const bool origin = (m == 0) && (iy == 0) && (mz == 0);
+              atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
+              atomicAdd(&Cs[bin], origin ? 1u : 2u);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = b.partial + (((size_t)f * p.ky + jy) * p.nchx + ch) * 2 * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    out[i] = Hs[i];
+    out[nb + i] = (double)Cs[i];
+  }
+}
+
 // Sums the per-block histograms of each frame (or of all frames) in a fixed
 // order.  grid = (num_bins, frames_out); per_frame selects overwrite of
 // Hf[f][b] vs accumulation into H[b].
