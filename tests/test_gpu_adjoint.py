@@ -77,6 +77,11 @@ def test_adjoint_pruned_n256(gpu):
     run_case(128, 60.0, 20000, qcut=0.5, seed=8)
 
 
+def test_adjoint_pruned_x_chunked(gpu):
+    """kz = 29 at FFT grid 256: the pruned x adjoint over two kz chunks."""
+    run_case(128, 60.0, 5000, qcut=1.5, seed=9)
+
+
 def test_adjoint_exact_bins(gpu):
     run_case(32, 40.0, 2000, qcut=0.6, dq=0.01, policy="exact", seed=5)
 

@@ -224,6 +224,17 @@ def test_pruned_z_n512_matches_full_transform(gpu):
     compare(H1, C1, H2, C2, tol=2e-6, label="pruned vs full, N=512")
 
 
+def test_pruned_x_pass_chunked_kz(gpu):
+    """FFT grid 256 with kz = 29 kept z planes: the pruned x pass splits the
+    kz lines over two workgroups (LDS), the pruned z and y passes run at
+    their largest kept range (kz <= N / 8)."""
+    box = np.diag([60.0] * 3)
+    pos, syms = O.synthetic_system(5000, 60.0, seed=23)
+    H, C = engine_hist([pos], syms, box, 128, 2.0, 0.0, 1.5)
+    Ho, Co = oracle_hist([pos], syms, box, 128, 2.0, 0.0, 1.5)
+    compare(H, C, Ho, Co, label="pruned 128->256, kz 29")
+
+
 def test_pruned_forward_independent_of_atomic_order(gpu):
     """The pruned path sums the spread columns and the border slabs in fixed
     point, so the spectra do not depend on the order of the atomics: repeated
