@@ -437,7 +437,13 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
 
 // LDS of k_adj_z: drho~ columns, FFT lines (the staged atom records alias
 // them after the transform), twiddles.
-__host__ __device__ inline size_t adjz_lds_bytes(const Params& p) { return spread_lds_bytes(p); }
+__host__ __device__ inline size_t adjz_lds_bytes(const Params& p) {
+  const size_t drho = (((size_t)p.tx * p.ty * p.n[2] * sizeof(float)) + 15) & ~size_t(15);
+  size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t recs = (size_t)kSpreadRecs * sizeof(float4);
+  if (buf < recs) buf = recs;
+  return drho + buf + (size_t)p.nn[2] * sizeof(float2);
+}
 
 // z-pass adjoint + B-spline gradient gather: per (column tile, e, f).
 template <int P, bool POW2>

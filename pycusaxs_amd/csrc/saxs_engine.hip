@@ -116,6 +116,30 @@ bool tile_sources(int n, int t, int P, std::vector<int4>& out) {
   return true;
 }
 
+// kz for a shell table of lut_len entries (as derive_ranges computes it).
+int kz_of(const saxs::Params& p, int lut_len) {
+  int K = 0;
+  while ((long long)(K + 1) * (K + 1) < (long long)lut_len) ++K;
+  return std::min(K + 1, p.nn[2] / 2 + 1);
+}
+
+// Whether the spreading kernel and the z adjoint that will run for column
+// tile (p.tx, p.ty) fit: the pruned kernels where they apply, else the
+// all-element or per-element Stockham ones (the launch picks in that order).
+bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len) {
+  p.kz = kz_of(p, lut_len);
+  const int Nz = p.nn[2], nl = p.tx * p.ty / 2;
+  const bool line_ok = nl <= fft_line_cap(p.fz);
+  const bool pruned = spread_all && pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
+                      2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 && nl * (Nz / 16) <= saxs::kWG;
+  const bool fwd = (pruned && saxs::spread_p_lds_bytes(p) <= kLdsLimit) ||
+                   (line_ok && spread_all && saxs::spread_all_lds_bytes(p) <= kLdsLimit) ||
+                   (line_ok && saxs::spread_lds_bytes(p) <= kLdsLimit);
+  const bool bwd = (pruned && saxs::adj_p_lds_bytes(p) <= kLdsLimit) ||
+                   (line_ok && saxs::adjz_lds_bytes(p) <= kLdsLimit);
+  return fwd && bwd;
+}
+
 // Number of tile offsets (tile - home tile, mod the tile count) that one
 // stencil base + [0, P) reaches along an axis of n cells in tiles of t; the
 // home tile holds the stencil's first point.
@@ -417,6 +441,8 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const bool pruned_ok = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
                          2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
                          (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_p <= kLdsLimit;
+  if (!pruned_ok && p.tx * p.ty / 2 > fft_line_cap(p.fz))
+    return fail(SAXS_EUNSUP, "column tile %dx%d exceeds the z transform's line capacity", p.tx, p.ty);
   if (pruned_ok) {
     dim3 ga(p.ntx * p.nty, n_frames);
     const bool zhalf = 2 * p.n[2] <= Nz;
@@ -568,11 +594,12 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   // tile + order - 1 <= n: the in-tile points of any stencil form one run
   if (tx + d->order - 1 > p.n[0]) tx = p.n[0] - d->order + 1;
   if (ty + d->order - 1 > p.n[1]) ty = (p.n[1] - d->order + 1) & ~1;
+  pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
+  pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   for (;;) {
     p.tx = tx;
     p.ty = ty;
-    const int nl = tx * ty / 2;
-    if (saxs::spread_lds_bytes(p) <= kLdsLimit && nl <= fft_line_cap(p.fz)) break;
+    if (tile_fits(pl->spread_all, pl->pruned_z, p, d->lut_len)) break;
     if (tx >= ty && tx > 1)
       tx = (tx + 1) / 2;
     else if (ty > 2)
@@ -595,8 +622,6 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   }
   pl->nsx = slot_count(p.n[0], p.tx, p.order);
   pl->nsy = slot_count(p.n[1], p.ty, p.order);
-  pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
-  pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   int rc = derive_ranges(pl, d->lut_len);
   if (rc) {
     free_plan(pl);
@@ -806,6 +831,8 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   const bool adj_pruned = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
                           2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
                           (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_ap <= kLdsLimit;
+  if (!adj_pruned && (p.tx * p.ty / 2 > fft_line_cap(p.fz) || saxs::adjz_lds_bytes(p) > kLdsLimit))
+    return fail(SAXS_EUNSUP, "column tile %dx%d does not fit the z adjoint", p.tx, p.ty);
   if (adj_pruned) {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 ga(p.ntx * p.nty, n_frames);

@@ -357,157 +357,6 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   b.sorted[pos] = rec;
 }
 
-// Atom records staged per round in k_spread_z.
-constexpr int kSpreadRecs = 512;
-
-// LDS bytes of k_spread_z: density columns, FFT lines (the staged atom
-// records alias the FFT lines; they are dead before the FFT starts) and
-// twiddles.
-__host__ __device__ inline size_t spread_lds_bytes(const Params& p) {
-  const size_t rho = (((size_t)p.tx * p.ty * p.n[2] * sizeof(float)) + 15) & ~size_t(15);
-  size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
-  const size_t recs = (size_t)kSpreadRecs * sizeof(float4);
-  if (buf < recs) buf = recs;
-  const size_t tw = (size_t)p.nn[2] * sizeof(float2);
-  return rho + buf + tw;
-}
-
-template <int P, bool POW2>
-__global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
-  const int bx = tile % p.ntx, by = tile / p.ntx;
-  const int X0 = bx * p.tx, Y0 = by * p.ty;
-  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
-  const int nz = p.n[2], nnz = p.nn[2];
-  const int ncol = p.tx * p.ty;
-  const int nl = ncol / 2;
-  float* rho = reinterpret_cast<float*>(smem);
-  float2* buf = reinterpret_cast<float2*>(smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15)));
-  float4* recs = reinterpret_cast<float4*>(buf);
-  const size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
-  float2* tw = reinterpret_cast<float2*>(reinterpret_cast<unsigned char*>(buf) +
-                                          (buf_bytes > kSpreadRecs * sizeof(float4)
-                                               ? buf_bytes : kSpreadRecs * sizeof(float4)));
-
-  // source segments: up to 3x3 bins whose stencils reach this tile
-  __shared__ int s_start[9], s_cnt[9], s_pref[10];
-  if (threadIdx.x < 9) {
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[threadIdx.x] = st;
-    s_cnt[threadIdx.x] = cnt;
-  }
-  for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
-  for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int k = 0; k < 9; ++k) {
-      s_pref[k] = acc;
-      acc += s_cnt[k];
-    }
-    s_pref[9] = acc;
-  }
-  __syncthreads();
-  const int total = s_pref[9];
-
-  // Gather-spread in rounds of kSpreadRecs atoms: stage the records in LDS
-  // with all threads, then one thread per atom computes its B-spline weights
-  // once and adds its stencil columns that fall inside this tile.
-  for (int base = 0; base < total; base += kSpreadRecs) {
-    const int chunk = min(kSpreadRecs, total - base);
-    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
-      const int a = base + i;
-      int sgm = 0;
-#pragma unroll
-      for (int k = 1; k < 9; ++k) sgm += (a >= s_pref[k]) ? 1 : 0;
-      recs[i] = b.sorted[s_start[sgm] + (a - s_pref[sgm])];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
-      const float4 rec = recs[i];
-      const int mx = cell_index(rec.x, p.n[0]);
-      const int my = cell_index(rec.y, p.n[1]);
-      const int mz = cell_index(rec.z, p.n[2]);
-      float thx[P], thy[P], thz[P];
-      bspline<P>(rec.x - (float)mx, thx);
-      bspline<P>(rec.y - (float)my, thy);
-      bspline<P>(rec.z - (float)mz, thz);
-      int gz0 = mz - P;
-      if (gz0 < 0) gz0 += nz;
-#pragma unroll
-      for (int o = 0; o < P; ++o) {
-        int gx = mx - P + o;
-        if (gx < 0) gx += p.n[0];
-        const int lx = gx - X0;
-        if (lx < 0 || lx >= wx) continue;
-#pragma unroll
-        for (int o2 = 0; o2 < P; ++o2) {
-          int gy = my - P + o2;
-          if (gy < 0) gy += p.n[1];
-          const int ly = gy - Y0;
-          if (ly < 0 || ly >= wy) continue;
-          const float fxy = thx[o] * thy[o2];
-          float* colp = rho + (lx * p.ty + ly) * nz;
-#pragma unroll
-          for (int q = 0; q < P; ++q) {
-            int gz = gz0 + q;
-            if (gz >= nz) gz -= nz;
-            atomicAdd(colp + gz, fxy * thz[q]);
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-  // z-transform: two real columns packed per complex line, the bulk-solvent
-  // mean subtracted inside the cell (split in two floats so that its
-  // rounding leaves no systematic box-shaped residue), zero padding beyond
-  // n_z.  The first FFT stage reads the columns directly.
-  const double pad = (double)b.pad_sum[f * p.E + e] * kFixInv * p.inv_count;
-  const float pad_hi = (float)pad;
-  const float pad_lo = (float)(pad - (double)pad_hi);
-  const int ty = p.ty;
-  auto load = [&](unsigned l, unsigned z) -> float2 {
-    const int c0 = 2 * (int)l;
-    const int lx = c0 / ty, ly = c0 - lx * ty;
-    float re = 0.0f, im = 0.0f;
-    if ((int)z < nz && lx < wx) {
-      if (ly < wy) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
-      if (ly + 1 < wy) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
-    }
-    return make_float2(re, im);
-  };
-  fft_lines_loaded<kMaxC, POW2>(buf, p.lsz, nl, p.fz, tw, load);
-
-  // Unpack the two real transforms: X_a(k) = (Z(k) + conj Z(-k))/2,
-  // X_b(k) = (Z(k) - conj Z(-k))/(2i).
-  const size_t fe = (size_t)f * p.E + e;
-  for (int i = threadIdx.x; i < nl * p.kz; i += blockDim.x) {
-    const int l = i / p.kz, k = i - l * p.kz;
-    const int c0 = 2 * l;
-    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
-    if (lx >= wx || ly >= wy) continue;
-    const float2 Z = buf[l * p.lsz + k];
-    const float2 Zm = buf[l * p.lsz + (k == 0 ? 0 : nnz - k)];
-    const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
-    const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
-    const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
-    b.zbuf[row * p.kz + k] = Xa;
-    if (ly + 1 < wy) b.zbuf[(row + 1) * p.kz + k] = Xb;
-  }
-}
-
 // ---- all elements of a column tile in one workgroup ----------------------
 
 // In-tile stencil range of one axis: stencil points base + o (mod n), o in
@@ -534,55 +383,6 @@ __device__ __forceinline__ void stencil_range(int m, int n, int T0, int w, int& 
   hi = min(P, w - d);
 }
 
-// Adds the in-tile stencil columns of `count` staged atoms to rho.
-template <int P>
-__device__ __forceinline__ void spread_staged(const float4* recs, int count, float* rho,
-                                              const Params& p, int X0, int Y0, int wx, int wy) {
-  const int nz = p.n[2];
-  for (int i = threadIdx.x; i < count; i += blockDim.x) {
-    const float4 rec = recs[i];
-    const int mx = cell_index(rec.x, p.n[0]);
-    const int my = cell_index(rec.y, p.n[1]);
-    const int mz = cell_index(rec.z, p.n[2]);
-    int xlo, xhi, dx, ylo, yhi, dy;
-    stencil_range<P>(mx, p.n[0], X0, wx, xlo, xhi, dx);
-    stencil_range<P>(my, p.n[1], Y0, wy, ylo, yhi, dy);
-    if (xlo >= xhi || ylo >= yhi) continue;
-    float thx[P], thy[P], thz[P];
-    bspline<P>(rec.x - (float)mx, thx);
-    bspline<P>(rec.y - (float)my, thy);
-    bspline<P>(rec.z - (float)mz, thz);
-    int gz0 = mz - P;
-    if (gz0 < 0) gz0 += nz;
-    for (int o = xlo; o < xhi; ++o) {
-      float tx = thx[0];
-#pragma unroll
-      for (int t = 1; t < P; ++t) tx = (o == t) ? thx[t] : tx;
-      for (int o2 = ylo; o2 < yhi; ++o2) {
-        float tyv = thy[0];
-#pragma unroll
-        for (int t = 1; t < P; ++t) tyv = (o2 == t) ? thy[t] : tyv;
-        const float fxy = tx * tyv;
-        int lx = dx + o, ly = dy + o2;
-        if (lx >= p.n[0]) lx -= p.n[0];
-        if (ly >= p.n[1]) ly -= p.n[1];
-        float* colp = rho + (lx * p.ty + ly) * nz;
-        if (gz0 + P <= nz) {
-#pragma unroll
-          for (int q = 0; q < P; ++q) atomicAdd(colp + gz0 + q, fxy * thz[q]);
-        } else {
-#pragma unroll
-          for (int q = 0; q < P; ++q) {
-            int gz = gz0 + q;
-            if (gz >= nz) gz -= nz;
-            atomicAdd(colp + gz, fxy * thz[q]);
-          }
-        }
-      }
-    }
-  }
-}
-
 // LDS column stride of the pruned spread: room for the z stencil to run
 // P - 1 points past nz (folded back onto z < P - 1 when the column is read),
 // rounded to 8 (mod 16) so that the two complex lines a half-wave reads in
@@ -592,12 +392,12 @@ __host__ __device__ constexpr int spread_col_stride(int nz) {
   return ((nz + P - 1 + 7) / 16) * 16 + 8;
 }
 
-
-// Same contribution as spread_staged into columns of stride cs with a z
-// halo (no z wrap), accumulated in fixed point (kFixInv): the stencil is
-// unrolled and predicated on the in-tile ranges, plane and column offsets
-// are computed once per record, and LPR lanes share a record, lane s taking
-// the x planes o = s, s + LPR, ...
+// Adds the in-tile stencil points of `count` staged atoms (B-spline weights
+// th_x th_y th_z of the atom's cell fraction, saxsDeviceKernels.cu:370-398)
+// to LDS columns of stride cs with a z halo (no z wrap), in fixed point
+// (kFixInv): the stencil is unrolled and predicated on the in-tile ranges,
+// plane and column offsets are computed once per record, and LPR lanes share
+// a record, lane s taking the x planes o = s, s + LPR, ...
 template <int P, int LPR>
 __device__ __forceinline__ void spread_cols(const float4* recs, int count,
                                             unsigned long long* rho, const Params& p, int X0,
@@ -649,6 +449,129 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
   }
 }
 
+// Atom records staged per round in k_spread_z.
+constexpr int kSpreadRecs = 512;
+
+// LDS bytes of k_spread_z: density columns, FFT lines (the staged atom
+// records alias the FFT lines; they are dead before the FFT starts) and
+// twiddles.
+__host__ __device__ inline size_t spread_lds_bytes(const Params& p) {
+  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
+  const size_t rho = (((size_t)p.tx * p.ty * cs * sizeof(unsigned long long)) + 15) & ~size_t(15);
+  size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
+  const size_t recs = (size_t)kSpreadRecs * sizeof(float4);
+  if (buf < recs) buf = recs;
+  const size_t tw = (size_t)p.nn[2] * sizeof(float2);
+  return rho + buf + tw;
+}
+
+template <int P, bool POW2>
+__global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], nnz = p.nn[2];
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int cs = spread_col_stride<P>(nz);
+  unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
+  float2* buf = reinterpret_cast<float2*>(
+      smem + ((ncol * cs * sizeof(unsigned long long) + 15) & ~size_t(15)));
+  float4* recs = reinterpret_cast<float4*>(buf);
+  const size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
+  float2* tw = reinterpret_cast<float2*>(reinterpret_cast<unsigned char*>(buf) +
+                                          (buf_bytes > kSpreadRecs * sizeof(float4)
+                                               ? buf_bytes : kSpreadRecs * sizeof(float4)));
+
+  // source segments: up to 3x3 bins whose stencils reach this tile
+  __shared__ int s_start[9], s_cnt[9], s_pref[10];
+  if (threadIdx.x < 9) {
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[threadIdx.x] = st;
+    s_cnt[threadIdx.x] = cnt;
+  }
+  for (int i = threadIdx.x; i < ncol * cs; i += blockDim.x) rho[i] = 0ull;
+  for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[k] = acc;
+      acc += s_cnt[k];
+    }
+    s_pref[9] = acc;
+  }
+  __syncthreads();
+  const int total = s_pref[9];
+
+  // Gather-spread in rounds of kSpreadRecs atoms: stage the records in LDS
+  // with all threads, then one thread per atom computes its B-spline weights
+  // once and adds its stencil columns that fall inside this tile.
+  for (int base = 0; base < total; base += kSpreadRecs) {
+    const int chunk = min(kSpreadRecs, total - base);
+    for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
+      const int a = base + i;
+      int sgm = 0;
+#pragma unroll
+      for (int k = 1; k < 9; ++k) sgm += (a >= s_pref[k]) ? 1 : 0;
+      recs[i] = b.sorted[s_start[sgm] + (a - s_pref[sgm])];
+    }
+    __syncthreads();
+    spread_cols<P, 1>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
+    __syncthreads();
+  }
+
+  // z-transform: two real columns packed per complex line, the bulk-solvent
+  // mean subtracted inside the cell in fixed point (one rounding to float,
+  // in units of 2^-32), zero padding beyond n_z.  The first FFT stage reads
+  // the columns directly.
+  const unsigned long long pad =
+      (unsigned long long)llrint((double)b.pad_sum[f * p.E + e] * p.inv_count);
+  const int ty = p.ty;
+  auto load = [&](unsigned l, unsigned z) -> float2 {
+    const int c0 = 2 * (int)l;
+    const int lx = c0 / ty, ly = c0 - lx * ty;
+    float re = 0.0f, im = 0.0f;
+    if ((int)z < nz && lx < wx) {
+      const unsigned long long* c = rho + c0 * cs + z;
+      const bool halo = (int)z < P - 1;               // fold the z halo back
+      if (ly < wy) re = fixed_minus_units(c[0] + (halo ? c[nz] : 0ull), pad);
+      if (ly + 1 < wy) im = fixed_minus_units(c[cs] + (halo ? c[cs + nz] : 0ull), pad);
+    }
+    return make_float2(re, im);
+  };
+  fft_lines_loaded<kMaxC, POW2>(buf, p.lsz, nl, p.fz, tw, load);
+
+  // Unpack the two real transforms: X_a(k) = (Z(k) + conj Z(-k))/2,
+  // X_b(k) = (Z(k) - conj Z(-k))/(2i).
+  const size_t fe = (size_t)f * p.E + e;
+  for (int i = threadIdx.x; i < nl * p.kz; i += blockDim.x) {
+    const int l = i / p.kz, k = i - l * p.kz;
+    const int c0 = 2 * l;
+    const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+    if (lx >= wx || ly >= wy) continue;
+    const float2 Z = buf[l * p.lsz + k];
+    const float2 Zm = buf[l * p.lsz + (k == 0 ? 0 : nnz - k)];
+    constexpr float h = 0.5f / 4294967296.0f;       // 1/2 and the 2^-32 unit
+    const float2 Xa = make_float2(h * (Z.x + Zm.x), h * (Z.y - Zm.y));
+    const float2 Xb = make_float2(h * (Z.y + Zm.y), -h * (Z.x - Zm.x));
+    const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
+    b.zbuf[row * p.kz + k] = Xa;
+    if (ly + 1 < wy) b.zbuf[(row + 1) * p.kz + k] = Xb;
+  }
+}
+
 #ifndef SAXS_SPREAD_LANES
 #define SAXS_SPREAD_LANES 1
 #endif
@@ -657,7 +580,8 @@ constexpr int kPrefetch = 2;                       // records per thread prefetc
 constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
 
 __host__ __device__ inline size_t spread_all_lds_bytes(const Params& p) {
-  const size_t rho = (((size_t)p.tx * p.ty * p.n[2] * sizeof(float)) + 15) & ~size_t(15);
+  const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
+  const size_t rho = (((size_t)p.tx * p.ty * cs * sizeof(unsigned long long)) + 15) & ~size_t(15);
   size_t buf = (size_t)(p.tx * p.ty / 2) * p.lsz * sizeof(float2);
   const size_t recs = (size_t)kAllRecs * sizeof(float4);
   if (buf < recs) buf = recs;
@@ -682,8 +606,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E;
-  float* rho = reinterpret_cast<float*>(smem);
-  unsigned char* after_rho = smem + ((ncol * nz * sizeof(float) + 15) & ~size_t(15));
+  const int cs = spread_col_stride<P>(nz);
+  unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
+  unsigned char* after_rho = smem + ((ncol * cs * sizeof(unsigned long long) + 15) & ~size_t(15));
   float2* buf = reinterpret_cast<float2*>(after_rho);
   float4* recs = reinterpret_cast<float4*>(after_rho);
   size_t buf_bytes = (size_t)nl * p.lsz * sizeof(float2);
@@ -741,16 +666,16 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
   float4 pre[kPrefetch];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
-    for (int i = threadIdx.x; i < ncol * nz; i += blockDim.x) rho[i] = 0.0f;
+    for (int i = threadIdx.x; i < ncol * cs; i += blockDim.x) rho[i] = 0ull;
     __syncthreads();
     // first round (already staged), then any further rounds synchronously
-    spread_staged<P>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy);
+    spread_cols<P, 1>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy, cs);
     for (int base = kAllRecs; base < total; base += kAllRecs) {
       const int chunk = min(kAllRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
-      spread_staged<P>(recs, chunk, rho, p, X0, Y0, wx, wy);
+      spread_cols<P, 1>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
     }
     // prefetch the first round of element e+1 into registers
     const int tot_next = (e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0;
@@ -762,17 +687,18 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
     __syncthreads();
 
     const size_t fe = (size_t)f * E + e;
-    const double pad = (double)b.pad_sum[fe] * kFixInv * p.inv_count;
-    const float pad_hi = (float)pad;
-    const float pad_lo = (float)(pad - (double)pad_hi);
+    const unsigned long long pad =
+        (unsigned long long)llrint((double)b.pad_sum[fe] * p.inv_count);
     const int ty = p.ty;
     auto load = [&](unsigned l, unsigned z) -> float2 {
       const int c0 = 2 * (int)l;
       const int lx = c0 / ty, ly = c0 - lx * ty;
       float re = 0.0f, im = 0.0f;
       if ((int)z < nz && lx < wx) {
-        if (ly < wy) re = (rho[c0 * nz + z] - pad_hi) - pad_lo;
-        if (ly + 1 < wy) im = (rho[(c0 + 1) * nz + z] - pad_hi) - pad_lo;
+        const unsigned long long* c = rho + c0 * cs + z;
+        const bool halo = (int)z < P - 1;             // fold the z halo back
+        if (ly < wy) re = fixed_minus_units(c[0] + (halo ? c[nz] : 0ull), pad);
+        if (ly + 1 < wy) im = fixed_minus_units(c[cs] + (halo ? c[cs + nz] : 0ull), pad);
       }
       return make_float2(re, im);
     };
@@ -784,8 +710,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
       if (lx >= wx || ly >= wy) continue;
       const float2 Z = buf[l * p.lsz + k];
       const float2 Zm = buf[l * p.lsz + (k == 0 ? 0 : nnz - k)];
-      const float2 Xa = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
-      const float2 Xb = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+      constexpr float h = 0.5f / 4294967296.0f;     // 1/2 and the 2^-32 unit
+      const float2 Xa = make_float2(h * (Z.x + Zm.x), h * (Z.y - Zm.y));
+      const float2 Xb = make_float2(h * (Z.y + Zm.y), -h * (Z.x - Zm.x));
       const size_t row = (fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly);
       b.zbuf[row * p.kz + k] = Xa;
       if (ly + 1 < wy) b.zbuf[(row + 1) * p.kz + k] = Xb;
