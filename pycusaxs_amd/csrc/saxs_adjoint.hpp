@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kWG) k_adj_y(Params p, Buffers b) {
 // the radix-R2 DFT over r' times W_N^{s j}; one LDS exchange (T_s[j] at
 // 17 j + s); stage B item (line, j) forms the radix-16 DFT over s and writes
 // the y < n_y outputs.  Lines and workgroups as in k_yfft_p.
-template <int N>
+template <int N, bool ENDS, bool HALF>
 __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
   constexpr int R2 = N / 16;
   constexpr int LY = 17 * R2 + 1;
@@ -397,12 +397,15 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
     for (int rr = 0; rr < R2; ++rr) t[rr] = make_float2(0.0f, 0.0f);
     if (g < nlines) {
       const int x = g / kz, k = g - x * kz;
+      const float2* in = b.ybuf + (fe * ky * nx + x) * kz + k;   // + jy nx kz
 #pragma unroll
       for (int rr = 0; rr < R2; ++rr) {
+        // ENDS: Ky < N / 8, kept m only at r' < KL or r' >= R2 - KL
+        if (ENDS && rr >= N / 128 && rr < R2 - N / 128) continue;
         const int m = s + 16 * rr;
         if (m <= Ky || m >= N - Ky) {
           const int jy = m <= Ky ? m : ky - (N - m);
-          const float2 v = b.ybuf[((fe * ky + jy) * nx + x) * kz + k];
+          const float2 v = in[(size_t)jy * nx * kz];
           t[rr] = make_float2(v.x, -v.y);
         }
       }
@@ -427,7 +430,7 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
       const int x = g / kz, k = g - x * kz;
       float2* dst = b.zbuf + (fe * nx + x) * (size_t)ny * kz + k;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < (HALF ? 8 : 16); ++r) {      // HALF: 2 n_y <= N
         const int y = j + R2 * r;
         if (y < ny) dst[(size_t)y * kz] = make_float2(v[r].x, -v[r].y);
       }

@@ -812,10 +812,19 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     Probe pr(pl, SAXS_K_ADJ_Y, st);
     const unsigned nfe = (unsigned)(n_frames * p.E);
 #define SAXS_AYP(NN)                                                                         \
-  hipLaunchKernelGGL((saxs::k_adj_y_p<NN>),                                                  \
-                     dim3((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), \
-                          nfe),                                                              \
-                     dim3(saxs::kWG), saxs::yfft_p_lds_bytes<NN>(), st, p, b)
+  do {                                                                                       \
+    const dim3 gay((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), nfe); \
+    const bool ends = p.Ky < NN / 8 && p.ky < NN, yhalf = 2 * p.n[1] <= NN;                  \
+    const size_t lq = saxs::yfft_p_lds_bytes<NN>();                                          \
+    if (ends && yhalf)                                                                       \
+      hipLaunchKernelGGL((saxs::k_adj_y_p<NN, true, true>), gay, dim3(saxs::kWG), lq, st, p, b); \
+    else if (ends)                                                                           \
+      hipLaunchKernelGGL((saxs::k_adj_y_p<NN, true, false>), gay, dim3(saxs::kWG), lq, st, p, b); \
+    else if (yhalf)                                                                          \
+      hipLaunchKernelGGL((saxs::k_adj_y_p<NN, false, true>), gay, dim3(saxs::kWG), lq, st, p, b); \
+    else                                                                                     \
+      hipLaunchKernelGGL((saxs::k_adj_y_p<NN, false, false>), gay, dim3(saxs::kWG), lq, st, p, b); \
+  } while (0)
     if (Ny == 128) SAXS_AYP(128); else if (Ny == 256) SAXS_AYP(256); else SAXS_AYP(512);
 #undef SAXS_AYP
   } else {
