@@ -82,6 +82,12 @@ def test_adjoint_pruned_x_chunked(gpu):
     run_case(128, 60.0, 5000, qcut=1.5, seed=9)
 
 
+def test_adjoint_small_tile_nine_slots(gpu):
+    """Column tile 2x2: an atom's stencil reaches up to 3 x 3 tiles (nine
+    partial-gradient slots), and the sort runs with global counters."""
+    run_case(128, 60.0, 8000, qcut=0.5, seed=10, tile=(2, 2))
+
+
 def test_adjoint_exact_bins(gpu):
     run_case(32, 40.0, 2000, qcut=0.6, dq=0.01, policy="exact", seed=5)
 

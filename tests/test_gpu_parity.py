@@ -235,6 +235,18 @@ def test_pruned_x_pass_chunked_kz(gpu):
     compare(H, C, Ho, Co, label="pruned 128->256, kz 29")
 
 
+def test_counting_sort_global_fallback(gpu):
+    """Column tile 2x2 on a 128^3 cell: E * 64 * 64 (element, tile) keys per
+    frame exceed the LDS histogram of the segmented sort (12288 keys), so
+    the counting sort with one global counter per key runs."""
+    box = np.diag([60.0] * 3)
+    pos, syms = O.synthetic_system(8000, 60.0, seed=25)
+    assert len(set(syms)) * 64 * 64 > 12288
+    H, C = engine_hist([pos], syms, box, 128, 2.0, 0.0, 0.5, tile=(2, 2))
+    Ho, Co = oracle_hist([pos], syms, box, 128, 2.0, 0.0, 0.5)
+    compare(H, C, Ho, Co, label="tile 2x2, global-counter sort")
+
+
 def test_pruned_forward_independent_of_atomic_order(gpu):
     """The pruned path sums the spread columns and the border slabs in fixed
     point, so the spectra do not depend on the order of the atomics: repeated
