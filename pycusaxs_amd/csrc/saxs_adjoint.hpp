@@ -635,7 +635,7 @@ __host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
   const int LY = adj_line_stride(p.n[2], p.order, N);
   const size_t A = ((size_t)(p.tx * p.ty / 2) * LY * sizeof(float2) + 15) & ~size_t(15);
-  const size_t B = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
+  const size_t B = ((size_t)kZpRecs * sizeof(float4) + 15) & ~size_t(15);
   const size_t tw = (size_t)N * sizeof(float2);
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
   return A + B + tw + seg;
@@ -740,7 +740,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   const int E = p.E, kz = p.kz;
   const int LY = adj_line_stride(nz, P, N);            // line stride = drho~ column stride
   const size_t A_b = ((size_t)nl * LY * sizeof(float2) + 15) & ~size_t(15);
-  const size_t B_b = ((size_t)kAllRecs * sizeof(float4) + 15) & ~size_t(15);
+  const size_t B_b = ((size_t)kZpRecs * sizeof(float4) + 15) & ~size_t(15);
   float* drho = reinterpret_cast<float*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
@@ -788,12 +788,13 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   for (int e = 0; e < E; ++e) {
     const size_t fe = (size_t)f * E + e;
     const int total = s_pref[e * 10 + 9];
+    const int staged = min(total, kZpRecs);
     // records of this element: loads in flight during the transform
-    float4 pre[kPrefetch];
+    float4 pre[kZpPre];
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
+    for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
-      if (a < total) pre[u] = fetch(e, a);
+      if (a < staged) pre[u] = fetch(e, a);
     }
     // stage 0 (item = line l, butterfly j): sparse conj(S) from zbuf, DFT16
     if (threadIdx.x < nl * R2) {
@@ -874,9 +875,9 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
     for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
+    for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
-      if (a < total) recs[a] = pre[u];
+      if (a < staged) recs[a] = pre[u];
     }
     __syncthreads();                                  // drho~ and records ready
     if (threadIdx.x == 0) {
@@ -884,9 +885,9 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
       atomicAdd(&g.padg[fe], sum);
     }
-    gather_cols<P>(recs, min(total, kAllRecs), drho, p, X0, Y0, wx, wy, LY, g, f, bx, by);
-    for (int base = kAllRecs; base < total; base += kAllRecs) {
-      const int chunk = min(kAllRecs, total - base);
+    gather_cols<P>(recs, staged, drho, p, X0, Y0, wx, wy, LY, g, f, bx, by);
+    for (int base = kZpRecs; base < total; base += kZpRecs) {
+      const int chunk = min(kZpRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();

@@ -578,6 +578,11 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
 constexpr int kSpreadLanes = SAXS_SPREAD_LANES;    // lanes per record in spread_cols
 constexpr int kPrefetch = 2;                       // records per thread prefetched
 constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
+// Records staged per round by the pruned z kernels: fewer, so that four
+// workgroups fit the LDS of a CU (a column tile and element holds ~100
+// records at 0.05 atoms per cell; denser tiles take more rounds).
+constexpr int kZpRecs = 160;
+constexpr int kZpPre = (kZpRecs + kWG - 1) / kWG;  // records prefetched per thread
 
 __host__ __device__ inline size_t spread_all_lds_bytes(const Params& p) {
   const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
@@ -752,7 +757,7 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
   const int cs = zp_col_stride(p.n[2], p.order, N);
   const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
-  const size_t B = (size_t)kAllRecs * sizeof(float4);
+  const size_t B = (size_t)kZpRecs * sizeof(float4);
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j s} as [j][s]
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
   return A + B + tw + seg;
@@ -774,7 +779,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   const int cs = zp_col_stride(nz, P, N);
   const int LY = 2 * cs;                               // line stride (float2)
   const size_t A_b = (size_t)ncol * cs * sizeof(unsigned long long);
-  const size_t B_b = (size_t)kAllRecs * sizeof(float4);
+  const size_t B_b = (size_t)kZpRecs * sizeof(float4);
   unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
@@ -819,29 +824,29 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
   };
   {
-    const int tot0 = s_pref[9];
+    const int tot0 = min(s_pref[9], kZpRecs);
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
+    for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
       if (a < tot0) recs[a] = fetch(0, a);
     }
   }
   const float inv_ty = 1.0f / (float)p.ty;
-  float4 pre[kPrefetch];
+  float4 pre[kZpPre];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
     __syncthreads();                                 // records staged, columns clear
-    spread_cols<P, kSpreadLanes>(recs, min(total, kAllRecs), rho, p, X0, Y0, wx, wy, cs);
-    for (int base = kAllRecs; base < total; base += kAllRecs) {
-      const int chunk = min(kAllRecs, total - base);
+    spread_cols<P, kSpreadLanes>(recs, min(total, kZpRecs), rho, p, X0, Y0, wx, wy, cs);
+    for (int base = kZpRecs; base < total; base += kZpRecs) {
+      const int chunk = min(kZpRecs, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
       spread_cols<P, kSpreadLanes>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
     }
-    const int tot_next = (e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0;
+    const int tot_next = min((e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0, kZpRecs);
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
+    for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
       if (a < tot_next) pre[u] = fetch(e + 1, a);
     }
@@ -952,7 +957,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
       for (int i = j; i < LY / 2; i += 16) line[i] = make_uint4(0u, 0u, 0u, 0u);
     }
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) {
+    for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
       if (a < tot_next) recs[a] = pre[u];
     }
