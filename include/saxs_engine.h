@@ -97,6 +97,16 @@ int saxs_plan_create(const saxs_desc* desc, saxs_plan** out);
 int saxs_plan_set_tables(saxs_plan* plan, int32_t lut_len, const int32_t* bin_lut,
                          const float* ff_lut);
 
+/* Bulk-solvent pad mode.  pad == NULL: the pad of each (frame, element) is
+ * the mean density over the border z-slabs (padding::avg, the reference's
+ * only live mode: Options.cpp:13, saxsDeviceKernels.cu:446-465,
+ * saxsKernel.cu:148-161).  Otherwise pad[num_elements] (host, atoms per
+ * cell voxel, each in [0, 2^40]) is used as a constant for every frame
+ * (padding::given, declared in Options.h:7-11 with its values from
+ * AtomCounter::calculateAtomCounts, AtomCounter.cpp:77-103); the adjoint
+ * then has no pad-mean term.  Synchronous; applies to later calls. */
+int saxs_plan_set_padding(saxs_plan* plan, const double* pad);
+
 /* Forward scattering of n_frames frames of one system, accumulated into the
  * caller's histogram: H[b] += sum over frames of the S6 weighted intensity,
  * C[b] += matching point counts (NPT mode, saxsDeviceKernels.cu:67-125).

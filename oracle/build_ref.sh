@@ -15,5 +15,5 @@ g++ -std=c++17 -O2 -o "$HERE/_ref/ref_driver" \
   -I"$REF/include" -I"$REF/Utilities" -I"$REF/System" \
   "$HERE/ref_driver.cpp" \
   "$REF/Utilities/BSpline.cpp" "$REF/Utilities/BSpmod.cpp" \
-  "$REF/Utilities/Scattering.cpp" "$REF/System/Cell.cpp"
+  "$REF/Utilities/Scattering.cpp" "$REF/System/Cell.cpp" "$REF/System/AtomCounter.cpp"
 echo "built $HERE/_ref/ref_driver"

@@ -16,6 +16,7 @@ where they decide integers (grid cell of an atom, FFT size, bin parameters):
           inverse_box                 Cell.cpp:111-126
   frame   spread (S1)                 saxsDeviceKernels.cu:338-398, Splines.cu:11-74
           pad (S2)                    saxsDeviceKernels.cu:446-465
+          pad, "given" mode           Options.h:7-11, AtomCounter.cpp:56-103
           embed (S3)                  saxsDeviceKernels.cu:413-429
           R2C + form factor (S4)      saxsKernel.cu:171, saxsDeviceKernels.cu:177-216,
                                       opsfact.cu:29-35
@@ -44,7 +45,7 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass, field
 from fractions import Fraction
-from typing import Dict, List, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -251,6 +252,10 @@ class OraclePlan:
     hp: HistParams
     moduli: Tuple[np.ndarray, np.ndarray, np.ndarray]
     border: int
+    # None: bulk-solvent pad = border-slab mean (padding::avg, the
+    # reference's only live mode).  A dict {symbol: atoms per cell voxel}:
+    # padding::given, elements not in the dict padded with 0.
+    pad_given: Optional[Dict[str, float]] = None
     _bin_cache: Dict[int, int] = field(default_factory=dict)
 
     def bins_for(self, nsq: np.ndarray) -> np.ndarray:
@@ -266,7 +271,7 @@ class OraclePlan:
 
 
 def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
-              dq_policy="reference") -> OraclePlan:
+              dq_policy="reference", pad_given=None) -> OraclePlan:
     n = tuple(int(x) for x in n)
     nn = tuple(int(x) for x in scaled) if scaled else tuple(find_closest_product(x, sigma) for x in n)
     oc = inverse_box(box)
@@ -274,7 +279,36 @@ def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
     if not (hp.spacing[0] == hp.spacing[1] == hp.spacing[2]):
         raise ValueError("oracle bins need an isotropic reciprocal grid")
     mods = tuple(bsp_moduli(nn[a], order) for a in range(3))
-    return OraclePlan(n, nn, order, hp, mods, border_planes(n[2]))
+    pads = None if pad_given is None else {str(k): float(v) for k, v in dict(pad_given).items()}
+    return OraclePlan(n, nn, order, hp, mods, border_planes(n[2]), pads)
+
+
+# Water densities in g/cm^3 and the constants of AtomCounter.cpp:20-29.
+WATER_DENSITY = {"SPCE": 0.998, "SPC": 0.998, "TIP3P": 0.982, "TIP4P": 0.999}
+AVOGADRO = 6.022e23
+WATER_MOLAR_MASS = 18.015
+
+
+def solvent_pads(lengths, n, water_model: str, ions: Dict[str, int]) -> Dict[str, float]:
+    """Atoms per cell voxel of the explicit solvent, AtomCounter::
+    calculateAtomCounts (AtomCounter.cpp:56-103) in its float32 arithmetic:
+    n_w = rho_w(model) * V * 1e-24 / 18.015 * 6.022e23 water molecules in
+    the box, O = n_w / (n_x n_y n_z), H = 2 n_w / (n_x n_y n_z), each ion
+    count / (n_x n_y n_z).  An unknown model raises, as std::map::at does
+    there (AtomCounter.cpp:58 looks the model up before the SPCE fallback)."""
+    if water_model not in WATER_DENSITY:
+        raise ValueError(f"unknown water model {water_model!r}")
+    lx, ly, lz = (F32(v) for v in lengths)
+    volume = F32(F32(lx * ly) * lz)
+    volume_cm3 = F32(volume * F32(1e-24))
+    mass = F32(F32(WATER_DENSITY[water_model]) * volume_cm3)
+    molecules = F32(F32(mass / F32(WATER_MOLAR_MASS)) * F32(AVOGADRO))
+    total = int(n[0]) * int(n[1]) * int(n[2])
+    tot = F32(total)
+    out = {"O": float(F32(molecules / tot)), "H": float(F32(F32(2.0) * molecules / tot))}
+    for ion, count in ions.items():
+        out[str(ion)] = float(F32(F32(int(count)) / tot))
+    return out
 
 
 # ------------------------------------------------------------ per frame ---
@@ -364,7 +398,8 @@ def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=
     for e, sym in enumerate(symbols):
         sel = np.asarray(elem_ids) == e
         g = spread(np.asarray(pos)[sel], oc, plan.n, plan.order)
-        pad = border_pad(g, plan.border)
+        pad = border_pad(g, plan.border) if plan.pad_given is None \
+            else plan.pad_given.get(sym, 0.0)
         if literal:
             sup = np.full(plan.nn, pad, dtype=np.float64)
             sup[: plan.n[0], : plan.n[1], : plan.n[2]] = g
@@ -491,7 +526,8 @@ def grad_linear_H(plan, pos, elem_ids, symbols, box, gH):
         mask[n[2] - d:] = True
         count = n[0] * n[1] * int(mask.sum())
         drho = rho_t.copy()
-        drho[:, :, mask] -= rho_t.sum() / count
+        if plan.pad_given is None:             # a given pad does not move with the atoms
+            drho[:, :, mask] -= rho_t.sum() / count
         sel = np.nonzero(elem_ids == e)[0]
         if len(sel) == 0:
             continue

@@ -88,7 +88,7 @@ def _validate(grid, scaled_grid, begin, end, stride, order, scale_factor, bin_si
 
 
 def summary_text(topology, trajectory, grid, scaled, begin, end, stride, order, sigma, dq, qcut,
-                 water_model, ions: Dict[str, int]) -> str:
+                 water_model, ions: Dict[str, int], padding: str = "avg") -> str:
     """Configuration summary (cudaSAXS.cu:97-217), plain text."""
     bar = "═" * 64
     s = ["", "╔" + bar + "╗", "║                  CuSAXS Configuration                         ║",
@@ -109,7 +109,7 @@ def summary_text(topology, trajectory, grid, scaled, begin, end, stride, order, 
         if count > 0:
             sym = ion + ("⁺" if ion in ("Na", "K") else "⁻" if ion == "Cl" else "")
             s.append(f"  {sym} atoms     : {count}")
-    s.append("  Padding Mode  : average")
+    s.append("  Padding Mode  : " + ("average" if padding == "avg" else "given"))
     return "\n".join(s) + "\n\n" + "-" * 66 + "\n"
 
 
@@ -155,8 +155,15 @@ def run(obj_topology, topology: str, trajectory: str, grid: Sequence[int],
         scaled_grid: Sequence[int] = (), begin: int = 0, end: int = 0, stride: int = 1,
         output: str = "", order: int = 4, scale_factor: float = 1.0, bin_size: float = 0.0,
         qcut: float = 0.0, water_model: str = "", sodium: int = 0, chlorine: int = 0,
-        simulation: str = "", *, dq_policy: str = "reference", quiet: bool = False):
+        simulation: str = "", *, dq_policy: str = "reference", quiet: bool = False,
+        padding: str = "avg"):
     """pycusaxs_cuda.run (cuda_bindings.cpp:172-195) on the MI355X engine.
+
+    Keyword-only extensions (defaults give the reference's behaviour):
+    dq_policy "exact" keeps the requested bin width; padding "given" pads
+    with explicit water of `water_model` plus the sodium/chlorine counts
+    (padding::given, Options.h:7-11, AtomCounter.cpp:77-103), computed for
+    the first frame's box, instead of the border-slab mean.
 
     Returns {"summary": str, "grid": [nx, ny, nz], "scaled_grid": [...]} where
     scaled_grid echoes the input ([0, 0, 0] when it was derived from
@@ -167,23 +174,32 @@ def run(obj_topology, topology: str, trajectory: str, grid: Sequence[int],
         raise ValueError("Topology path must not be empty.")
     if not trajectory:
         raise ValueError("Trajectory path must not be empty.")
+    if padding not in ("avg", "given"):
+        raise ValueError(f"padding must be 'avg' or 'given', got {padding!r}")
+    if padding == "given":
+        from .geometry import WATER_MODELS
+        if water_model not in WATER_MODELS:
+            raise ValueError(f"padding 'given' needs a water model ({', '.join(sorted(WATER_MODELS))}),"
+                             f" got {water_model!r}")
     sigma = float(np.float32(scale_factor))
     out_path = output or DEFAULT_OUTPUT
     sim = simulation or DEFAULT_SIMULATION
     ions = {"Na": int(sodium), "Cl": int(chlorine)}
     summary = summary_text(topology, trajectory, grid3, scaled3, int(begin), int(end), int(stride),
-                           int(order), sigma, float(bin_size), float(qcut), water_model, ions)
+                           int(order), sigma, float(bin_size), float(qcut), water_model, ions,
+                           padding)
     log = (lambda *a, **k: None) if quiet else print
     log(summary, end="")
     result = {"summary": summary, "grid": list(grid3),
               "scaled_grid": list(scaled3) if scaled3 else [0, 0, 0]}
     _run_frames(obj_topology, grid3, scaled3, int(begin), int(end), int(stride), int(order), sigma,
-                float(bin_size), float(qcut), out_path, sim, water_model, dq_policy, log)
+                float(bin_size), float(qcut), out_path, sim, water_model, dq_policy, log,
+                ions if padding == "given" else None)
     return result
 
 
 def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, qcut_req,
-                out_path, sim, water_model, dq_policy, log):
+                out_path, sim, water_model, dq_policy, log, given_ions=None):
     import torch
 
     from .engine import SaxsEngine, SystemSpec, resolve_grid
@@ -196,7 +212,11 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     spec = SystemSpec(n, nn, order, tuple(symbols), sigma, dq_req, qcut_req, dq_policy)
     n_atoms = len(idx)
     batch = max(1, FRAMES_PER_BATCH)
-    eng = SaxsEngine(spec, box0, max_atoms=max(n_atoms, 1), max_frames=batch)
+    pads = None
+    if given_ions is not None:
+        from .geometry import explicit_solvent_pads
+        pads = explicit_solvent_pads(np.diag(box0), n, water_model, given_ions)
+    eng = SaxsEngine(spec, box0, max_atoms=max(n_atoms, 1), max_frames=batch, pad_given=pads)
     h = eng.hist
     if h.qcut_reset:
         log("\n--------------------------------------------------")

@@ -899,9 +899,10 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
 
 // Per atom: the sum of the partial gradients of the tiles its stencil
 // reaches (slots recomputed from the same cell indices as the gather), minus
-// the bulk-solvent mean term (padg_e / count) grad(border weight), border
-// weight = (sum th_x)(sum th_y)(sum over border z of th_z).  Writes every
-// entry of grad (zero for atoms with an invalid element id).
+// the bulk-solvent mean term (padg_e / count) grad(border weight) when the
+// pad is the border mean, border weight = (sum th_x)(sum th_y)(sum over
+// border z of th_z).  Writes every entry of grad (zero for atoms with an
+// invalid element id).
 template <int P>
 __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjBuffers g) {
   const int f = blockIdx.y;
@@ -979,7 +980,8 @@ __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjB
       dSz += dth[o];
     }
   }
-  const double c = g.padg[(size_t)f * p.E + e] * p.inv_count;
+  // a given pad is a constant: no mean term
+  const double c = b.pad_given ? 0.0 : g.padg[(size_t)f * p.E + e] * p.inv_count;
   gp[0] = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
   gp[1] = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
   gp[2] = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);

@@ -222,6 +222,8 @@ struct saxs_plan {
   int2* keyslot = nullptr;
   float4* sorted = nullptr;
   unsigned long long* pad_sum = nullptr;
+  unsigned long long* pad_given = nullptr;  // [E] padding::given pads, 2^-32 units
+  std::vector<double> pad_given_host;       // empty: border-slab mean (padding::avg)
   float2* zbuf = nullptr;
   float2* ybuf = nullptr;
   double* partial = nullptr;
@@ -323,7 +325,7 @@ void free_plan(saxs_plan* pl) {
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
                   pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp,
-                  pl->gpart, pl->segoff};
+                  pl->gpart, pl->segoff, pl->pad_given};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete pl;
@@ -338,6 +340,7 @@ saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem)
   b.keyslot = pl->keyslot;
   b.sorted = pl->sorted;
   b.pad_sum = pl->pad_sum;
+  b.pad_given = pl->pad_given_host.empty() ? nullptr : pl->pad_given;
   b.srcx = pl->srcx;
   b.srcy = pl->srcy;
   b.zbuf = pl->zbuf;
@@ -532,7 +535,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 
 extern "C" {
 
-int32_t saxs_abi_version(void) { return 1; }
+int32_t saxs_abi_version(void) { return 2; }
 
 const char* saxs_last_error(void) { return g_err.c_str(); }
 
@@ -663,6 +666,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   }
   ALLOC(pl->sorted, F * N);
   ALLOC(pl->pad_sum, F * E);
+  ALLOC(pl->pad_given, E);
   ALLOC(pl->padg, F * E);
   ALLOC(pl->zbuf, F * E * p.n[0] * p.n[1] * p.kz);
   ALLOC(pl->ybuf, F * E * p.ky * p.n[0] * p.kz);
@@ -704,6 +708,29 @@ int saxs_plan_set_tables(saxs_plan* pl, int32_t lut_len, const int32_t* bin_lut,
   pl->last_frames_valid = false;
   HIP_TRY(hipMemcpy(pl->lut_bin, bin_lut, sizeof(int32_t) * lut_len, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pl->lut_ff, ff_lut, sizeof(float) * lut_len * pl->p.E, hipMemcpyHostToDevice));
+  return SAXS_OK;
+}
+
+int saxs_plan_set_padding(saxs_plan* pl, const double* pad) {
+  g_err.clear();
+  if (!pl) return fail(SAXS_EINVAL, "plan is NULL");
+  if (!pad) {
+    pl->pad_given_host.clear();
+    pl->last_frames_valid = false;
+    return SAXS_OK;
+  }
+  const int E = pl->p.E;
+  std::vector<unsigned long long> units(E);
+  for (int e = 0; e < E; ++e) {
+    // 2^40 atoms per voxel keeps pad * 2^32 and the column sums below 2^63
+    if (!(pad[e] >= 0.0 && pad[e] <= 1099511627776.0))
+      return fail(SAXS_EINVAL, "pad[%d] = %g outside [0, 2^40] atoms per voxel", e, pad[e]);
+    units[e] = (unsigned long long)std::llrint(pad[e] * 4294967296.0);
+  }
+  HIP_TRY(hipMemcpy(pl->pad_given, units.data(), sizeof(unsigned long long) * E,
+                    hipMemcpyHostToDevice));
+  pl->pad_given_host.assign(pad, pad + E);
+  pl->last_frames_valid = false;
   return SAXS_OK;
 }
 
@@ -969,7 +996,8 @@ extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t
     }
     hipLaunchKernelGGL(saxs::k_plain_pad, dim3(256), dim3(saxs::kWG), 0, st, p, pl->grid, pl->padp);
     hipLaunchKernelGGL(saxs::k_plain_embed, dim3((unsigned)((nn3 + 255) / 256)), dim3(256), 0, st,
-                       p, pl->grid, pl->sup, pl->padp);
+                       p, pl->grid, pl->sup, pl->padp,
+                       pl->pad_given_host.empty() ? -1.0 : pl->pad_given_host[e]);
     const int np = pl->plain_pairs;
     const size_t npairs = (size_t)q.nn[0] * (q.nn[1] / 2);
     const size_t lds_z = (size_t)np * q.lsz * sizeof(float2) + q.nn[2] * sizeof(float2);

@@ -54,6 +54,7 @@ struct Buffers {
   int2* keyslot;         // [F*N]
   float4* sorted;        // [F*N]  (r_x, r_y, r_z, atom index bits)
   unsigned long long* pad_sum;   // [F*E] border-slab weight, fixed point (to_fixed_d)
+  const unsigned long long* pad_given;  // [E] given pad in units of 2^-32, or NULL
   const int4* srcx;      // [ntx] {count, bin0, bin1, bin2}
   const int4* srcy;      // [nty]
   float2* zbuf;
@@ -70,6 +71,15 @@ struct Buffers {
   int* segoff;           // [F][nseg][keys per frame] segmented sort (nseg > 0)
   int nseg, seglen;      // segments per frame and atoms per segment
 };
+
+// Bulk-solvent pad of (frame, element) fe in units of 2^-32 atoms per voxel:
+// the border-slab mean (padding::avg, saxsDeviceKernels.cu:446-465) or the
+// caller's constant (padding::given, Options.h:7-11).
+__device__ __forceinline__ unsigned long long pad_units(const Buffers& b, const Params& p,
+                                                        size_t fe, int e) {
+  return b.pad_given ? b.pad_given[e]
+                     : (unsigned long long)llrint((double)b.pad_sum[fe] * p.inv_count);
+}
 
 // ----------------------------------------------------------- atom helpers --
 
@@ -536,8 +546,7 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   // mean subtracted inside the cell in fixed point (one rounding to float,
   // in units of 2^-32), zero padding beyond n_z.  The first FFT stage reads
   // the columns directly.
-  const unsigned long long pad =
-      (unsigned long long)llrint((double)b.pad_sum[f * p.E + e] * p.inv_count);
+  const unsigned long long pad = pad_units(b, p, (size_t)f * p.E + e, e);
   const int ty = p.ty;
   auto load = [&](unsigned l, unsigned z) -> float2 {
     const int c0 = 2 * (int)l;
@@ -692,8 +701,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
     __syncthreads();
 
     const size_t fe = (size_t)f * E + e;
-    const unsigned long long pad =
-        (unsigned long long)llrint((double)b.pad_sum[fe] * p.inv_count);
+    const unsigned long long pad = pad_units(b, p, fe, e);
     const int ty = p.ty;
     auto load = [&](unsigned l, unsigned z) -> float2 {
       const int c0 = 2 * (int)l;
@@ -856,8 +864,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
     // bulk-solvent mean in the columns' fixed-point units (a column outside
     // the cell keeps its zeros)
-    const unsigned long long pad =
-        (unsigned long long)llrint((double)b.pad_sum[fe] * p.inv_count);
+    const unsigned long long pad = pad_units(b, p, fe, e);
     // stage 0 (item = line l, butterfly j): columns 2l, 2l+1 -> line l in place
     if (threadIdx.x < nl * R2) {
       const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;

@@ -83,7 +83,9 @@ __global__ void __launch_bounds__(kWG) k_plain_pad(Params p, const float* grid, 
   }
 }
 
-__global__ void k_plain_embed(Params p, const float* grid, float* sup, const double* pad) {
+// given >= 0: padding::given constant instead of the border mean.
+__global__ void k_plain_embed(Params p, const float* grid, float* sup, const double* pad,
+                              double given) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)p.nn[0] * p.nn[1] * p.nn[2];
   if (i >= total) return;
@@ -93,7 +95,7 @@ __global__ void k_plain_embed(Params p, const float* grid, float* sup, const dou
   const int x = (int)(xy / p.nn[1]);
   float v = 0.0f;
   if (x < p.n[0] && y < p.n[1] && z < p.n[2]) {
-    const double m = pad[0] * p.inv_count;
+    const double m = given >= 0.0 ? given : pad[0] * p.inv_count;
     const float hi = (float)m, lo = (float)(m - (double)hi);
     v = (grid[((size_t)x * p.n[1] + y) * p.n[2] + z] - hi) - lo;
   }

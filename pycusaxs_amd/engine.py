@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import Optional, Sequence, Tuple
+from typing import Dict, Mapping, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -59,7 +59,7 @@ class SaxsEngine:
 
     def __init__(self, spec: SystemSpec, box, max_atoms: int, max_frames: int = 1,
                  tile: Tuple[int, int] = (0, 0), device: Optional[torch.device] = None,
-                 flags: int = 0):
+                 flags: int = 0, pad_given: Optional[Mapping[str, float]] = None):
         if not torch.cuda.is_available():
             raise RuntimeError("the MI355X SAXS engine needs a ROCm GPU (torch.cuda is unavailable)")
         self.lib = _lib.load()
@@ -103,6 +103,9 @@ class SaxsEngine:
         self.frames_done = 0
         self._generation = 0
         self._last = None
+        self.pad_given: Optional[Dict[str, float]] = None
+        if pad_given is not None:
+            self.set_padding(pad_given)
 
     # -- tables ---------------------------------------------------------
     @staticmethod
@@ -138,6 +141,24 @@ class SaxsEngine:
         self._box_key = key
         self.oc = oc
         return oc
+
+    def set_padding(self, pad_given: Optional[Mapping[str, float]]):
+        """Bulk-solvent pad mode.  None: each frame's pad per element is the
+        mean density of the border z-slabs (padding::avg, the reference's
+        mode).  A mapping {symbol: atoms per cell voxel}: that constant for
+        every frame (padding::given; elements it does not name get 0), e.g.
+        geometry.explicit_solvent_pads(...) for explicit water and ions."""
+        torch.cuda.current_stream().synchronize()
+        if pad_given is None:
+            _lib.check(self.lib.saxs_plan_set_padding(self._plan, None), "saxs_plan_set_padding")
+            self.pad_given = None
+            return
+        pads = {str(k): float(v) for k, v in dict(pad_given).items()}
+        arr = np.array([pads.get(s, 0.0) for s in self.spec.symbols], dtype=np.float64)
+        _lib.check(self.lib.saxs_plan_set_padding(self._plan, arr.ctypes.data_as(C.c_void_p)),
+                   "saxs_plan_set_padding")
+        self.pad_given = pads
+        self._last = None
 
     # -- compute --------------------------------------------------------
     def _prep(self, xyz: torch.Tensor, elem: torch.Tensor):

@@ -10,12 +10,13 @@ the engine's C ABI as plain arrays.
   shell_bins           exact bin of every shell n = |m|^2
   bspline_moduli       BSpmod.cpp:45-211
   form_factor_table    opsfact.cu:29-35 on the shell q values
+  explicit_solvent_pads  AtomCounter.cpp:56-103 (padding::given, Options.h:7-11)
 """
 from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Sequence, Tuple
+from typing import Dict, Mapping, Sequence, Tuple
 
 import numpy as np
 
@@ -181,3 +182,34 @@ def form_factor_table(symbols: Sequence[str], spacing: float, n_shells: int) -> 
         out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
                   + a4 * np.exp(-b4 * s2) + c)
     return out
+
+
+# g/cm^3 per water model, Avogadro's number and the molar mass of water
+# (AtomCounter.cpp:20-29).
+WATER_MODELS = {"SPCE": 0.998, "SPC": 0.998, "TIP3P": 0.982, "TIP4P": 0.999}
+_N_A = F32(6.022e23)
+_M_WATER = F32(18.015)
+
+
+def explicit_solvent_pads(lengths: Sequence[float], n: Sequence[int], water_model: str,
+                          ions: Mapping[str, int]) -> Dict[str, float]:
+    """Per-voxel atom counts of a box of explicit water plus ions: the values
+    of padding::given.  AtomCounter::calculateAtomCounts evaluates, in float32,
+    water molecules = (rho_w * L_x L_y L_z * 1e-24 / M_w) * N_A, then
+    O = molecules / cells, H = 2 molecules / cells and ion = count / cells
+    over the n_x n_y n_z cell grid (AtomCounter.cpp:56-103).  That code looks
+    the model up with std::map::at before its SPCE fallback, so an unknown
+    model is an error there too."""
+    if water_model not in WATER_MODELS:
+        raise ValueError(f"Unknown water model '{water_model}' "
+                         f"(expected one of {', '.join(sorted(WATER_MODELS))})")
+    v = F32(1.0)
+    for a in range(3):
+        v = F32(v * F32(lengths[a]))
+    grams = F32(F32(WATER_MODELS[water_model]) * F32(v * F32(1e-24)))
+    molecules = F32(F32(grams / _M_WATER) * _N_A)
+    cells = F32(int(n[0]) * int(n[1]) * int(n[2]))
+    pads = {"O": float(F32(molecules / cells)), "H": float(F32(F32(F32(2.0) * molecules) / cells))}
+    for ion, count in ions.items():
+        pads[str(ion)] = float(F32(F32(int(count)) / cells))
+    return pads

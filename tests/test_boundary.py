@@ -35,6 +35,9 @@ def call(**kw):
     (dict(topology=""), "Topology path"),
     (dict(trajectory=""), "Trajectory path"),
     (dict(order=5), "not supported"),
+    (dict(padding="none"), "padding must be"),
+    (dict(padding="given"), "needs a water model"),
+    (dict(padding="given", water_model="TIP5P"), "needs a water model"),
 ])
 def test_invalid_arguments_raise_value_error(kw, msg):
     with pytest.raises(ValueError, match=msg):

@@ -61,3 +61,28 @@ def test_run_with_scaled_grid_and_nvt(gpu, tmp_path):
     assert len(got) == len(want) and len(got) > 0
     for (qg, ig), (qw, iw) in zip(got, want):
         assert abs(ig - iw) <= 2e-5 * abs(iw)
+
+
+def test_run_given_padding(gpu, tmp_path):
+    """padding="given" (keyword-only extension): explicit SPCE water plus the
+    sodium/chlorine counts as the bulk-solvent pad, against the oracle with
+    the AtomCounter pads of the first frame's box."""
+    L = 37.0
+    base, syms = O.synthetic_system(3000, L, seed=24)
+    frames = [O.frame_positions(base, 24, f, 0.5) for f in range(3)]
+    topo = FakeTopology(frames, syms, np.diag([L] * 3))
+    out = tmp_path / "g.dat"
+    res = pycusaxs_cuda.run(topo, "sys.tpr", "traj.xtc", [48], end=2, output=str(out),
+                            scale_factor=2.0, bin_size=0.02, qcut=0.8, water_model="SPCE",
+                            sodium=4, chlorine=4, padding="given")
+    assert "Padding Mode  : given" in res["summary"]
+    symbols, ids = O.element_ids(syms)
+    pads = O.solvent_pads((L, L, L), (48, 48, 48), "SPCE", {"Na": 4, "Cl": 4})
+    plan = O.make_plan((48,) * 3, np.diag([L] * 3), 4, 2.0, None, 0.02, 0.8, pad_given=pads)
+    H, C = O.trajectory_histogram(plan, [(f, np.diag([L] * 3)) for f in frames], ids, symbols)
+    want = O.saxs_rows(plan, H, C)
+    got = parse(out)
+    assert len(got) == len(want) and len(got) > 0
+    for (qg, ig), (qw, iw) in zip(got, want):
+        assert ("%10.5f" % qg) == ("%10.5f" % qw)
+        assert abs(ig - iw) <= 1e-5 * abs(iw) + 1e-5 * 10 ** np.floor(np.log10(abs(iw)))

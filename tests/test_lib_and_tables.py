@@ -153,3 +153,35 @@ def test_cell_inverse_matches_reference():
     oc = np.array([float(v) for v in line.split()]).reshape(3, 3)
     assert np.array_equal(np.diag(oc).astype(np.float32), G.inverse_box_diag(np.diag([37, 41.5, 100])))
     assert np.array_equal(np.diag(oc).astype(np.float32), O.inverse_box(np.diag([37, 41.5, 100])))
+
+
+@needs_ref
+@pytest.mark.parametrize("lengths,n,model,ions", [
+    ((37.0, 37.0, 37.0), (64, 64, 64), "SPCE", {}),
+    ((100.0, 100.0, 100.0), (128, 128, 128), "TIP3P", {"Na": 150, "Cl": 150}),
+    ((80.5, 61.25, 90.0), (96, 72, 108), "TIP4P", {"K": 7, "Cl": 9, "Ca": 1}),
+    ((215.0, 215.0, 215.0), (256, 256, 256), "SPC", {"Mg": 3}),
+])
+def test_solvent_pads_match_reference(lengths, n, model, ions):
+    """padding::given values, bit for bit in float32, from the engine's host
+    code and from the oracle against AtomCounter compiled from its source."""
+    args = [*lengths, model, *n]
+    for k, v in ions.items():
+        args += [k, v]
+    got = {}
+    for line in ref("atomcount", *args):
+        sym, val = line.split()
+        got[sym] = np.float32(float(val))
+    mine = G.explicit_solvent_pads(lengths, n, model, ions)
+    orc = O.solvent_pads(lengths, n, model, ions)
+    assert set(got) == set(mine) == set(orc)
+    for s in got:
+        assert np.float32(mine[s]) == got[s], s
+        assert np.float32(orc[s]) == got[s], s
+
+
+@needs_ref
+def test_unknown_water_model_fails_like_reference():
+    assert ref("atomcount", 37, 37, 37, "XYZ", 64, 64, 64) == ["Invalid water model 'XYZ'. Using SPCE as default.", "error"]
+    with pytest.raises(ValueError):
+        G.explicit_solvent_pads((37.0,) * 3, (64,) * 3, "XYZ", {})

@@ -67,3 +67,31 @@ def test_pad_term_matters(exact_coords):
     # the border slab really carries density here, so the pad mean is nonzero
     g = O.spread(pos, O.inverse_box(box), plan.n, 4)
     assert O.border_pad(g, plan.border) > 0
+
+
+def test_given_pad_adjoint_matches_central_differences(exact_coords):
+    """padding::given: the pad is a constant, so the adjoint has no pad-mean
+    term; central differences of the given-mode forward pin that, with atoms
+    placed in the border slabs where the avg-mode term would be nonzero."""
+    L, n = 20.0, 16
+    box = np.diag([L] * 3)
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(0, L, size=(30, 3))
+    pos[:10, 2] = rng.uniform(0, 1.5, size=10)
+    syms = list(rng.choice(["O", "H", "C"], size=30))
+    symbols, ids = O.element_ids(syms)
+    pads = O.solvent_pads((L, L, L), (n, n, n), "SPCE", {})
+    plan = O.make_plan((n,) * 3, box, 4, 2.0, None, 0.0, 100.0, pad_given=pads)
+    H0, _ = O.frame_histogram(plan, pos, ids, symbols, box, literal=False)
+    gH = rng.normal(size=H0.shape) / np.maximum(np.abs(H0), 1.0)
+    grad = O.grad_linear_H(plan, pos, ids, symbols, box, gH)
+    h = 1e-5
+    for i in list(range(3)) + list(rng.choice(np.arange(10, 30), size=3, replace=False)):
+        for a in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, a] += h
+            pm[i, a] -= h
+            lp = gH @ O.frame_histogram(plan, pp, ids, symbols, box, literal=False)[0]
+            lm = gH @ O.frame_histogram(plan, pm, ids, symbols, box, literal=False)[0]
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - grad[i, a]) <= 1e-6 * max(1.0, np.abs(grad).max()), (i, a, fd, grad[i, a])

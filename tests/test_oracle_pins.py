@@ -242,3 +242,56 @@ def test_find_closest_product():
 
 def test_border_planes():
     assert [O.border_planes(n) for n in (64, 128, 256)] == [2, 4, 8]
+
+
+def test_given_pad_without_atoms_is_dirichlet_kernel():
+    """padding::given with no atoms: F_e = pad_e * (nn^3 delta - prod_a D(m_a))
+    with |D(m)| = |sin(pi m n/nn) / sin(pi m/nn)| (D(0) = n), so
+    I = (sum_e f_e pad_e)^2 prod_a D(m_a)^2 * modulus away from the origin;
+    zero at even m != 0 on every axis for nn = 2n."""
+    n, nn, L = 8, 16, 12.0
+    box = np.diag([L] * 3)
+    pads = {"O": 0.031, "H": 0.062}
+    plan = O.make_plan((n,) * 3, box, 4, 2.0, None, 0.0, 100.0, pad_given=pads)
+    symbols = ["H", "O"]
+    empty = np.zeros((0, 3))
+    for literal in (True, False):
+        I, inr, bins = O.frame_intensity(plan, empty, np.zeros(0, dtype=np.int64), symbols, box,
+                                         literal)
+        mx, my, mz = O.half_spectrum_m(plan.nn)
+
+        def dirichlet(m):
+            m = np.asarray(m, dtype=np.float64)
+            out = np.full(m.shape, float(n))
+            nz = m != 0
+            out[nz] = np.abs(np.sin(np.pi * m[nz] * n / nn) / np.sin(np.pi * m[nz] / nn))
+            return out
+
+        D = dirichlet(mx)[:, None, None] * dirichlet(my)[None, :, None] * dirichlet(mz)[None, None, :]
+        nsq = mx[:, None, None] ** 2 + my[None, :, None] ** 2 + mz[None, None, :] ** 2
+        q = float(plan.hp.spacing[0]) * np.sqrt(nsq.astype(np.float64))
+        amp = sum(O.form_factor(s, q) * pads[s] for s in symbols)
+        mod = plan.moduli[0][:, None, None] * plan.moduli[1][None, :, None] \
+            * plan.moduli[2][None, None, : I.shape[2]]
+        expect = amp ** 2 * D ** 2 * mod
+        sel = inr.copy()
+        sel[0, 0, 0] = False
+        assert sel.sum() > 100
+        assert np.allclose(I[sel], expect[sel], rtol=1e-10, atol=1e-12 * expect.max())
+        even = (mx[:, None, None] % 2 == 0) & (my[None, :, None] % 2 == 0) & (mz[None, None, :] % 2 == 0)
+        assert np.all(np.abs(I[sel & even]) <= 1e-20 * expect.max())
+
+
+def test_solvent_pads_closed_form():
+    """0.998 g/cm^3 water: n_w = 0.998e-24 V / 18.015 * 6.022e23 = 0.03336 V
+    molecules (0.3336 e/A^3 at 10 e per molecule); per voxel of a 64^3 cell in
+    a 37 A box, O = n_w / 64^3 and H = 2 O; ions count / 64^3."""
+    pads = O.solvent_pads((37.0, 37.0, 37.0), (64, 64, 64), "SPCE", {"Na": 10, "Cl": 12})
+    n_w = 0.998e-24 * 37.0 ** 3 / 18.015 * 6.022e23
+    assert abs(n_w / 37.0 ** 3 - 0.03336) < 1e-5
+    assert pads["O"] == pytest.approx(n_w / 64 ** 3, rel=1e-6)
+    assert pads["H"] == pytest.approx(2 * n_w / 64 ** 3, rel=1e-6)
+    assert pads["Na"] == pytest.approx(10 / 64 ** 3, rel=1e-7)
+    assert pads["Cl"] == pytest.approx(12 / 64 ** 3, rel=1e-7)
+    with pytest.raises(ValueError):
+        O.solvent_pads((37.0,) * 3, (64,) * 3, "NOPE", {})
