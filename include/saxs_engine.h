@@ -51,10 +51,16 @@ typedef struct saxs_plan saxs_plan;
  *  num_bins     histogram bins (int(qcut/dq)+1, saxsKernel.cu:247).
  *  border       z-slab half width d of the bulk-solvent border
  *               (saxsKernel.h:107-110).
- *  lut_len      entries of the shell tables, indexed by n = |m|^2; shells
- *               n >= lut_len are outside q <= qcut.
- *  bin_lut      [lut_len] bin of shell n, or -1 (outside qcut/bins).
- *  ff_lut       [num_elements * lut_len] form factor f_e(q(n)) (opsfact.cu:29-35).
+ *  lut_len      entries of the q tables.  Shell keys (kmax all 0): indexed
+ *               by n = |m|^2, shells n >= lut_len are outside q <= qcut.
+ *               Point keys (kmax not all 0): indexed by
+ *               (|m_x| (kmax[1]+1) + |m_y|) (kmax[2]+1) + |m_z| over the box
+ *               |m_a| <= kmax[a]; lut_len = prod (kmax[a]+1).  Point keys are
+ *               for orthorhombic cells whose reciprocal spacings differ
+ *               (q^2 = sum_a (2 pi oc_aa m_a n_a/nn_a)^2, saxsDeviceKernels.cu:
+ *               84-91), where q is not a function of |m|^2.
+ *  bin_lut      [lut_len] bin of each key, or -1 (outside qcut/bins).
+ *  ff_lut       [num_elements * lut_len] form factor f_e(q) per key (opsfact.cu:29-35).
  *  mod_x/y/z    [nn[a]] B-spline modulus tables lambda^2/|b|^2 (BSpmod.cpp:45-129).
  *  max_atoms, max_frames  capacity of the plan's workspace.
  *  tile_x, tile_y spreading column tile (0 = engine default).
@@ -83,6 +89,7 @@ typedef struct saxs_desc {
   int32_t tile_x;
   int32_t tile_y;
   int32_t flags;
+  int32_t kmax[3];
 } saxs_desc;
 
 /* Creates a plan: copies tables, builds FFT radix plans and twiddles,
@@ -96,6 +103,13 @@ int saxs_plan_create(const saxs_desc* desc, saxs_plan** out);
  * not exceed what the workspace was sized for).  Synchronous. */
 int saxs_plan_set_tables(saxs_plan* plan, int32_t lut_len, const int32_t* bin_lut,
                          const float* ff_lut);
+
+/* saxs_plan_set_tables for point keys over the box |m_a| <= kmax[a] (host
+ * int32 [3]); lut_len must be prod (kmax[a]+1).  Switches the plan to point
+ * keys (saxs_plan_set_tables switches it back to shell keys).  Replaces the
+ * per-frame histogram setup for a new box, saxsKernel.cu:489-526. */
+int saxs_plan_set_point_tables(saxs_plan* plan, const int32_t* kmax, int32_t lut_len,
+                               const int32_t* bin_lut, const float* ff_lut);
 
 /* Bulk-solvent pad mode.  pad == NULL: the pad of each (frame, element) is
  * the mean density over the border z-slabs (padding::avg, the reference's

@@ -170,8 +170,21 @@ def hist_params(n, nn, oc_diag, sigma, dq_req, qcut_req, dq_policy="reference") 
 
 def bin_of_n(nsq: int, s: np.float32, dq: np.float32, qcut: np.float32, num_bins: int) -> int:
     """Exact bin of the shell n = |m|^2: -1 outside q <= qcut or b >= num_bins."""
-    S, D, Q = Fraction(float(s)), Fraction(float(dq)), Fraction(float(qcut))
-    q2 = nsq * S * S
+    S = Fraction(float(s))
+    return bin_of_q2(nsq * S * S, dq, qcut, num_bins)
+
+
+def bin_of_m(m, spacing, dq, qcut, num_bins: int) -> int:
+    """Exact bin of the frequency m on an orthorhombic reciprocal grid:
+    q^2 = sum_a (m_a s_a)^2 as an exact rational of the float32 spacings
+    (the reference evaluates 2 pi oc_aa m_a per axis, saxsDeviceKernels.cu:84-91)."""
+    q2 = sum(int(m[a]) ** 2 * Fraction(float(spacing[a])) ** 2 for a in range(3))
+    return bin_of_q2(q2, dq, qcut, num_bins)
+
+
+def bin_of_q2(q2: Fraction, dq, qcut, num_bins: int) -> int:
+    """b = max{b : b^2 dq^2 <= q^2}, -1 if q^2 > qcut^2 or b >= num_bins."""
+    D, Q = Fraction(float(dq)), Fraction(float(qcut))
     if q2 > Q * Q:
         return -1
     b = int(math.isqrt(int(q2 / (D * D))))
@@ -256,18 +269,29 @@ class OraclePlan:
     # reference's only live mode).  A dict {symbol: atoms per cell voxel}:
     # padding::given, elements not in the dict padded with 0.
     pad_given: Optional[Dict[str, float]] = None
-    _bin_cache: Dict[int, int] = field(default_factory=dict)
+    _bin_cache: Dict[tuple, int] = field(default_factory=dict)
 
-    def bins_for(self, nsq: np.ndarray) -> np.ndarray:
-        s = self.hp.spacing[0]
-        uniq, inv = np.unique(nsq, return_inverse=True)
+    @property
+    def isotropic(self) -> bool:
+        s = self.hp.spacing
+        return bool(s[0] == s[1] == s[2])
+
+    def bins_for(self, am: np.ndarray) -> np.ndarray:
+        """Bins of frequencies given as rows (|m_x|, |m_y|, |m_z|): by the shell
+        n = |m|^2 when the spacings are equal, else point by point."""
+        iso = self.isotropic
+        keys = (am ** 2).sum(axis=1)[:, None] if iso else am
+        uniq, inv = np.unique(keys, axis=0, return_inverse=True)
         vals = np.empty(len(uniq), dtype=np.int64)
-        for i, v in enumerate(uniq):
-            v = int(v)
-            if v not in self._bin_cache:
-                self._bin_cache[v] = bin_of_n(v, s, self.hp.dq, self.hp.qcut, self.hp.num_bins)
-            vals[i] = self._bin_cache[v]
-        return vals[inv].reshape(nsq.shape)
+        hp = self.hp
+        for i, row in enumerate(uniq):
+            k = tuple(int(v) for v in row)
+            if k not in self._bin_cache:
+                self._bin_cache[k] = (bin_of_n(k[0], hp.spacing[0], hp.dq, hp.qcut, hp.num_bins)
+                                      if iso else bin_of_m(k, hp.spacing, hp.dq, hp.qcut,
+                                                           hp.num_bins))
+            vals[i] = self._bin_cache[k]
+        return vals[np.asarray(inv).reshape(-1)]
 
 
 def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
@@ -276,8 +300,6 @@ def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
     nn = tuple(int(x) for x in scaled) if scaled else tuple(find_closest_product(x, sigma) for x in n)
     oc = inverse_box(box)
     hp = hist_params(n, nn, oc, sigma, dq_req, qcut_req, dq_policy)
-    if not (hp.spacing[0] == hp.spacing[1] == hp.spacing[2]):
-        raise ValueError("oracle bins need an isotropic reciprocal grid")
     mods = tuple(bsp_moduli(nn[a], order) for a in range(3))
     pads = None if pad_given is None else {str(k): float(v) for k, v in dict(pad_given).items()}
     return OraclePlan(n, nn, order, hp, mods, border_planes(n[2]), pads)
@@ -388,12 +410,16 @@ def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=
     zero padding, which differs only at m = 0."""
     oc = inverse_box(box)
     mx, my, mz = half_spectrum_m(plan.nn)
-    nsq = (mx[:, None, None] ** 2 + my[None, :, None] ** 2 + mz[None, None, :] ** 2)
-    bins = np.full(nsq.shape, -1, dtype=np.int64)
-    cand = nsq <= int((float(plan.hp.qcut) / float(plan.hp.spacing[0])) ** 2) + 2
-    bins[cand] = plan.bins_for(nsq[cand])
+    sx, sy, sz = (float(v) for v in plan.hp.spacing)
+    q2 = ((mx * sx)[:, None, None] ** 2 + (my * sy)[None, :, None] ** 2
+          + (mz * sz)[None, None, :] ** 2)
+    bins = np.full(q2.shape, -1, dtype=np.int64)
+    cand = q2 <= float(plan.hp.qcut) ** 2 * (1.0 + 1e-9)
+    ii, jj, kk = np.nonzero(cand)
+    am = np.stack([np.abs(mx[ii]), np.abs(my[jj]), np.abs(mz[kk])], axis=1)
+    bins[cand] = plan.bins_for(am)
     inr = bins >= 0
-    q = float(plan.hp.spacing[0]) * np.sqrt(nsq.astype(np.float64))
+    q = np.sqrt(q2)
     Fs = []
     for e, sym in enumerate(symbols):
         sel = np.asarray(elem_ids) == e

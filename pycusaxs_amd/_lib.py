@@ -11,7 +11,7 @@ from pathlib import Path
 from ._build import LIB_PATH
 
 SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP = 0, -1, -2, -3, -4
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # every symbol include/saxs_engine.h declares
 EXPORTED = (
@@ -19,6 +19,7 @@ EXPORTED = (
     "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
     "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version", "saxs_plan_set_timing",
     "saxs_plan_kernel_times", "saxs_backward_ex", "saxs_plan_set_padding",
+    "saxs_plan_set_point_tables",
 )
 KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
                 "adj_x", "adj_y", "adj_z", "grad_sum", "plain")
@@ -43,6 +44,7 @@ class SaxsDesc(C.Structure):
         ("tile_x", C.c_int32),
         ("tile_y", C.c_int32),
         ("flags", C.c_int32),
+        ("kmax", C.c_int32 * 3),
     ]
 
 
@@ -72,6 +74,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_plan_create.argtypes = [C.POINTER(SaxsDesc), C.POINTER(vp)]
     lib.saxs_plan_set_tables.argtypes = [vp, i32, vp, vp]
     lib.saxs_plan_set_padding.argtypes = [vp, vp]
+    lib.saxs_plan_set_point_tables.argtypes = [vp, vp, i32, vp, vp]
     lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
@@ -88,7 +91,7 @@ def load(path: Path = LIB_PATH):
     for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
                  "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy",
                  "saxs_plan_set_timing", "saxs_plan_kernel_times", "saxs_backward_ex",
-                 "saxs_plan_set_padding"):
+                 "saxs_plan_set_padding", "saxs_plan_set_point_tables"):
         getattr(lib, name).restype = C.c_int
     if lib.saxs_abi_version() != ABI_VERSION:
         raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")

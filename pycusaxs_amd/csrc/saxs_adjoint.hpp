@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kWG) k_adj_x(Params p, Buffers b, AdjBuffers g
         const int ix = kept_to_index(jx, p.kx, p.Kx, nnx);
         const int mx = ix < p.nf[0] ? ix : ix - nnx;
         const int mz = k0 + k;
-        const int nsq = mx * mx + my * my + mz * mz;
+        const int nsq = shell_key(mx, my, mz, p);
         if (nsq < p.lut_len) {
           const float fv = ff[nsq];
           const float2 Fv = buf[k * p.lsx + ix];
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(kWG) k_adj_x(Params p, Buffers b, AdjBuffers g
       const int ix = kept_to_index(jx, p.kx, p.Kx, nnx);
       const int mx = ix < p.nf[0] ? ix : ix - nnx;
       const int kz = k0 + k;
-      nsq = mx * mx + my * my + kz * kz;
+      nsq = shell_key(mx, my, kz, p);
       if (nsq < p.lut_len) {
         const int bin = b.lut_bin[nsq];
         if (bin >= 0 && bin < nb) {
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers
         const int m = s + 16 * rr;
         if (m <= Kx || m >= N - Kx) {
           const int mx = m < p.nf[0] ? m : m - N;
-          nsq_o[u][o] = mx * mx + my * my + (k0 + k) * (k0 + k);
+          nsq_o[u][o] = shell_key(mx, my, k0 + k, p);
         }
       }
     }

@@ -116,11 +116,23 @@ bool tile_sources(int n, int t, int P, std::vector<int4>& out) {
   return true;
 }
 
-// kz for a shell table of lut_len entries (as derive_ranges computes it).
+// Largest kept |m| per axis: from the point-key box, or K with K^2 <= lut_len - 1
+// for shell tables (shells n = |m|^2 < lut_len).
+void kept_max(const saxs::Params& p, int lut_len, int K[3]) {
+  if (p.point_keys) {
+    for (int a = 0; a < 3; ++a) K[a] = p.kmax[a];
+    return;
+  }
+  int k = 0;
+  while ((long long)(k + 1) * (k + 1) < (long long)lut_len) ++k;
+  K[0] = K[1] = K[2] = k;
+}
+
+// kz for tables of lut_len entries (as derive_ranges computes it).
 int kz_of(const saxs::Params& p, int lut_len) {
-  int K = 0;
-  while ((long long)(K + 1) * (K + 1) < (long long)lut_len) ++K;
-  return std::min(K + 1, p.nn[2] / 2 + 1);
+  int K[3];
+  kept_max(p, lut_len, K);
+  return std::min(K[2] + 1, p.nn[2] / 2 + 1);
 }
 
 // Whether the spreading kernel and the z adjoint that will run for column
@@ -203,7 +215,8 @@ struct saxs_plan {
   saxs::Params p{};            // geometry; F/N/oc set per call
   saxs_desc desc{};
   int max_frames = 0, max_atoms = 0, nkeys_max = 0;
-  int lut_cap = 0;             // lut_len the workspace was sized for
+  int lut_cap = 0;             // q-table entries the tables were sized for
+  int kz_cap = 0, ky_cap = 0, nchx_cap = 0;  // kept ranges the workspace was sized for
   size_t bytes = 0;
   // tables
   int4* srcx = nullptr;
@@ -250,21 +263,21 @@ namespace {
 int derive_ranges(saxs_plan* pl, int lut_len) {
   saxs::Params& p = pl->p;
   p.lut_len = lut_len;
-  int K = 0;
-  while ((long long)(K + 1) * (K + 1) < (long long)lut_len) ++K;  // K^2 <= lut_len-1
-  auto kept = [&](int N, int& k, int& Kout) {
-    if (2 * K + 1 >= N) {
+  int K[3];
+  kept_max(p, lut_len, K);
+  auto kept = [&](int N, int Ka, int& k, int& Kout) {
+    if (2 * Ka + 1 >= N) {
       k = N;
       Kout = N / 2;
     } else {
-      k = 2 * K + 1;
-      Kout = K;
+      k = 2 * Ka + 1;
+      Kout = Ka;
     }
   };
-  kept(p.nn[0], p.kx, p.Kx);
-  kept(p.nn[1], p.ky, p.Ky);
+  kept(p.nn[0], K[0], p.kx, p.Kx);
+  kept(p.nn[1], K[1], p.ky, p.Ky);
   const int h = p.nn[2] / 2 + 1;
-  p.kz = std::min(K + 1, h);
+  p.kz = std::min(K[2] + 1, h);
 
   const int capy = fft_line_cap(p.fy), capx = fft_line_cap(p.fx);
   const size_t budget = 48 * 1024;
@@ -294,6 +307,41 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
   }
   if (saxs::yfft_lds_bytes(p) > kLdsLimit || (!pl->x_pruned && saxs::xfft_lds_bytes(p) > kLdsLimit))
     return fail(SAXS_EUNSUP, "pass LDS exceeds 64 KiB (num_bins %d too large?)", p.num_bins);
+  return SAXS_OK;
+}
+
+// A point-key box |m_a| <= kmax[a] must have lut_len = prod (kmax[a] + 1) entries.
+int check_point_box(const int32_t* kmax, int32_t lut_len) {
+  long long cells = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (kmax[a] < 0 || kmax[a] > 4096) return fail(SAXS_EINVAL, "kmax[%d] = %d invalid", a, kmax[a]);
+    cells *= kmax[a] + 1;
+  }
+  if (cells != lut_len)
+    return fail(SAXS_EINVAL, "lut_len %d != (kmax+1) product %lld", lut_len, cells);
+  return SAXS_OK;
+}
+
+// Replaces the q tables: shell keys (kmax NULL) or point keys.
+int set_tables(saxs_plan* pl, const int32_t* kmax, int32_t lut_len, const int32_t* bin_lut,
+               const float* ff_lut) {
+  if (!pl || !bin_lut || !ff_lut) return fail(SAXS_EINVAL, "NULL argument");
+  if (lut_len < 1 || lut_len > pl->lut_cap)
+    return fail(SAXS_EINVAL, "lut_len %d outside [1, %d] (recreate the plan)", lut_len, pl->lut_cap);
+  saxs::Params saved = pl->p;
+  pl->p.point_keys = kmax ? 1 : 0;
+  for (int a = 0; a < 3; ++a) pl->p.kmax[a] = kmax ? kmax[a] : 0;
+  int rc = derive_ranges(pl, lut_len);
+  if (!rc && (pl->p.kz > pl->kz_cap || pl->p.ky > pl->ky_cap || pl->p.nchx > pl->nchx_cap))
+    rc = fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
+  if (rc) {
+    pl->p = saved;
+    (void)derive_ranges(pl, saved.lut_len);          // restores the pass flags
+    return rc;
+  }
+  pl->last_frames_valid = false;
+  HIP_TRY(hipMemcpy(pl->lut_bin, bin_lut, sizeof(int32_t) * lut_len, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(pl->lut_ff, ff_lut, sizeof(float) * lut_len * pl->p.E, hipMemcpyHostToDevice));
   return SAXS_OK;
 }
 
@@ -535,7 +583,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 
 extern "C" {
 
-int32_t saxs_abi_version(void) { return 2; }
+int32_t saxs_abi_version(void) { return 3; }
 
 const char* saxs_last_error(void) { return g_err.c_str(); }
 
@@ -563,6 +611,11 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   if (d->lut_len < 1 || !d->bin_lut || !d->ff_lut || !d->mod_x || !d->mod_y || !d->mod_z)
     return fail(SAXS_EINVAL, "tables missing");
   if (d->max_atoms < 0 || d->max_frames < 1) return fail(SAXS_EINVAL, "capacity invalid");
+  const bool point_keys = d->kmax[0] > 0 || d->kmax[1] > 0 || d->kmax[2] > 0;
+  if (point_keys) {
+    const int rc = check_point_box(d->kmax, d->lut_len);
+    if (rc) return rc;
+  }
   if ((long long)d->max_frames * d->num_elements > 65535)
     return fail(SAXS_EINVAL, "max_frames * num_elements must not exceed 65535");
 
@@ -579,6 +632,8 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   p.E = d->num_elements;
   p.border = d->border;
   p.num_bins = d->num_bins;
+  p.point_keys = point_keys ? 1 : 0;
+  for (int a = 0; a < 3; ++a) p.kmax[a] = point_keys ? d->kmax[a] : 0;
   p.inv_count = 1.0 / ((double)d->n[0] * d->n[1] * (2 * d->border + 1));
   if (!factor_radices(p.nn[2], p.fz) || !factor_radices(p.nn[1], p.fy) ||
       !factor_radices(p.nn[0], p.fx) || p.nn[0] < 2 || p.nn[1] < 2 || p.nn[2] < 2) {
@@ -630,7 +685,19 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
     free_plan(pl);
     return rc;
   }
-  pl->lut_cap = d->lut_len;
+  // Room for a box change (NPT) that keeps one more frequency per axis, and
+  // for switching between shell and point tables: q tables up to the point
+  // box (K+2)^3, kept ranges one wider.
+  {
+    int K[3];
+    kept_max(p, d->lut_len, K);
+    const long long kb = (long long)std::max({K[0], K[1], K[2]}) + 2;
+    pl->lut_cap = (int)std::min<long long>(std::max<long long>(d->lut_len, kb * kb * kb), 1 << 26);
+    pl->lut_cap = std::max(pl->lut_cap, d->lut_len);
+    pl->kz_cap = std::min(p.kz + 1, p.nn[2] / 2 + 1);
+    pl->ky_cap = std::min(p.ky + 2, p.nn[1]);
+    pl->nchx_cap = p.nchx + 1;
+  }
   pl->max_frames = d->max_frames;
   pl->max_atoms = d->max_atoms;
   pl->nkeys_max = d->max_frames * p.E * p.nty * p.ntx;
@@ -654,8 +721,8 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->modx, (size_t)p.nn[0]);
   ALLOC(pl->mody, (size_t)p.nn[1]);
   ALLOC(pl->modz, (size_t)p.nn[2]);
-  ALLOC(pl->lut_bin, (size_t)d->lut_len);
-  ALLOC(pl->lut_ff, E * d->lut_len);
+  ALLOC(pl->lut_bin, (size_t)pl->lut_cap);
+  ALLOC(pl->lut_ff, E * pl->lut_cap);
   ALLOC(pl->counts, (size_t)pl->nkeys_max + 1);
   ALLOC(pl->offsets, (size_t)pl->nkeys_max + 1);
   ALLOC(pl->keyslot, F * N);
@@ -668,9 +735,9 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->pad_sum, F * E);
   ALLOC(pl->pad_given, E);
   ALLOC(pl->padg, F * E);
-  ALLOC(pl->zbuf, F * E * p.n[0] * p.n[1] * p.kz);
-  ALLOC(pl->ybuf, F * E * p.ky * p.n[0] * p.kz);
-  ALLOC(pl->partial, F * p.ky * p.nchx * 2 * p.num_bins);
+  ALLOC(pl->zbuf, F * E * p.n[0] * p.n[1] * pl->kz_cap);
+  ALLOC(pl->ybuf, F * E * pl->ky_cap * p.n[0] * pl->kz_cap);
+  ALLOC(pl->partial, F * pl->ky_cap * pl->nchx_cap * 2 * p.num_bins);
   ALLOC(pl->hframe, F * 2 * p.num_bins);
 #undef ALLOC
   const std::vector<float2> tz = twiddles(p.nn[2]), ty_ = twiddles(p.nn[1]), tx_ = twiddles(p.nn[0]);
@@ -692,23 +759,16 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
 
 int saxs_plan_set_tables(saxs_plan* pl, int32_t lut_len, const int32_t* bin_lut, const float* ff_lut) {
   g_err.clear();
-  if (!pl || !bin_lut || !ff_lut) return fail(SAXS_EINVAL, "NULL argument");
-  if (lut_len < 1 || lut_len > pl->lut_cap)
-    return fail(SAXS_EINVAL, "lut_len %d outside [1, %d] (recreate the plan)", lut_len, pl->lut_cap);
-  saxs::Params saved = pl->p;
-  int rc = derive_ranges(pl, lut_len);
-  if (rc) {
-    pl->p = saved;
-    return rc;
-  }
-  if (pl->p.kz > saved.kz || pl->p.ky > saved.ky || pl->p.nchx > saved.nchx) {
-    pl->p = saved;
-    return fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
-  }
-  pl->last_frames_valid = false;
-  HIP_TRY(hipMemcpy(pl->lut_bin, bin_lut, sizeof(int32_t) * lut_len, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(pl->lut_ff, ff_lut, sizeof(float) * lut_len * pl->p.E, hipMemcpyHostToDevice));
-  return SAXS_OK;
+  return set_tables(pl, nullptr, lut_len, bin_lut, ff_lut);
+}
+
+int saxs_plan_set_point_tables(saxs_plan* pl, const int32_t* kmax, int32_t lut_len,
+                               const int32_t* bin_lut, const float* ff_lut) {
+  g_err.clear();
+  if (!kmax) return fail(SAXS_EINVAL, "NULL argument");
+  const int rc = check_point_box(kmax, lut_len);
+  if (rc) return rc;
+  return set_tables(pl, kmax, lut_len, bin_lut, ff_lut);
 }
 
 int saxs_plan_set_padding(saxs_plan* pl, const double* pad) {
@@ -966,6 +1026,9 @@ extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t
   hipStream_t st = (hipStream_t)stream;
   saxs::Params p = call_params(pl, n_atoms, 1, oc_diag);     // cell geometry
   saxs::Params q = pl->pp;                                   // full grid geometry
+  q.lut_len = p.lut_len;                                     // current q tables
+  q.point_keys = p.point_keys;
+  for (int a = 0; a < 3; ++a) q.kmax[a] = p.kmax[a];
   q.F = 1;
   q.N = n_atoms;
   for (int a = 0; a < 3; ++a) q.oc[a] = oc_diag[a];

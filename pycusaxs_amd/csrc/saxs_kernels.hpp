@@ -42,6 +42,8 @@ struct Params {
   float oc[3];
   int kcy, nchy, kcx, nchx;
   int lsz, lsy, lsx;     // LDS line strides (complex) of the three passes
+  int point_keys;        // q tables indexed by points (|m_x|,|m_y|,|m_z|), not shells
+  int kmax[3];           // point-key box |m_a| <= kmax[a] (point_keys only)
   double inv_count;      // 1 / number of border voxels
   FftDesc fz, fy, fx;
 };
@@ -71,6 +73,17 @@ struct Buffers {
   int* segoff;           // [F][nseg][keys per frame] segmented sort (nseg > 0)
   int nseg, seglen;      // segments per frame and atoms per segment
 };
+
+// Index of frequency m in the q tables (bin and form factor): the shell
+// n = |m|^2 when the three reciprocal spacings are equal, else the point
+// (|m_x|, |m_y|, |m_z|) of the box |m_a| <= kmax[a] (orthorhombic cells, where
+// q^2 = sum_a (m_a s_a)^2 is not a function of |m|^2); lut_len when outside.
+__device__ __forceinline__ int shell_key(int mx, int my, int mz, const Params& p) {
+  if (!p.point_keys) return mx * mx + my * my + mz * mz;
+  const int ax = abs(mx), ay = abs(my), az = abs(mz);
+  if (ax > p.kmax[0] || ay > p.kmax[1] || az > p.kmax[2]) return p.lut_len;
+  return (ax * (p.kmax[1] + 1) + ay) * (p.kmax[2] + 1) + az;
+}
 
 // Bulk-solvent pad of (frame, element) fe in units of 2^-32 atoms per voxel:
 // the border-slab mean (padding::avg, saxsDeviceKernels.cu:446-465) or the
@@ -1148,7 +1161,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
         const int ix = (p.kx == nnx) ? jx : (jx <= p.Kx ? jx : nnx - (p.kx - jx));
         const int mx = ix < p.nf[0] ? ix : ix - nnx;
         const int mz = k0 + k;
-        const int nsq = mx * mx + my * my + mz * mz;
+        const int nsq = shell_key(mx, my, mz, p);
         if (nsq < p.lut_len) {
           const float fv = ff[nsq];
           const float2 Fv = buf[k * p.lsx + ix];
@@ -1167,7 +1180,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
       const int ix = (p.kx == nnx) ? jx : (jx <= p.Kx ? jx : nnx - (p.kx - jx));
       const int mx = ix < p.nf[0] ? ix : ix - nnx;
       const int kz = k0 + k;
-      const int nsq = mx * mx + my * my + kz * kz;
+      const int nsq = shell_key(mx, my, kz, p);
       if (nsq < p.lut_len) {
         const int bin = b.lut_bin[nsq];
         if (bin >= 0 && bin < nb) {
@@ -1272,7 +1285,7 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
           const int m = s + 16 * rr;
           if (m <= Kx || m >= N - Kx) {
             const int mx = m < p.nf[0] ? m : m - N;
-            const int nsq = mx * mx + my * my + mz * mz;
+            const int nsq = shell_key(mx, my, mz, p);
             if (nsq < p.lut_len) {
               const float fv = ff[nsq];
               A[u][o].x += fv * t[rr].x;
@@ -1296,7 +1309,7 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
         const int m = s + 16 * rr;
         if (m <= Kx || m >= N - Kx) {
           const int mx = m < p.nf[0] ? m : m - N;
-          const int nsq = mx * mx + my * my + mz * mz;
+          const int nsq = shell_key(mx, my, mz, p);
           if (nsq < p.lut_len) {
             const int bin = b.lut_bin[nsq];
             if (bin >= 0 && bin < nb) {

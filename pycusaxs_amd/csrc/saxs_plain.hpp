@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(kWG) k_plain_xfft_scatter(Params p, Buffers b,
     const int kz = k0 + k;
     const int mx = ix < p.nf[0] ? ix : ix - nnx;
     const int mz = kz < p.nf[2] ? kz : kz - p.nn[2];
-    const long long nsq = (long long)mx * mx + (long long)my * my + (long long)mz * mz;
+    const int nsq = shell_key(mx, my, mz, p);
     if (nsq < p.lut_len) {
       const float fv = b.lut_ff[(size_t)e * p.lut_len + nsq];
       const float2 F = buf[k * p.lsx + ix];
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(kWG) k_plain_hist(Params p, Buffers b, const f
     const int mx = x < p.nf[0] ? x : x - p.nn[0];
     const int my = y < p.nf[1] ? y : y - p.nn[1];
     const int mz = k < p.nf[2] ? k : k - p.nn[2];
-    const long long nsq = (long long)mx * mx + (long long)my * my + (long long)mz * mz;
+    const int nsq = shell_key(mx, my, mz, p);
     if (nsq >= p.lut_len) continue;
     const int bin = b.lut_bin[nsq];
     if (bin < 0 || bin >= nb) continue;

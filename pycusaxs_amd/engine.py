@@ -16,8 +16,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_table,
-                       hist_params, inverse_box_diag, shell_bins)
+from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_points,
+                       form_factor_table, hist_params, inverse_box_diag, point_bins, point_kmax,
+                       shell_bins)
 
 
 @dataclass
@@ -70,10 +71,9 @@ class SaxsEngine:
         self.oc = inverse_box_diag(box)
         self.hist: Histogram = hist_params(spec.n, spec.nn, self.oc, spec.sigma, spec.dq_req,
                                            spec.qcut_req, spec.dq_policy)
-        self._check_isotropic(self.hist)
         self.border = border_planes(spec.n[2])
         self.moduli = [bspline_moduli(spec.nn[a], spec.order) for a in range(3)]
-        bins, ff = self._shell_tables(self.hist, self.oc)
+        bins, ff, kmax = self._q_tables(self.hist)
         self._box_key = tuple(float(v) for v in self.oc)
         mods = [np.ascontiguousarray(m, dtype=np.float32) for m in self.moduli]
         desc = _lib.SaxsDesc()
@@ -93,6 +93,7 @@ class SaxsEngine:
         desc.max_frames = self.max_frames
         desc.tile_x, desc.tile_y = int(tile[0]), int(tile[1])
         desc.flags = int(flags)
+        desc.kmax[:] = list(kmax) if kmax is not None else [0, 0, 0]
         plan = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.saxs_plan_create(C.byref(desc), C.byref(plan)), "saxs_plan_create")
@@ -108,17 +109,22 @@ class SaxsEngine:
             self.set_padding(pad_given)
 
     # -- tables ---------------------------------------------------------
-    @staticmethod
-    def _check_isotropic(h: Histogram):
-        s = h.spacing
-        if not (s[0] == s[1] == s[2]):
-            raise ValueError("the engine needs equal reciprocal spacings on the three axes "
-                             f"(L_a*nn_a/n_a equal); got {s}")
-
-    def _shell_tables(self, hist: Histogram, oc):
-        bins = shell_bins(hist.spacing[0], hist.dq, hist.qcut, hist.num_bins)
-        ff = form_factor_table(self.spec.symbols, hist.spacing[0], len(bins)).astype(np.float32)
-        return np.ascontiguousarray(bins, dtype=np.int32), np.ascontiguousarray(ff)
+    def _q_tables(self, hist: Histogram):
+        """Bin and form-factor tables: per shell n = |m|^2 when the three
+        reciprocal spacings are equal (kmax None), else per point of the box
+        |m_a| <= kmax[a] (orthorhombic cells with unequal spacings)."""
+        s = hist.spacing
+        if s[0] == s[1] == s[2]:
+            bins = shell_bins(s[0], hist.dq, hist.qcut, hist.num_bins)
+            ff = form_factor_table(self.spec.symbols, s[0], len(bins))
+            kmax = None
+        else:
+            kmax = point_kmax(s, hist.qcut)
+            kmax = tuple(min(k, nn // 2) for k, nn in zip(kmax, self.spec.nn))
+            bins = point_bins(s, hist.dq, hist.qcut, hist.num_bins, kmax)
+            ff = form_factor_points(self.spec.symbols, s, kmax)
+        return (np.ascontiguousarray(bins, dtype=np.int32),
+                np.ascontiguousarray(ff, dtype=np.float32), kmax)
 
     def set_box(self, box):
         """Rebuilds the shell tables for a new box (NPT).  dq, qcut and
@@ -131,13 +137,20 @@ class SaxsEngine:
                         self.spec.qcut_req, self.spec.dq_policy)
         h = Histogram(self.hist.dq, self.hist.qcut, self.hist.num_bins, h.spacing,
                       self.hist.dq_reset, self.hist.qcut_reset)
-        self._check_isotropic(h)
-        bins, ff = self._shell_tables(h, oc)
+        bins, ff, kmax = self._q_tables(h)
         torch.cuda.current_stream().synchronize()
-        _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
-                                                 bins.ctypes.data_as(C.c_void_p),
-                                                 ff.ctypes.data_as(C.c_void_p)),
-                   "saxs_plan_set_tables")
+        if kmax is None:
+            _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
+                                                     bins.ctypes.data_as(C.c_void_p),
+                                                     ff.ctypes.data_as(C.c_void_p)),
+                       "saxs_plan_set_tables")
+        else:
+            km = np.array(kmax, dtype=np.int32)
+            _lib.check(self.lib.saxs_plan_set_point_tables(self._plan, km.ctypes.data_as(C.c_void_p),
+                                                           len(bins),
+                                                           bins.ctypes.data_as(C.c_void_p),
+                                                           ff.ctypes.data_as(C.c_void_p)),
+                       "saxs_plan_set_point_tables")
         self._box_key = key
         self.oc = oc
         return oc

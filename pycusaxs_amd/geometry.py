@@ -8,6 +8,7 @@ the engine's C ABI as plain arrays.
   inverse_box_diag     Cell.cpp:111-126
   hist_params          saxsKernel.cu:489-526 and :247
   shell_bins           exact bin of every shell n = |m|^2
+  point_bins           exact bin of every point of an orthorhombic grid
   bspline_moduli       BSpmod.cpp:45-211
   form_factor_table    opsfact.cu:29-35 on the shell q values
   explicit_solvent_pads  AtomCounter.cpp:56-103 (padding::given, Options.h:7-11)
@@ -131,6 +132,66 @@ def shell_bins(spacing: float, dq: float, qcut: float, num_bins: int) -> np.ndar
     for v in range(n_max + 1):
         b = math.isqrt((v * num) // den)
         out[v] = b if b < num_bins else -1
+    return out
+
+
+def point_kmax(spacing: Sequence[float], qcut: float) -> Tuple[int, int, int]:
+    """Largest |m_a| with (m_a s_a)^2 <= qcut^2 on each axis (exact)."""
+    qn, qd = float(qcut).as_integer_ratio()
+    out = []
+    for a in range(3):
+        sn, sd = float(spacing[a]).as_integer_ratio()
+        if sn <= 0 or qn < 0:
+            out.append(0)
+            continue
+        out.append(math.isqrt((qn * qn * sd * sd) // (sn * sn * qd * qd)))
+    return tuple(out)
+
+
+def point_bins(spacing: Sequence[float], dq: float, qcut: float, num_bins: int,
+               kmax: Sequence[int]) -> np.ndarray:
+    """bin of every point (|m_x|, |m_y|, |m_z|) of the box |m_a| <= kmax[a], laid
+    out as (|m_x| (kmax_y+1) + |m_y|) (kmax_z+1) + |m_z|, for an orthorhombic
+    reciprocal grid: q^2 = sum_a (m_a s_a)^2, bin = max{b : b^2 dq^2 <= q^2},
+    -1 outside q <= qcut or from num_bins on.  Exact in integers: the float
+    values are expanded to integer ratios over one common denominator."""
+    sr = [float(v).as_integer_ratio() for v in spacing]
+    dn, dd = float(dq).as_integer_ratio()
+    qn, qd = float(qcut).as_integer_ratio()
+    den = 1
+    for _, sd in sr:
+        den = den * sd * sd // math.gcd(den, sd * sd)
+    w = [sn * sn * (den // (sd * sd)) for sn, sd in sr]     # q^2 * den = sum m_a^2 w_a
+    kx, ky, kz = (int(v) for v in kmax)
+    out = np.empty((kx + 1) * (ky + 1) * (kz + 1), dtype=np.int32)
+    i = 0
+    for ax in range(kx + 1):
+        for ay in range(ky + 1):
+            base = ax * ax * w[0] + ay * ay * w[1]
+            for az in range(kz + 1):
+                q2 = base + az * az * w[2]                   # q^2 = q2 / den
+                if dn <= 0 or q2 * qd * qd > qn * qn * den:
+                    out[i] = -1
+                else:
+                    b = math.isqrt((q2 * dd * dd) // (dn * dn * den))
+                    out[i] = b if b < num_bins else -1
+                i += 1
+    return out
+
+
+def form_factor_points(symbols: Sequence[str], spacing: Sequence[float],
+                       kmax: Sequence[int]) -> np.ndarray:
+    """f_e(q) at q = sqrt(sum_a (m_a s_a)^2) for every point of the box
+    |m_a| <= kmax[a] in point_bins order (float64)."""
+    ax = [np.arange(int(k) + 1, dtype=np.float64) * float(s) for k, s in zip(kmax, spacing)]
+    q2 = (ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2)
+    q = np.sqrt(q2.reshape(-1))
+    s2 = (q / (4.0 * np.pi)) ** 2
+    out = np.empty((len(symbols), len(q)))
+    for e, sym in enumerate(symbols):
+        a1, a2, a3, a4, b1, b2, b3, b4, c = coefficients(sym)
+        out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
+                  + a4 * np.exp(-b4 * s2) + c)
     return out
 
 

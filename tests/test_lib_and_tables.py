@@ -185,3 +185,32 @@ def test_unknown_water_model_fails_like_reference():
     assert ref("atomcount", 37, 37, 37, "XYZ", 64, 64, 64) == ["Invalid water model 'XYZ'. Using SPCE as default.", "error"]
     with pytest.raises(ValueError):
         G.explicit_solvent_pads((37.0,) * 3, (64,) * 3, "XYZ", {})
+
+
+@pytest.mark.parametrize("Ls,n,dq,qcut,policy", [((40.0, 50.0, 60.0), 32, 0.0, 0.9, "reference"),
+                                                 ((33.0, 40.0, 37.0), 30, 0.0, 1.2, "reference"),
+                                                 ((60.0, 70.0, 80.0), 64, 0.01, 0.6, "exact")])
+def test_point_bins_match_oracle(Ls, n, dq, qcut, policy):
+    """Point-keyed tables of orthorhombic cells (engine host code, integer
+    arithmetic) against the oracle's exact rational rule, point by point."""
+    oc = G.inverse_box_diag(np.diag(Ls))
+    nn = (2 * n,) * 3
+    h = G.hist_params((n,) * 3, nn, oc, 2.0, dq, qcut, policy)
+    km = G.point_kmax(h.spacing, h.qcut)
+    ho = O.hist_params((n,) * 3, nn, O.inverse_box(np.diag(Ls)), 2.0, dq, qcut, policy)
+    assert tuple(float(v) for v in ho.spacing) == tuple(h.spacing)
+    for a in range(3):      # kmax is the largest |m_a| inside q <= qcut on its own
+        s = float(h.spacing[a])
+        assert (km[a] * s) ** 2 <= float(h.qcut) ** 2 * (1 + 1e-12)
+        assert ((km[a] + 1) * s) ** 2 > float(h.qcut) ** 2
+    bins = G.point_bins(h.spacing, h.dq, h.qcut, h.num_bins, km)
+    ff = G.form_factor_points(["C", "O"], h.spacing, km)
+    i = 0
+    for ax in range(km[0] + 1):
+        for ay in range(km[1] + 1):
+            for az in range(km[2] + 1):
+                assert bins[i] == O.bin_of_m((ax, ay, az), ho.spacing, ho.dq, ho.qcut, ho.num_bins)
+                q = np.sqrt(sum((m * float(s)) ** 2 for m, s in zip((ax, ay, az), h.spacing)))
+                assert ff[1, i] == pytest.approx(O.form_factor("O", q), rel=1e-12)
+                i += 1
+    assert i == len(bins) and (bins >= 0).sum() > 50

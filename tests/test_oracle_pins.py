@@ -295,3 +295,72 @@ def test_solvent_pads_closed_form():
     assert pads["Cl"] == pytest.approx(12 / 64 ** 3, rel=1e-7)
     with pytest.raises(ValueError):
         O.solvent_pads((37.0,) * 3, (64,) * 3, "NOPE", {})
+
+
+def test_on_node_atoms_direct_structure_factor_orthorhombic():
+    """As above on a 16 x 32 x 64 A box with an 8 x 16 x 32 cell grid (unequal
+    reciprocal spacings s_a = 2 pi / (L_a sigma); powers of two keep the
+    float32 cell coordinates of the nodes exact): q = sqrt(sum (m_a s_a)^2)
+    and the phase sum_a m_a g_a / nn_a per axis."""
+    Ls = (16.0, 32.0, 64.0)
+    n3 = (8, 16, 32)
+    box = np.diag(Ls)
+    plan = O.make_plan(n3, box, 4, 2.0, None, 0.0, 100.0)
+    assert not plan.isotropic and plan.nn == (16, 32, 64)
+    d = plan.border
+    rng = np.random.default_rng(6)
+    nodes = np.stack([rng.integers(4, n3[0], 5), rng.integers(4, n3[1], 5),
+                      rng.integers(d + 5, n3[2] - d, 5)], axis=1)
+    pos = (nodes * (np.array(Ls) / np.array(n3))).astype(np.float32)
+    syms = ["C", "O", "N", "O", "C"]
+    symbols, ids = O.element_ids(syms)
+    I, inr, bins = O.frame_intensity(plan, pos, ids, symbols, box)
+    mx, my, mz = O.half_spectrum_m(plan.nn)
+    s = [float(v) for v in plan.hp.spacing]
+    ii, jj, kk = np.nonzero(inr)
+    assert len(ii) > 1000
+    for t in range(0, len(ii), max(1, len(ii) // 60)):
+        i, j, k = ii[t], jj[t], kk[t]
+        m = np.array([mx[i], my[j], mz[k]])
+        q = math.sqrt(sum((m[a] * s[a]) ** 2 for a in range(3)))
+        A = 0j
+        for a, sym in zip(nodes, syms):
+            A += O.form_factor(sym, q) * np.exp(-2j * np.pi * sum(m[x] * a[x] / plan.nn[x]
+                                                                   for x in range(3)))
+        lam2 = 1.0
+        for ax in range(3):
+            x = math.pi * m[ax] / plan.nn[ax]
+            if m[ax] != 0:
+                g = lambda r: 1 + sum((x / (x + math.pi * jj_)) ** r + (x / (x - math.pi * jj_)) ** r
+                                      for jj_ in range(1, 51))
+                lam2 *= (g(4) / g(8)) ** 2
+        expect = abs(A) ** 2 * lam2
+        assert abs(I[i, j, k] - expect) <= 1e-9 * max(1.0, expect), (m, I[i, j, k], expect)
+
+
+def test_orthorhombic_counts_match_reference_loop():
+    """Counts on an orthorhombic grid against a literal enumeration of the
+    reference's loop (q^2 = sum (m_a s_a)^2, exact rationals of the float32
+    spacings, count 2 per half-spectrum point and 1 at the origin)."""
+    Ls, n3 = (30.0, 41.5, 52.0), (32, 32, 32)
+    plan = O.make_plan(n3, np.diag(Ls), 4, 2.0, None, 0.0, 0.9)
+    S = [Fraction(float(v)) for v in plan.hp.spacing]
+    D, Q = Fraction(float(plan.hp.dq)), Fraction(float(plan.hp.qcut))
+    C = [0] * plan.hp.num_bins
+    M = [int(float(Q / S[a])) + 1 for a in range(3)]
+    for i in range(-M[0], M[0] + 1):
+        for j in range(-M[1], M[1] + 1):
+            for k in range(0, M[2] + 1):
+                q2 = i * i * S[0] ** 2 + j * j * S[1] ** 2 + k * k * S[2] ** 2
+                if q2 > Q * Q:
+                    continue
+                b = 0
+                while (b + 1) ** 2 * D * D <= q2:
+                    b += 1
+                if b < plan.hp.num_bins:
+                    C[b] += 1 if (i, j, k) == (0, 0, 0) else 2
+    pos, syms = O.synthetic_system(200, 30.0, seed=3)
+    symbols, ids = O.element_ids(syms)
+    H, Cg = O.frame_histogram(plan, pos * np.array(Ls) / 30.0, ids, symbols, np.diag(Ls))
+    assert list(Cg.astype(int)) == C
+    assert sum(C) > 500
