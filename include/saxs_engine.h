@@ -58,12 +58,23 @@ typedef struct saxs_plan saxs_plan;
  *               |m_a| <= kmax[a]; lut_len = prod (kmax[a]+1).  Point keys are
  *               for orthorhombic cells whose reciprocal spacings differ
  *               (q^2 = sum_a (2 pi oc_aa m_a n_a/nn_a)^2, saxsDeviceKernels.cu:
- *               84-91), where q is not a function of |m|^2.
+ *               84-91), where q is not a function of |m|^2.  Triclinic cells
+ *               (cell with off-diagonal terms) key the signed point:
+ *               ((m_x+kmax[0]) (2kmax[1]+1) + m_y+kmax[1]) (2kmax[2]+1) +
+ *               m_z+kmax[2], lut_len = prod (2 kmax[a]+1).
  *  bin_lut      [lut_len] bin of each key, or -1 (outside qcut/bins).
  *  ff_lut       [num_elements * lut_len] form factor f_e(q) per key (opsfact.cu:29-35).
  *  mod_x/y/z    [nn[a]] B-spline modulus tables lambda^2/|b|^2 (BSpmod.cpp:45-129).
  *  max_atoms, max_frames  capacity of the plan's workspace.
  *  tile_x, tile_y spreading column tile (0 = engine default).
+ *  cell         row-major inverse cell M (fractional coordinates u = M x);
+ *               all zero or diagonal: orthorhombic, the per-call oc_diag is
+ *               used.  With off-diagonal terms the plan is triclinic: every
+ *               call's oc_diag must equal diag(M), and the q tables use
+ *               signed point keys.  The reference computes u = oc x with oc
+ *               from Cell.cpp:111-126, whose inverse drops the off-diagonal
+ *               terms of MDAnalysis's lower-triangular boxes; M is that
+ *               formula applied to the transposed box (the true inverse).
  *  flags        SAXS_FLAG_PER_ELEMENT_SPREAD: one spreading workgroup per
  *               (tile, element) instead of one per tile for all elements;
  *               SAXS_FLAG_NO_PRUNED_Z: full Stockham z and y transforms even
@@ -90,6 +101,7 @@ typedef struct saxs_desc {
   int32_t tile_y;
   int32_t flags;
   int32_t kmax[3];
+  float cell[9];
 } saxs_desc;
 
 /* Creates a plan: copies tables, builds FFT radix plans and twiddles,
@@ -105,11 +117,14 @@ int saxs_plan_set_tables(saxs_plan* plan, int32_t lut_len, const int32_t* bin_lu
                          const float* ff_lut);
 
 /* saxs_plan_set_tables for point keys over the box |m_a| <= kmax[a] (host
- * int32 [3]); lut_len must be prod (kmax[a]+1).  Switches the plan to point
- * keys (saxs_plan_set_tables switches it back to shell keys).  Replaces the
- * per-frame histogram setup for a new box, saxsKernel.cu:489-526. */
-int saxs_plan_set_point_tables(saxs_plan* plan, const int32_t* kmax, int32_t lut_len,
-                               const int32_t* bin_lut, const float* ff_lut);
+ * int32 [3]) and the cell of the new box (host float [9] as in saxs_desc, or
+ * NULL for orthorhombic); lut_len must be prod (kmax[a]+1), or
+ * prod (2 kmax[a]+1) for a triclinic cell.  Switches the plan to point keys
+ * (saxs_plan_set_tables switches it back to shell keys and an orthorhombic
+ * cell).  Replaces the per-frame histogram setup for a new box,
+ * saxsKernel.cu:489-526. */
+int saxs_plan_set_point_tables(saxs_plan* plan, const float* cell, const int32_t* kmax,
+                               int32_t lut_len, const int32_t* bin_lut, const float* ff_lut);
 
 /* Bulk-solvent pad mode.  pad == NULL: the pad of each (frame, element) is
  * the mean density over the border z-slabs (padding::avg, the reference's

@@ -13,7 +13,7 @@ where they decide integers (grid cell of an atom, FFT size, bin parameters):
           hist_params (dq/qcut reset) saxsKernel.cu:489-526, :247
           border_planes               saxsKernel.h:107-110, saxsKernel.cu:68-70
           bsp_moduli                  BSpmod.cpp:45-129, BSpmod.h:22
-          inverse_box                 Cell.cpp:111-126
+          inverse_box(_matrix)        Cell.cpp:111-126 (applied to the transposed box)
   frame   spread (S1)                 saxsDeviceKernels.cu:338-398, Splines.cu:11-74
           pad (S2)                    saxsDeviceKernels.cu:446-465
           pad, "given" mode           Options.h:7-11, AtomCounter.cpp:56-103
@@ -138,6 +138,44 @@ def inverse_box(box) -> np.ndarray:
     return np.array([F32(1.0 / float(box[a, a])) for a in range(3)], dtype=np.float32)
 
 
+def inverse_box_matrix(box) -> np.ndarray:
+    """M with fractional coordinates u = M x for a box whose rows are the cell
+    vectors a = (a_x, 0, 0), b = (b_x, b_y, 0), c = (c_x, c_y, c_z) (the
+    MDAnalysis/GROMACS lower-triangular form), so x = B^T u and M = (B^T)^-1.
+    B^T is upper-triangular, whose inverse is the formula of Cell.cpp:118-125
+    (float32 products, 1.0/x in double): applied to B^T it gives the true
+    inverse, where the reference applies it to B itself and loses the
+    off-diagonal terms (Cell.cpp:111-126)."""
+    B = np.asarray(box, dtype=np.float32).reshape(3, 3)
+    if B[0, 1] != 0 or B[0, 2] != 0 or B[1, 2] != 0:
+        raise ValueError("box must be lower-triangular (rows a, b, c with a || x, b in xy)")
+    co = B.T
+    M = np.zeros((3, 3), dtype=np.float32)
+    M[0, 0] = F32(1.0 / float(co[0, 0]))
+    M[0, 1] = F32(-co[0, 1] / F32(co[0, 0] * co[1, 1]))
+    M[0, 2] = F32(F32(F32(co[0, 1] * co[1, 2]) - F32(co[0, 2] * co[1, 1]))
+                  / F32(F32(co[0, 0] * co[1, 1]) * co[2, 2]))
+    M[1, 1] = F32(1.0 / float(co[1, 1]))
+    M[1, 2] = F32(-co[1, 2] / F32(co[1, 1] * co[2, 2]))
+    M[2, 2] = F32(1.0 / float(co[2, 2]))
+    return M
+
+
+def reciprocal_basis(M, n, nn) -> np.ndarray:
+    """Rows b_a (float32) with q = sum_a m_a b_a for FFT index m: the
+    transform of a function of u = M x at frequency m_a n_a / nn_a per cell is
+    exp(-i q.x) with q_j = 2 pi sum_a M_aj m_a n_a / nn_a.  Each entry is
+    evaluated like the spacing of hist_params, so a diagonal M gives
+    diag(s_a) bit for bit."""
+    tp = F32(F32(2.0) * F32(math.pi))
+    b = np.zeros((3, 3), dtype=np.float32)
+    for a in range(3):
+        ratio = F32(F32(nn[a]) / F32(n[a]))
+        for j in range(3):
+            b[a, j] = F32(F32(tp * F32(M[a, j])) / ratio)
+    return b
+
+
 @dataclass
 class HistParams:
     dq: np.float32          # effective bin width (float32, saxsKernel.cu:506)
@@ -179,6 +217,16 @@ def bin_of_m(m, spacing, dq, qcut, num_bins: int) -> int:
     q^2 = sum_a (m_a s_a)^2 as an exact rational of the float32 spacings
     (the reference evaluates 2 pi oc_aa m_a per axis, saxsDeviceKernels.cu:84-91)."""
     q2 = sum(int(m[a]) ** 2 * Fraction(float(spacing[a])) ** 2 for a in range(3))
+    return bin_of_q2(q2, dq, qcut, num_bins)
+
+
+def bin_of_vec(m, basis, dq, qcut, num_bins: int) -> int:
+    """Exact bin of frequency m on a triclinic reciprocal grid: q = sum_a m_a b_a
+    with the float32 rows b_a as exact rationals."""
+    q2 = Fraction(0)
+    for j in range(3):
+        qj = sum(int(m[a]) * Fraction(float(basis[a, j])) for a in range(3))
+        q2 += qj * qj
     return bin_of_q2(q2, dq, qcut, num_bins)
 
 
@@ -269,27 +317,42 @@ class OraclePlan:
     # reference's only live mode).  A dict {symbol: atoms per cell voxel}:
     # padding::given, elements not in the dict padded with 0.
     pad_given: Optional[Dict[str, float]] = None
+    # fractional-coordinate matrix u = M x and reciprocal rows b_a (q = sum m_a b_a)
+    M: Optional[np.ndarray] = None
+    basis: Optional[np.ndarray] = None
     _bin_cache: Dict[tuple, int] = field(default_factory=dict)
+
+    @property
+    def triclinic(self) -> bool:
+        return self.M is not None and bool(np.any(self.M != np.diag(np.diag(self.M))))
 
     @property
     def isotropic(self) -> bool:
         s = self.hp.spacing
-        return bool(s[0] == s[1] == s[2])
+        return bool(s[0] == s[1] == s[2]) and not self.triclinic
 
-    def bins_for(self, am: np.ndarray) -> np.ndarray:
-        """Bins of frequencies given as rows (|m_x|, |m_y|, |m_z|): by the shell
-        n = |m|^2 when the spacings are equal, else point by point."""
-        iso = self.isotropic
-        keys = (am ** 2).sum(axis=1)[:, None] if iso else am
+    def bins_for(self, m: np.ndarray) -> np.ndarray:
+        """Bins of frequencies given as rows (m_x, m_y, m_z): by the shell
+        n = |m|^2 when the spacings are equal, by (|m_x|, |m_y|, |m_z|) for other
+        orthorhombic grids, by the signed m for triclinic cells."""
+        iso, tric = self.isotropic, self.triclinic
+        if iso:
+            keys = (m ** 2).sum(axis=1)[:, None]
+        else:
+            keys = m if tric else np.abs(m)
         uniq, inv = np.unique(keys, axis=0, return_inverse=True)
         vals = np.empty(len(uniq), dtype=np.int64)
         hp = self.hp
         for i, row in enumerate(uniq):
             k = tuple(int(v) for v in row)
             if k not in self._bin_cache:
-                self._bin_cache[k] = (bin_of_n(k[0], hp.spacing[0], hp.dq, hp.qcut, hp.num_bins)
-                                      if iso else bin_of_m(k, hp.spacing, hp.dq, hp.qcut,
-                                                           hp.num_bins))
+                if iso:
+                    b = bin_of_n(k[0], hp.spacing[0], hp.dq, hp.qcut, hp.num_bins)
+                elif tric:
+                    b = bin_of_vec(k, self.basis, hp.dq, hp.qcut, hp.num_bins)
+                else:
+                    b = bin_of_m(k, hp.spacing, hp.dq, hp.qcut, hp.num_bins)
+                self._bin_cache[k] = b
             vals[i] = self._bin_cache[k]
         return vals[np.asarray(inv).reshape(-1)]
 
@@ -298,11 +361,12 @@ def make_plan(n, box, order=4, sigma=2.0, scaled=None, dq_req=0.0, qcut_req=0.0,
               dq_policy="reference", pad_given=None) -> OraclePlan:
     n = tuple(int(x) for x in n)
     nn = tuple(int(x) for x in scaled) if scaled else tuple(find_closest_product(x, sigma) for x in n)
-    oc = inverse_box(box)
-    hp = hist_params(n, nn, oc, sigma, dq_req, qcut_req, dq_policy)
+    M = inverse_box_matrix(box)
+    hp = hist_params(n, nn, np.diag(M), sigma, dq_req, qcut_req, dq_policy)
     mods = tuple(bsp_moduli(nn[a], order) for a in range(3))
     pads = None if pad_given is None else {str(k): float(v) for k, v in dict(pad_given).items()}
-    return OraclePlan(n, nn, order, hp, mods, border_planes(n[2]), pads)
+    return OraclePlan(n, nn, order, hp, mods, border_planes(n[2]), pads, M,
+                      reciprocal_basis(M, n, nn))
 
 
 # Water densities in g/cm^3 and the constants of AtomCounter.cpp:20-29.
@@ -341,17 +405,32 @@ def solvent_pads(lengths, n, water_model: str, ions: Dict[str, int]) -> Dict[str
 EXACT_COORDS = False
 
 
-def grid_coords(pos, oc_diag, n):
+def grid_coords(pos, oc, n):
     """Cell index m and fraction w per atom and axis, following the
-    reference's float32/float64 mix (saxsDeviceKernels.cu:352-368)."""
+    reference's float32/float64 mix (saxsDeviceKernels.cu:352-368).  `oc` is
+    the diagonal of an orthorhombic inverse box, or the full matrix M; for a
+    non-diagonal M, u_a = (M_a0 x + M_a1 y) + M_a2 z with every product and
+    sum rounded to float32 (no fused multiply-add)."""
+    oc = np.asarray(oc, dtype=np.float32)
+    if oc.ndim == 2 and not np.any(oc != np.diag(np.diag(oc))):
+        oc = np.diag(oc).copy()
     if EXACT_COORDS:
         pos = np.asarray(pos, dtype=np.float64)
-        ud = pos * np.asarray(oc_diag, dtype=np.float64)[None, :]
+        if oc.ndim == 2:
+            ud = pos @ oc.astype(np.float64).T
+        else:
+            ud = pos * oc.astype(np.float64)[None, :]
         r = np.asarray(n, dtype=np.float64)[None, :] * (ud - np.rint(ud - 0.5))
         m = np.floor(r).astype(np.int64)
         return m, r - m
     pos = np.asarray(pos, dtype=np.float32)
-    u = (pos * oc_diag[None, :].astype(np.float32)).astype(np.float32)
+    if oc.ndim == 2:
+        u = np.empty_like(pos)
+        for a in range(3):
+            t = (oc[a, 0] * pos[:, 0]).astype(np.float32) + (oc[a, 1] * pos[:, 1]).astype(np.float32)
+            u[:, a] = t.astype(np.float32) + (oc[a, 2] * pos[:, 2]).astype(np.float32)
+    else:
+        u = (pos * oc[None, :]).astype(np.float32)
     ud = u.astype(np.float64)
     r = (np.asarray(n, dtype=np.float64)[None, :] * (ud - np.rint(ud - 0.5))).astype(np.float32)
     m = r.astype(np.int64)                      # truncation of non-negative floats
@@ -408,16 +487,16 @@ def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=
     literal=True embeds exactly like the reference (pad everywhere outside
     the cell, S3); literal=False transforms (rho - pad) inside the cell with
     zero padding, which differs only at m = 0."""
-    oc = inverse_box(box)
+    oc = inverse_box_matrix(box)
     mx, my, mz = half_spectrum_m(plan.nn)
-    sx, sy, sz = (float(v) for v in plan.hp.spacing)
-    q2 = ((mx * sx)[:, None, None] ** 2 + (my * sy)[None, :, None] ** 2
-          + (mz * sz)[None, None, :] ** 2)
+    bv = plan.basis.astype(np.float64)
+    qv = (mx[:, None, None, None] * bv[0] + my[None, :, None, None] * bv[1]
+          + mz[None, None, :, None] * bv[2])
+    q2 = (qv ** 2).sum(axis=3)
     bins = np.full(q2.shape, -1, dtype=np.int64)
     cand = q2 <= float(plan.hp.qcut) ** 2 * (1.0 + 1e-9)
     ii, jj, kk = np.nonzero(cand)
-    am = np.stack([np.abs(mx[ii]), np.abs(my[jj]), np.abs(mz[kk])], axis=1)
-    bins[cand] = plan.bins_for(am)
+    bins[cand] = plan.bins_for(np.stack([mx[ii], my[jj], mz[kk]], axis=1))
     inr = bins >= 0
     q = np.sqrt(q2)
     Fs = []
@@ -451,7 +530,10 @@ def frame_intensity(plan, pos, elem_ids, symbols, box, literal=True):
 
 def histogram(plan, I, inr, bins):
     """S6, literally: every in-range half-spectrum point adds I[i,j,k] +
-    I[-i,-j,k] with count 2; the origin adds I once with count 1."""
+    I[-i,-j,k] with count 2; the origin adds I once with count 1.  In a
+    triclinic cell (-i,-j,k) is not at the same |q| as (i,j,k), so each point
+    adds 2 I at its own bin instead (I(-m) = I(m) at q(-m) = -q(m): the point
+    stands for itself and its Hermitian mirror)."""
     nb = plan.hp.num_bins
     H = np.zeros(nb)
     C = np.zeros(nb)
@@ -461,7 +543,8 @@ def histogram(plan, I, inr, bins):
     jb = (nny - jj) % nny
     b = bins[ii, jj, kk]
     origin = (ii == 0) & (jj == 0) & (kk == 0)
-    v = np.where(origin, I[ii, jj, kk], I[ii, jj, kk] + I[ib, jb, kk])
+    mirror = 2.0 * I[ii, jj, kk] if plan.triclinic else I[ii, jj, kk] + I[ib, jb, kk]
+    v = np.where(origin, I[ii, jj, kk], mirror)
     c = np.where(origin, 1.0, 2.0)
     np.add.at(H, b, v)
     np.add.at(C, b, c)
@@ -522,7 +605,7 @@ def grad_linear_H(plan, pos, elem_ids, symbols, box, gH):
     form factor -> R2C^T (half-spectrum inverse DFT, real part) -> crop +
     pad-mean term -> B-spline gradient gather."""
     pos = np.asarray(pos, dtype=np.float64 if EXACT_COORDS else np.float32)
-    oc = inverse_box(box)
+    oc = inverse_box_matrix(box)
     n, nn, p = plan.n, plan.nn, plan.order
     Fs, inr, bins, q = frame_spectrum_terms(plan, pos, elem_ids, symbols, box, literal=False)
     A = np.zeros(inr.shape, dtype=np.complex128)
@@ -565,11 +648,11 @@ def grad_linear_H(plan, pos, elem_ids, symbols, box, gH):
             i = m[:, a][:, None] - p + np.arange(p)[None, :]
             idx.append(np.where(i < 0, i + n[a], i))
         G = drho[idx[0][:, :, None, None], idx[1][:, None, :, None], idx[2][:, None, None, :]]
-        scale = [float(n[a]) * float(oc[a]) for a in range(3)]
-        gx = np.einsum("aijk,ai,aj,ak->a", G, dth[0], th[1], th[2]) * scale[0]
-        gy = np.einsum("aijk,ai,aj,ak->a", G, th[0], dth[1], th[2]) * scale[1]
-        gz = np.einsum("aijk,ai,aj,ak->a", G, th[0], th[1], dth[2]) * scale[2]
-        grad[sel] = np.stack([gx, gy, gz], axis=1)
+        # dL/dr_a (r_a = n_a u_a + const), then x through u = M x
+        gr = np.stack([np.einsum("aijk,ai,aj,ak->a", G, dth[0], th[1], th[2]),
+                       np.einsum("aijk,ai,aj,ak->a", G, th[0], dth[1], th[2]),
+                       np.einsum("aijk,ai,aj,ak->a", G, th[0], th[1], dth[2])], axis=1)
+        grad[sel] = (gr * np.asarray(n, dtype=np.float64)[None, :]) @ oc.astype(np.float64)
     return grad
 
 

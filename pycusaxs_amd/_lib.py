@@ -11,7 +11,7 @@ from pathlib import Path
 from ._build import LIB_PATH
 
 SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP = 0, -1, -2, -3, -4
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # every symbol include/saxs_engine.h declares
 EXPORTED = (
@@ -45,6 +45,7 @@ class SaxsDesc(C.Structure):
         ("tile_y", C.c_int32),
         ("flags", C.c_int32),
         ("kmax", C.c_int32 * 3),
+        ("cell", C.c_float * 9),
     ]
 
 
@@ -74,7 +75,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_plan_create.argtypes = [C.POINTER(SaxsDesc), C.POINTER(vp)]
     lib.saxs_plan_set_tables.argtypes = [vp, i32, vp, vp]
     lib.saxs_plan_set_padding.argtypes = [vp, vp]
-    lib.saxs_plan_set_point_tables.argtypes = [vp, vp, i32, vp, vp]
+    lib.saxs_plan_set_point_tables.argtypes = [vp, vp, vp, i32, vp, vp]
     lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]

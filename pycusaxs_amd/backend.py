@@ -16,8 +16,9 @@ which is the reference's double buffer (RunSaxs.cu:141-170).
 
 Deliberate differences (the reference's behaviour there is a silent failure
 or undefined): a malformed frame dict, an exception raised by the frame
-iterator, an element with no form factor, and a triclinic box raise instead
-of ending the run early / reading garbage.
+iterator and an element with no form factor raise instead of ending the run
+early / reading garbage; a triclinic box uses the true inverse of the cell
+(Cell.cpp:111-126 drops its off-diagonal terms).
 """
 from __future__ import annotations
 

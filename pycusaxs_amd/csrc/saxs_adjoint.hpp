@@ -544,8 +544,9 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
   // gather: every atom whose stencil reaches this tile accumulates
   // sum drho~ grad(theta_x theta_y theta_z) over its in-tile stencil points
   const int total = s_pref[9];
-  const float sx = (float)p.n[0] * p.oc[0], sy = (float)p.n[1] * p.oc[1],
-              sz = (float)p.n[2] * p.oc[2];
+  const float sx = p.tric ? (float)p.n[0] : (float)p.n[0] * p.oc[0],
+              sy = p.tric ? (float)p.n[1] : (float)p.n[1] * p.oc[1],
+              sz = p.tric ? (float)p.n[2] : (float)p.n[2] * p.oc[2];
   for (int base = 0; base < total; base += kSpreadRecs) {
     const int chunk = min(kSpreadRecs, total - base);
     __syncthreads();
@@ -653,8 +654,9 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
   const int nz = p.n[2];
   const int plane = p.ty * cs;
   const int nslot = g.nsx * g.nsy;
-  const float sx = (float)p.n[0] * p.oc[0], sy = (float)p.n[1] * p.oc[1],
-              sz = (float)p.n[2] * p.oc[2];
+  const float sx = p.tric ? (float)p.n[0] : (float)p.n[0] * p.oc[0],
+              sy = p.tric ? (float)p.n[1] : (float)p.n[1] * p.oc[1],
+              sz = p.tric ? (float)p.n[2] : (float)p.n[2] * p.oc[2];
   for (int i = threadIdx.x; i < count; i += blockDim.x) {
     const float4 rec = recs[i];
     const int mx = cell_index(rec.x, p.n[0]);
@@ -917,11 +919,12 @@ __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjB
     return;
   }
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
-  float w[3];
+  float w[3], rr[3];
   int base[3];
+  cell_coords(x, p, rr);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float r = cell_coord(x[a], p.oc[a], p.n[a]);
+    const float r = rr[a];
     const int m = cell_index(r, p.n[a]);
     w[a] = r - (float)m;
     base[a] = wrap_base(m, P, p.n[a]);
@@ -982,9 +985,18 @@ __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjB
   }
   // a given pad is a constant: no mean term
   const double c = b.pad_given ? 0.0 : g.padg[(size_t)f * p.E + e] * p.inv_count;
-  gp[0] = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
-  gp[1] = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
-  gp[2] = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
+  if (!p.tric) {
+    gp[0] = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
+    gp[1] = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
+    gp[2] = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
+    return;
+  }
+  // triclinic: the sums are dL/dr_a n_a; dL/dx_j = sum_a (dL/dr_a n_a) M_aj
+  const float t0 = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0]);
+  const float t1 = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1]);
+  const float t2 = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) gp[j] = t0 * p.cm[j] + t1 * p.cm[3 + j] + t2 * p.cm[6 + j];
 }
 
 }  // namespace saxs

@@ -310,27 +310,42 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
   return SAXS_OK;
 }
 
-// A point-key box |m_a| <= kmax[a] must have lut_len = prod (kmax[a] + 1) entries.
-int check_point_box(const int32_t* kmax, int32_t lut_len) {
+// A point-key box |m_a| <= kmax[a] must have lut_len = prod (kmax[a] + 1)
+// entries, or prod (2 kmax[a] + 1) with signed keys.
+int check_point_box(const int32_t* kmax, int32_t lut_len, bool signed_keys) {
   long long cells = 1;
   for (int a = 0; a < 3; ++a) {
     if (kmax[a] < 0 || kmax[a] > 4096) return fail(SAXS_EINVAL, "kmax[%d] = %d invalid", a, kmax[a]);
-    cells *= kmax[a] + 1;
+    cells *= signed_keys ? 2LL * kmax[a] + 1 : kmax[a] + 1;
   }
   if (cells != lut_len)
-    return fail(SAXS_EINVAL, "lut_len %d != (kmax+1) product %lld", lut_len, cells);
+    return fail(SAXS_EINVAL, "lut_len %d != %lld entries of the kmax box", lut_len, cells);
   return SAXS_OK;
 }
 
-// Replaces the q tables: shell keys (kmax NULL) or point keys.
-int set_tables(saxs_plan* pl, const int32_t* kmax, int32_t lut_len, const int32_t* bin_lut,
-               const float* ff_lut) {
+// Whether a row-major 3x3 inverse cell has off-diagonal terms.
+bool is_triclinic(const float* cell) {
+  return cell && (cell[1] != 0.0f || cell[2] != 0.0f || cell[3] != 0.0f || cell[5] != 0.0f ||
+                  cell[6] != 0.0f || cell[7] != 0.0f);
+}
+
+void set_cell(saxs::Params& p, const float* cell) {
+  p.tric = is_triclinic(cell) ? 1 : 0;
+  for (int i = 0; i < 9; ++i) p.cm[i] = p.tric ? cell[i] : 0.0f;
+}
+
+// Replaces the q tables: shell keys (kmax NULL) or point keys, signed for a
+// triclinic cell.
+int set_tables(saxs_plan* pl, const float* cell, const int32_t* kmax, int32_t lut_len,
+               const int32_t* bin_lut, const float* ff_lut) {
   if (!pl || !bin_lut || !ff_lut) return fail(SAXS_EINVAL, "NULL argument");
   if (lut_len < 1 || lut_len > pl->lut_cap)
     return fail(SAXS_EINVAL, "lut_len %d outside [1, %d] (recreate the plan)", lut_len, pl->lut_cap);
   saxs::Params saved = pl->p;
-  pl->p.point_keys = kmax ? 1 : 0;
+  const bool tric = is_triclinic(cell);
+  pl->p.point_keys = kmax ? (tric ? 2 : 1) : 0;
   for (int a = 0; a < 3; ++a) pl->p.kmax[a] = kmax ? kmax[a] : 0;
+  set_cell(pl->p, cell);
   int rc = derive_ranges(pl, lut_len);
   if (!rc && (pl->p.kz > pl->kz_cap || pl->p.ky > pl->ky_cap || pl->p.nchx > pl->nchx_cap))
     rc = fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
@@ -417,10 +432,14 @@ int check_call(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms
     return fail(SAXS_EINVAL, "n_frames %d outside [1, %d]", n_frames, pl->max_frames);
   if (n_atoms > 0 && (!xyz || !elem)) return fail(SAXS_EINVAL, "xyz/elem pointer is NULL");
   if (!oc_diag) return fail(SAXS_EINVAL, "oc_diag is NULL");
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
     if (!(oc_diag[a] > 0.0f) || !std::isfinite(oc_diag[a]))
       return fail(SAXS_EINVAL, "oc_diag[%d] = %g is not a positive finite inverse length", a,
                   (double)oc_diag[a]);
+    if (pl->p.tric && oc_diag[a] != pl->p.cm[4 * a])
+      return fail(SAXS_EINVAL, "oc_diag[%d] = %g differs from the plan's triclinic cell (%g)", a,
+                  (double)oc_diag[a], (double)pl->p.cm[4 * a]);
+  }
   return SAXS_OK;
 }
 
@@ -583,7 +602,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 
 extern "C" {
 
-int32_t saxs_abi_version(void) { return 3; }
+int32_t saxs_abi_version(void) { return 4; }
 
 const char* saxs_last_error(void) { return g_err.c_str(); }
 
@@ -611,9 +630,10 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   if (d->lut_len < 1 || !d->bin_lut || !d->ff_lut || !d->mod_x || !d->mod_y || !d->mod_z)
     return fail(SAXS_EINVAL, "tables missing");
   if (d->max_atoms < 0 || d->max_frames < 1) return fail(SAXS_EINVAL, "capacity invalid");
-  const bool point_keys = d->kmax[0] > 0 || d->kmax[1] > 0 || d->kmax[2] > 0;
+  const bool tric = is_triclinic(d->cell);
+  const bool point_keys = tric || d->kmax[0] > 0 || d->kmax[1] > 0 || d->kmax[2] > 0;
   if (point_keys) {
-    const int rc = check_point_box(d->kmax, d->lut_len);
+    const int rc = check_point_box(d->kmax, d->lut_len, tric);
     if (rc) return rc;
   }
   if ((long long)d->max_frames * d->num_elements > 65535)
@@ -632,8 +652,9 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   p.E = d->num_elements;
   p.border = d->border;
   p.num_bins = d->num_bins;
-  p.point_keys = point_keys ? 1 : 0;
+  p.point_keys = point_keys ? (tric ? 2 : 1) : 0;
   for (int a = 0; a < 3; ++a) p.kmax[a] = point_keys ? d->kmax[a] : 0;
+  set_cell(p, d->cell);
   p.inv_count = 1.0 / ((double)d->n[0] * d->n[1] * (2 * d->border + 1));
   if (!factor_radices(p.nn[2], p.fz) || !factor_radices(p.nn[1], p.fy) ||
       !factor_radices(p.nn[0], p.fx) || p.nn[0] < 2 || p.nn[1] < 2 || p.nn[2] < 2) {
@@ -691,7 +712,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   {
     int K[3];
     kept_max(p, d->lut_len, K);
-    const long long kb = (long long)std::max({K[0], K[1], K[2]}) + 2;
+    const long long kb = 2LL * std::max({K[0], K[1], K[2]}) + 5;   // signed box, K + 2
     pl->lut_cap = (int)std::min<long long>(std::max<long long>(d->lut_len, kb * kb * kb), 1 << 26);
     pl->lut_cap = std::max(pl->lut_cap, d->lut_len);
     pl->kz_cap = std::min(p.kz + 1, p.nn[2] / 2 + 1);
@@ -759,16 +780,16 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
 
 int saxs_plan_set_tables(saxs_plan* pl, int32_t lut_len, const int32_t* bin_lut, const float* ff_lut) {
   g_err.clear();
-  return set_tables(pl, nullptr, lut_len, bin_lut, ff_lut);
+  return set_tables(pl, nullptr, nullptr, lut_len, bin_lut, ff_lut);
 }
 
-int saxs_plan_set_point_tables(saxs_plan* pl, const int32_t* kmax, int32_t lut_len,
-                               const int32_t* bin_lut, const float* ff_lut) {
+int saxs_plan_set_point_tables(saxs_plan* pl, const float* cell, const int32_t* kmax,
+                               int32_t lut_len, const int32_t* bin_lut, const float* ff_lut) {
   g_err.clear();
   if (!kmax) return fail(SAXS_EINVAL, "NULL argument");
-  const int rc = check_point_box(kmax, lut_len);
+  const int rc = check_point_box(kmax, lut_len, is_triclinic(cell));
   if (rc) return rc;
-  return set_tables(pl, kmax, lut_len, bin_lut, ff_lut);
+  return set_tables(pl, cell, kmax, lut_len, bin_lut, ff_lut);
 }
 
 int saxs_plan_set_padding(saxs_plan* pl, const double* pad) {

@@ -42,8 +42,10 @@ struct Params {
   float oc[3];
   int kcy, nchy, kcx, nchx;
   int lsz, lsy, lsx;     // LDS line strides (complex) of the three passes
-  int point_keys;        // q tables indexed by points (|m_x|,|m_y|,|m_z|), not shells
+  int point_keys;        // q-table keys: 0 shells |m|^2, 1 points |m_a|, 2 signed points
   int kmax[3];           // point-key box |m_a| <= kmax[a] (point_keys only)
+  int tric;              // triclinic cell: u = cm x (row-major 3x3), else u_a = oc_a x_a
+  float cm[9];
   double inv_count;      // 1 / number of border voxels
   FftDesc fz, fy, fx;
 };
@@ -78,11 +80,15 @@ struct Buffers {
 // n = |m|^2 when the three reciprocal spacings are equal, else the point
 // (|m_x|, |m_y|, |m_z|) of the box |m_a| <= kmax[a] (orthorhombic cells, where
 // q^2 = sum_a (m_a s_a)^2 is not a function of |m|^2); lut_len when outside.
+// Triclinic cells (q = sum_a m_a b_a is not even in each m_a) key the signed
+// point, ((m_x + K_x)(2K_y + 1) + m_y + K_y)(2K_z + 1) + m_z + K_z.
 __device__ __forceinline__ int shell_key(int mx, int my, int mz, const Params& p) {
   if (!p.point_keys) return mx * mx + my * my + mz * mz;
   const int ax = abs(mx), ay = abs(my), az = abs(mz);
   if (ax > p.kmax[0] || ay > p.kmax[1] || az > p.kmax[2]) return p.lut_len;
-  return (ax * (p.kmax[1] + 1) + ay) * (p.kmax[2] + 1) + az;
+  if (p.point_keys == 1) return (ax * (p.kmax[1] + 1) + ay) * (p.kmax[2] + 1) + az;
+  return ((mx + p.kmax[0]) * (2 * p.kmax[1] + 1) + my + p.kmax[1]) * (2 * p.kmax[2] + 1) + mz +
+         p.kmax[2];
 }
 
 // Bulk-solvent pad of (frame, element) fe in units of 2^-32 atoms per voxel:
@@ -96,13 +102,34 @@ __device__ __forceinline__ unsigned long long pad_units(const Buffers& b, const 
 
 // ----------------------------------------------------------- atom helpers --
 
-// Cell coordinate r_a = float(n_a (u - rint(u - 0.5))), u = float(oc_a x_a),
-// the float/double mix of saxsDeviceKernels.cu:355-361 (orthorhombic oc).
-__device__ __forceinline__ float cell_coord(float x, float oc, int n) {
-  const float u = __fmul_rn(oc, x);
+// Cell coordinate r_a = float(n_a (u - rint(u - 0.5))) of the fractional
+// coordinate u, the float/double mix of saxsDeviceKernels.cu:355-361.
+__device__ __forceinline__ float cell_coord_u(float u, int n) {
   const double ud = (double)u;
   const double r = (double)n * (ud - rint(ud - 0.5));
   return (float)r;
+}
+
+// Orthorhombic cell: u = float(oc_a x_a).
+__device__ __forceinline__ float cell_coord(float x, float oc, int n) {
+  return cell_coord_u(__fmul_rn(oc, x), n);
+}
+
+// All three cell coordinates of position x: orthorhombic, or triclinic with
+// u_a = (M_a0 x + M_a1 y) + M_a2 z, each product and sum rounded to float
+// (saxsDeviceKernels.cu:355-357 with the true inverse M of the cell).
+__device__ __forceinline__ void cell_coords(const float* x, const Params& p, float r[3]) {
+  if (!p.tric) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r[a] = cell_coord(x[a], p.oc[a], p.n[a]);
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float u = __fadd_rn(__fadd_rn(__fmul_rn(p.cm[3 * a], x[0]), __fmul_rn(p.cm[3 * a + 1], x[1])),
+                              __fmul_rn(p.cm[3 * a + 2], x[2]));
+    r[a] = cell_coord_u(u, p.n[a]);
+  }
 }
 
 // Cardinal B-spline weights th[k] = M_P(w + P - 1 - k) by the recursion of
@@ -197,10 +224,11 @@ __device__ __forceinline__ void atom_key_border(const Params& p, const Buffers& 
                                                 int e, int& key, unsigned long long& border) {
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
   int base[3];
-  float w[3];
+  float w[3], rr[3];
+  cell_coords(x, p, rr);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float r = cell_coord(x[a], p.oc[a], p.n[a]);
+    const float r = rr[a];
     const int m = cell_index(r, p.n[a]);
     w[a] = r - (float)m;
     base[a] = wrap_base(m, P, p.n[a]);
@@ -369,9 +397,11 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   const int2 ks = b.keyslot[(size_t)f * p.N + i];
   if (ks.x < 0) return;
   float4 rec;
-  rec.x = cell_coord(x[0], p.oc[0], p.n[0]);
-  rec.y = cell_coord(x[1], p.oc[1], p.n[1]);
-  rec.z = cell_coord(x[2], p.oc[2], p.n[2]);
+  float rr[3];
+  cell_coords(x, p, rr);
+  rec.x = rr[0];
+  rec.y = rr[1];
+  rec.z = rr[2];
   rec.w = __int_as_float(i);
   const int pos = b.nseg > 0
                       ? b.segoff[((size_t)f * b.nseg + i / b.seglen) * (p.E * p.nty * p.ntx) +

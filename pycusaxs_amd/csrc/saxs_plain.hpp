@@ -34,10 +34,11 @@ __global__ void __launch_bounds__(kWG) k_plain_spread(Params p, const float* xyz
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.N || elem[i] != e) return;
   int m[3];
-  float th[3][P];
+  float th[3][P], rr[3];
+  cell_coords(xyz + (size_t)i * 3, p, rr);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float r = cell_coord(xyz[(size_t)i * 3 + a], p.oc[a], p.n[a]);
+    const float r = rr[a];
     m[a] = cell_index(r, p.n[a]);
     bspline<P>(r - (float)m[a], th[a]);
   }
@@ -190,6 +191,7 @@ __global__ void k_plain_modulus(Params p, Buffers b, float2* acc) {
 
 // calculate_histogram: each in-range half-spectrum point adds I[i,j,k] +
 // I[-i,-j,k] (count 2), the origin I once (count 1); per-block bins in LDS.
+// Triclinic cells add 2 I[i,j,k] at the bin of (i,j,k) itself.
 __global__ void __launch_bounds__(kWG) k_plain_hist(Params p, Buffers b, const float2* I, double* H, double* C) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nb = p.num_bins;
@@ -215,6 +217,11 @@ __global__ void __launch_bounds__(kWG) k_plain_hist(Params p, Buffers b, const f
     if (x == 0 && y == 0 && k == 0) {
       atomicAdd(&Hs[bin], (double)I[i].x);
       atomicAdd(&Cs[bin], 1.0);
+    } else if (p.point_keys == 2) {
+      // triclinic: (-i,-j,k) is not at the same |q|; the point stands for
+      // itself and its Hermitian mirror -m
+      atomicAdd(&Hs[bin], 2.0 * (double)I[i].x);
+      atomicAdd(&Cs[bin], 2.0);
     } else {
       const int xb = x == 0 ? 0 : p.nn[0] - x, yb = y == 0 ? 0 : p.nn[1] - y;
       const size_t j = ((size_t)xb * p.nn[1] + yb) * h + k;

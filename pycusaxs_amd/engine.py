@@ -17,8 +17,9 @@ import torch
 
 from . import _lib
 from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_points,
-                       form_factor_table, hist_params, inverse_box_diag, point_bins, point_kmax,
-                       shell_bins)
+                       form_factor_table, form_factor_vectors, hist_params, inverse_box_matrix,
+                       is_triclinic, point_bins, point_kmax, reciprocal_basis, shell_bins,
+                       triclinic_bins, triclinic_kmax)
 
 
 @dataclass
@@ -68,13 +69,14 @@ class SaxsEngine:
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.max_atoms = int(max_atoms)
         self.max_frames = int(max_frames)
-        self.oc = inverse_box_diag(box)
+        self.cell = inverse_box_matrix(box)
+        self.oc = np.ascontiguousarray(np.diag(self.cell), dtype=np.float32)
         self.hist: Histogram = hist_params(spec.n, spec.nn, self.oc, spec.sigma, spec.dq_req,
                                            spec.qcut_req, spec.dq_policy)
         self.border = border_planes(spec.n[2])
         self.moduli = [bspline_moduli(spec.nn[a], spec.order) for a in range(3)]
-        bins, ff, kmax = self._q_tables(self.hist)
-        self._box_key = tuple(float(v) for v in self.oc)
+        bins, ff, kmax = self._q_tables(self.hist, self.cell)
+        self._box_key = tuple(float(v) for v in self.cell.ravel())
         mods = [np.ascontiguousarray(m, dtype=np.float32) for m in self.moduli]
         desc = _lib.SaxsDesc()
         desc.n[:] = list(spec.n)
@@ -94,6 +96,7 @@ class SaxsEngine:
         desc.tile_x, desc.tile_y = int(tile[0]), int(tile[1])
         desc.flags = int(flags)
         desc.kmax[:] = list(kmax) if kmax is not None else [0, 0, 0]
+        desc.cell[:] = [float(v) for v in self.cell.ravel()] if is_triclinic(self.cell) else [0.0] * 9
         plan = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.saxs_plan_create(C.byref(desc), C.byref(plan)), "saxs_plan_create")
@@ -109,12 +112,18 @@ class SaxsEngine:
             self.set_padding(pad_given)
 
     # -- tables ---------------------------------------------------------
-    def _q_tables(self, hist: Histogram):
-        """Bin and form-factor tables: per shell n = |m|^2 when the three
-        reciprocal spacings are equal (kmax None), else per point of the box
-        |m_a| <= kmax[a] (orthorhombic cells with unequal spacings)."""
+    def _q_tables(self, hist: Histogram, cell):
+        """Bin and form-factor tables: per shell n = |m|^2 when the cell is
+        orthorhombic with equal reciprocal spacings (kmax None), per point
+        (|m_x|, |m_y|, |m_z|) of the box |m_a| <= kmax[a] for other
+        orthorhombic cells, per signed point for triclinic cells."""
         s = hist.spacing
-        if s[0] == s[1] == s[2]:
+        if is_triclinic(cell):
+            basis = reciprocal_basis(cell, self.spec.n, self.spec.nn)
+            kmax = triclinic_kmax(basis, hist.qcut, self.spec.nn)
+            bins = triclinic_bins(basis, hist.dq, hist.qcut, hist.num_bins, kmax)
+            ff = form_factor_vectors(self.spec.symbols, basis, kmax)
+        elif s[0] == s[1] == s[2]:
             bins = shell_bins(s[0], hist.dq, hist.qcut, hist.num_bins)
             ff = form_factor_table(self.spec.symbols, s[0], len(bins))
             kmax = None
@@ -129,15 +138,16 @@ class SaxsEngine:
     def set_box(self, box):
         """Rebuilds the shell tables for a new box (NPT).  dq, qcut and
         num_bins stay those of the first frame, as in the reference."""
-        oc = inverse_box_diag(box)
-        key = tuple(float(v) for v in oc)
+        cell = inverse_box_matrix(box)
+        oc = np.ascontiguousarray(np.diag(cell), dtype=np.float32)
+        key = tuple(float(v) for v in cell.ravel())
         if key == self._box_key:
             return oc
         h = hist_params(self.spec.n, self.spec.nn, oc, self.spec.sigma, self.spec.dq_req,
                         self.spec.qcut_req, self.spec.dq_policy)
         h = Histogram(self.hist.dq, self.hist.qcut, self.hist.num_bins, h.spacing,
                       self.hist.dq_reset, self.hist.qcut_reset)
-        bins, ff, kmax = self._q_tables(h)
+        bins, ff, kmax = self._q_tables(h, cell)
         torch.cuda.current_stream().synchronize()
         if kmax is None:
             _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
@@ -146,13 +156,17 @@ class SaxsEngine:
                        "saxs_plan_set_tables")
         else:
             km = np.array(kmax, dtype=np.int32)
-            _lib.check(self.lib.saxs_plan_set_point_tables(self._plan, km.ctypes.data_as(C.c_void_p),
+            cm = np.ascontiguousarray(cell, dtype=np.float32)
+            cptr = cm.ctypes.data_as(C.c_void_p) if is_triclinic(cell) else None
+            _lib.check(self.lib.saxs_plan_set_point_tables(self._plan, cptr,
+                                                           km.ctypes.data_as(C.c_void_p),
                                                            len(bins),
                                                            bins.ctypes.data_as(C.c_void_p),
                                                            ff.ctypes.data_as(C.c_void_p)),
                        "saxs_plan_set_point_tables")
         self._box_key = key
         self.oc = oc
+        self.cell = cell
         return oc
 
     def set_padding(self, pad_given: Optional[Mapping[str, float]]):

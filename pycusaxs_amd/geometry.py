@@ -6,9 +6,11 @@ the engine's C ABI as plain arrays.
   fft_size             saxsKernel.cu:377-469 (findClosestProduct)
   border_planes        saxsKernel.h:107-110
   inverse_box_diag     Cell.cpp:111-126
+  inverse_box_matrix   Cell.cpp:118-125 applied to the transposed (MDAnalysis) box
   hist_params          saxsKernel.cu:489-526 and :247
   shell_bins           exact bin of every shell n = |m|^2
   point_bins           exact bin of every point of an orthorhombic grid
+  triclinic_bins       exact bin of every signed point of a triclinic grid
   bspline_moduli       BSpmod.cpp:45-211
   form_factor_table    opsfact.cu:29-35 on the shell q values
   explicit_solvent_pads  AtomCounter.cpp:56-103 (padding::given, Options.h:7-11)
@@ -75,6 +77,121 @@ def inverse_box_diag(box) -> np.ndarray:
     if np.any(~np.isfinite(d)) or np.any(d <= 0):
         raise ValueError(f"box lengths must be positive, got {d.tolist()}")
     return np.array([F32(1.0 / float(x)) for x in d], dtype=np.float32)
+
+
+def inverse_box_matrix(box) -> np.ndarray:
+    """Fractional-coordinate matrix M (u = M x, float32, row-major) of a cell
+    given MDAnalysis-style: rows a, b, c with a along x and b in the xy plane
+    (lower-triangular).  Cartesian x = B^T u; B^T is upper-triangular and
+    Cell.cpp:118-125's float32 formula for an upper-triangular inverse gives
+    M = (B^T)^-1 exactly as the reference would for that orientation (the
+    reference applies it to B and so drops the off-diagonal terms).  A
+    diagonal box gives diag(float32(1/L_a)), as inverse_box_diag."""
+    b = np.asarray(box, dtype=np.float32).reshape(3, 3)
+    if b[0, 1] != 0 or b[0, 2] != 0 or b[1, 2] != 0:
+        raise ValueError("box must be lower-triangular: rows a = (ax, 0, 0), b = (bx, by, 0), "
+                         "c = (cx, cy, cz)")
+    d = np.diag(b)
+    if np.any(~np.isfinite(b)) or np.any(d <= 0):
+        raise ValueError(f"box diagonal must be positive and finite, got {d.tolist()}")
+    t = b.T                                          # upper-triangular, columns a, b, c
+    M = np.zeros((3, 3), dtype=np.float32)
+    M[0, 0] = F32(1.0 / float(t[0, 0]))
+    M[1, 1] = F32(1.0 / float(t[1, 1]))
+    M[2, 2] = F32(1.0 / float(t[2, 2]))
+    M[0, 1] = F32(-t[0, 1] / F32(t[0, 0] * t[1, 1]))
+    M[1, 2] = F32(-t[1, 2] / F32(t[1, 1] * t[2, 2]))
+    M[0, 2] = F32(F32(F32(t[0, 1] * t[1, 2]) - F32(t[0, 2] * t[1, 1]))
+                  / F32(F32(t[0, 0] * t[1, 1]) * t[2, 2]))
+    return M
+
+
+def is_triclinic(M) -> bool:
+    M = np.asarray(M)
+    return bool(np.any(M != np.diag(np.diag(M))))
+
+
+def reciprocal_basis(M, n: Sequence[int], nn: Sequence[int]) -> np.ndarray:
+    """Rows b_a (float32) with q = sum_a m_a b_a for FFT index m:
+    b_a = 2 pi M_a. n_a / nn_a, each entry in the float32 expression of the
+    reciprocal spacing (so a diagonal M gives diag(spacing))."""
+    tp = F32(F32(2.0) * F32(math.pi))
+    out = np.zeros((3, 3), dtype=np.float32)
+    for a in range(3):
+        ratio = F32(F32(nn[a]) / F32(n[a]))
+        for j in range(3):
+            out[a, j] = F32(F32(tp * F32(M[a, j])) / ratio)
+    return out
+
+
+def _basis_integers(basis):
+    """Integer rows B and a denominator D with basis = B / D exactly."""
+    ratios = [[float(v).as_integer_ratio() for v in row] for row in basis]
+    D = 1
+    for row in ratios:
+        for _, den in row:
+            D = D * den // math.gcd(D, den)
+    return [[num * (D // den) for num, den in row] for row in ratios], D
+
+
+def triclinic_kmax(basis, qcut: float, nn: Sequence[int]) -> Tuple[int, int, int]:
+    """Largest |m_a| of the points with |sum m_a b_a| <= qcut (the ellipsoid's
+    extent qcut sqrt((G^-1)_aa), G = b b^T), clipped to nn_a/2; the exact
+    table marks the box corners outside qcut."""
+    b = np.asarray(basis, dtype=np.float64)
+    Ginv = np.linalg.inv(b @ b.T)
+    out = []
+    for a in range(3):
+        k = int(math.floor(float(qcut) * math.sqrt(max(Ginv[a, a], 0.0)) * (1 + 1e-9))) + 1
+        out.append(min(k, int(nn[a]) // 2))
+    return tuple(out)
+
+
+def triclinic_bins(basis, dq: float, qcut: float, num_bins: int,
+                   kmax: Sequence[int]) -> np.ndarray:
+    """bin of every signed point m of the box |m_a| <= kmax[a] (layout
+    ((m_x+K_x)(2K_y+1) + m_y+K_y)(2K_z+1) + m_z+K_z): q = sum_a m_a b_a with
+    the float32 rows as exact integers over one denominator,
+    bin = max{b : b^2 dq^2 <= |q|^2}, -1 outside |q| <= qcut or from num_bins."""
+    B, D = _basis_integers(basis)
+    dn, dd = float(dq).as_integer_ratio()
+    qn, qd = float(qcut).as_integer_ratio()
+    kx, ky, kz = (int(v) for v in kmax)
+    out = np.empty((2 * kx + 1) * (2 * ky + 1) * (2 * kz + 1), dtype=np.int32)
+    lim = qn * qn * D * D                          # |q D|^2 qd^2 <= qn^2 D^2
+    den = dn * dn * D * D
+    i = 0
+    for mx in range(-kx, kx + 1):
+        for my in range(-ky, ky + 1):
+            base = [mx * B[0][j] + my * B[1][j] for j in range(3)]
+            for mz in range(-kz, kz + 1):
+                s2 = 0
+                for j in range(3):
+                    v = base[j] + mz * B[2][j]
+                    s2 += v * v                    # |q|^2 D^2
+                if dn <= 0 or s2 * qd * qd > lim:
+                    out[i] = -1
+                else:
+                    bb = math.isqrt((s2 * dd * dd) // den)
+                    out[i] = bb if bb < num_bins else -1
+                i += 1
+    return out
+
+
+def form_factor_vectors(symbols: Sequence[str], basis, kmax: Sequence[int]) -> np.ndarray:
+    """f_e(|q|), q = sum_a m_a b_a, over the signed box in triclinic_bins order."""
+    b = np.asarray(basis, dtype=np.float64)
+    ax = [np.arange(-int(k), int(k) + 1, dtype=np.float64) for k in kmax]
+    qv = (ax[0][:, None, None, None] * b[0] + ax[1][None, :, None, None] * b[1]
+          + ax[2][None, None, :, None] * b[2])
+    q = np.sqrt((qv ** 2).sum(axis=3)).reshape(-1)
+    s2 = (q / (4.0 * np.pi)) ** 2
+    out = np.empty((len(symbols), len(q)))
+    for e, sym in enumerate(symbols):
+        a1, a2, a3, a4, b1, b2, b3, b4, c = coefficients(sym)
+        out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
+                  + a4 * np.exp(-b4 * s2) + c)
+    return out
 
 
 @dataclass(frozen=True)

@@ -95,7 +95,12 @@ def test_unknown_element_raises():
         formfactors.coefficients("Xx")
 
 
-def test_triclinic_box_rejected():
+def test_non_lower_triangular_box_rejected():
+    with pytest.raises(ValueError):
+        G.inverse_box_matrix([[30, 5, 0], [0, 30, 0], [0, 0, 30]])
+
+
+def test_orthorhombic_helper_rejects_triclinic():
     with pytest.raises(ValueError):
         G.inverse_box_diag([[30, 0, 0], [5, 30, 0], [0, 0, 30]])
 
@@ -214,3 +219,54 @@ def test_point_bins_match_oracle(Ls, n, dq, qcut, policy):
                 assert ff[1, i] == pytest.approx(O.form_factor("O", q), rel=1e-12)
                 i += 1
     assert i == len(bins) and (bins >= 0).sum() > 50
+
+
+@needs_ref
+@pytest.mark.parametrize("B", [
+    [[37.0, 0, 0], [0, 41.5, 0], [0, 0, 100.0]],
+    [[30.0, 0, 0], [7.5, 28.0, 0], [-5.0, 6.25, 33.0]],
+    [[61.3, 0, 0], [-20.4, 57.7, 0], [-20.4, -28.9, 50.0]],      # truncated-octahedron-like
+])
+def test_triclinic_inverse_matches_reference(B):
+    """M = Cell.cpp's upper-triangular inverse applied to the transposed box,
+    bit for bit in float32 (the reference helper is given B^T)."""
+    B = np.asarray(B, dtype=np.float32)
+    line = ref("cell", *[float(v) for v in B.T.ravel()])[0]
+    M_ref = np.array([float(v) for v in line.split()], dtype=np.float64).reshape(3, 3)
+    assert np.array_equal(M_ref.astype(np.float32), G.inverse_box_matrix(B))
+    assert np.array_equal(M_ref.astype(np.float32), O.inverse_box_matrix(B))
+
+
+def test_triclinic_bins_match_oracle():
+    """Signed point tables (engine host code, integers) against the oracle's
+    exact rational rule, and kmax covers every point inside qcut."""
+    B = np.array([[30.0, 0, 0], [7.5, 28.0, 0], [-5.0, 6.25, 33.0]])
+    n, nn = (24, 24, 24), (48, 48, 48)
+    M = G.inverse_box_matrix(B)
+    assert G.is_triclinic(M)
+    h = G.hist_params(n, nn, np.diag(M), 2.0, 0.0, 0.7, "reference")
+    basis = G.reciprocal_basis(M, n, nn)
+    assert np.array_equal(basis, O.reciprocal_basis(O.inverse_box_matrix(B), n, nn))
+    km = G.triclinic_kmax(basis, h.qcut, nn)
+    bins = G.triclinic_bins(basis, h.dq, h.qcut, h.num_bins, km)
+    ff = G.form_factor_vectors(["S"], basis, km)
+    i = 0
+    for mx in range(-km[0], km[0] + 1):
+        for my in range(-km[1], km[1] + 1):
+            for mz in range(-km[2], km[2] + 1):
+                assert bins[i] == O.bin_of_vec((mx, my, mz), basis, h.dq, h.qcut, h.num_bins)
+                q = np.linalg.norm(mx * basis[0].astype(float) + my * basis[1].astype(float)
+                                   + mz * basis[2].astype(float))
+                assert ff[0, i] == pytest.approx(O.form_factor("S", q), rel=1e-12)
+                i += 1
+    assert (bins >= 0).sum() > 100
+    # nothing inside qcut lies outside the box
+    for a in range(3):
+        for extra in (km[a] + 1, -km[a] - 1):
+            m = [0, 0, 0]
+            m[a] = extra
+            for u in range(-km[(a + 1) % 3], km[(a + 1) % 3] + 1):
+                m[(a + 1) % 3] = u
+                for v in range(-km[(a + 2) % 3], km[(a + 2) % 3] + 1):
+                    m[(a + 2) % 3] = v
+                    assert O.bin_of_vec(tuple(m), basis, h.dq, h.qcut, h.num_bins) == -1

@@ -364,3 +364,92 @@ def test_orthorhombic_counts_match_reference_loop():
     H, Cg = O.frame_histogram(plan, pos * np.array(Ls) / 30.0, ids, symbols, np.diag(Ls))
     assert list(Cg.astype(int)) == C
     assert sum(C) > 500
+
+
+TRIC_BOX = np.array([[16.0, 0.0, 0.0], [8.0, 16.0, 0.0], [4.0, 8.0, 32.0]])
+
+
+def test_triclinic_inverse_is_exact_inverse():
+    """u = M x inverts x = B^T u: M B^T = I (dyadic box: exactly)."""
+    M = O.inverse_box_matrix(TRIC_BOX)
+    assert np.array_equal(M.astype(np.float64) @ TRIC_BOX.T, np.eye(3))
+    rng = np.random.default_rng(1)
+    B = np.tril(rng.uniform(5, 50, size=(3, 3)))
+    np.fill_diagonal(B, rng.uniform(30, 60, size=3))
+    M = O.inverse_box_matrix(B).astype(np.float64)
+    assert np.allclose(M @ B.astype(np.float32).T, np.eye(3), atol=1e-6)
+    with pytest.raises(ValueError):
+        O.inverse_box_matrix(B.T)
+
+
+def test_on_node_atoms_direct_structure_factor_triclinic():
+    """Triclinic cell a = (16,0,0), b = (8,16,0), c = (4,8,32) A on an
+    8 x 8 x 16 grid: atoms on grid nodes x = B^T g/n (exact in float32) must
+    give |sum_j f_j(|q|) e^{-2 pi i sum_a m_a g_a / nn_a}|^2 prod lambda^2 with
+    q = sum_a m_a b_a, b_a = 2 pi (M^T)_a n_a / nn_a, binned by |q| of each
+    point itself."""
+    n3 = (8, 8, 16)
+    plan = O.make_plan(n3, TRIC_BOX, 4, 2.0, None, 0.0, 100.0)
+    assert plan.triclinic and plan.nn == (16, 16, 32)
+    d = plan.border
+    rng = np.random.default_rng(8)
+    nodes = np.stack([rng.integers(4, n3[0], 5), rng.integers(4, n3[1], 5),
+                      rng.integers(d + 5, n3[2] - d, 5)], axis=1)
+    pos = ((nodes / np.array(n3)) @ TRIC_BOX).astype(np.float32)
+    syms = ["C", "O", "N", "O", "C"]
+    symbols, ids = O.element_ids(syms)
+    I, inr, bins = O.frame_intensity(plan, pos, ids, symbols, TRIC_BOX)
+    M = O.inverse_box_matrix(TRIC_BOX).astype(np.float64)
+    mx, my, mz = O.half_spectrum_m(plan.nn)
+    ii, jj, kk = np.nonzero(inr)
+    assert len(ii) > 1000
+    for t in range(0, len(ii), max(1, len(ii) // 80)):
+        i, j, k = ii[t], jj[t], kk[t]
+        m = np.array([mx[i], my[j], mz[k]], dtype=np.float64)
+        qv = 2 * np.pi * (M.T @ (m * np.array(n3) / np.array(plan.nn)))
+        q = float(np.linalg.norm(qv))
+        A = 0j
+        for g, sym in zip(nodes, syms):
+            A += O.form_factor(sym, q) * np.exp(-2j * np.pi * sum(m[a] * g[a] / plan.nn[a]
+                                                                   for a in range(3)))
+        lam2 = 1.0
+        for ax in range(3):
+            x = math.pi * m[ax] / plan.nn[ax]
+            if m[ax] != 0:
+                gsum = lambda r: 1 + sum((x / (x + math.pi * jj_)) ** r + (x / (x - math.pi * jj_)) ** r
+                                         for jj_ in range(1, 51))
+                lam2 *= (gsum(4) / gsum(8)) ** 2
+        expect = abs(A) ** 2 * lam2
+        assert abs(I[i, j, k] - expect) <= 1e-6 * max(1.0, expect), (m, I[i, j, k], expect)
+        # and the bin is that of |q| itself
+        b = bins[i, j, k]
+        assert b * float(plan.hp.dq) <= q * (1 + 1e-6) and q < (b + 1) * float(plan.hp.dq) * (1 + 1e-6)
+
+
+def test_triclinic_counts_match_enumeration():
+    """Counts on a triclinic grid: every half-spectrum point with |q| <= qcut
+    counts 2 (the origin 1) in the bin of its own |q|, exact rationals."""
+    B = np.array([[30.0, 0, 0], [7.5, 28.0, 0], [-5.0, 6.25, 33.0]])
+    n3 = (24, 24, 24)
+    plan = O.make_plan(n3, B, 4, 2.0, None, 0.0, 0.8)
+    bas = [[Fraction(float(v)) for v in row] for row in plan.basis]
+    D, Q = Fraction(float(plan.hp.dq)), Fraction(float(plan.hp.qcut))
+    C = [0] * plan.hp.num_bins
+    mx, my, mz = O.half_spectrum_m(plan.nn)
+    R = 12
+    for i in mx[np.abs(mx) <= R]:
+        for j in my[np.abs(my) <= R]:
+            for k in mz[np.abs(mz) <= R]:
+                qv = [i * bas[0][c] + j * bas[1][c] + k * bas[2][c] for c in range(3)]
+                q2 = sum(v * v for v in qv)
+                if q2 > Q * Q:
+                    continue
+                b = 0
+                while (b + 1) ** 2 * D * D <= q2:
+                    b += 1
+                if b < plan.hp.num_bins:
+                    C[b] += 1 if (i, j, k) == (0, 0, 0) else 2
+    pos, syms = O.synthetic_system(100, 1.0, seed=4)
+    symbols, ids = O.element_ids(syms)
+    _, Cg = O.frame_histogram(plan, (pos @ B).astype(np.float32), ids, symbols, B)
+    assert list(Cg.astype(int)) == C and sum(C) > 300
