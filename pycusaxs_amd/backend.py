@@ -10,9 +10,10 @@ Mirrors, in order:
       iter_frames_stream(begin, end+1, stride) (RunSaxs.cu:137-138)
       per frame: "--> Frame: ..." line, forward (saxsKernel.cu:110-113)
       saxs.dat rows "%10.5f%12.5e" (RunSaxs.cu:217-240), timing banner
-Frames are batched into one engine launch chain per batch; the Python side
-reads frame k+1 while the GPU works on frame k (launches are asynchronous),
-which is the reference's double buffer (RunSaxs.cu:141-170).
+Frames are batched into one engine launch chain per batch and staged through
+two pinned buffers and a copy stream (FrameStager): the host reads and
+uploads batch k+1 while the GPU computes batch k, the reference's double
+buffer (RunSaxs.cu:141-170).
 
 Deliberate differences (the reference's behaviour there is a silent failure
 or undefined): a malformed frame dict, an exception raised by the frame
@@ -230,40 +231,30 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     log(banner(n, nn, order, sigma, h.dq, h.qcut, water_model))
     dev = eng.device
     ids_t = torch.from_numpy(ids).to(dev)
-    idx_t = torch.from_numpy(idx).to(dev)
+    stager = FrameStager(n_atoms, batch, idx, dev)
     gpu_ms = 0.0
     n_done = 0
-    pending: List[np.ndarray] = []
-    pending_box = None
 
     def flush():
-        nonlocal gpu_ms, pending, pending_box
-        if not pending:
-            return
+        nonlocal gpu_ms
         t0 = time.perf_counter()
-        host = torch.from_numpy(np.stack(pending)).pin_memory()
-        full = host.to(dev, non_blocking=True)
-        xyz = full.index_select(1, idx_t) if n_atoms else full[:, :0]
         try:
-            eng.forward(xyz, ids_t, box=pending_box)
+            stager.launch(eng, ids_t)
         except (ValueError, RuntimeError) as exc:
             # RunSaxs.cu:195-199 logs a failing frame and goes on
             print(f"Error processing frames: {exc}", file=sys.stderr)
         gpu_ms += (time.perf_counter() - t0) * 1e3
-        pending = []
-        pending_box = None
 
     frames_iter = topo.iter_frames_stream(begin, end + 1, stride)
     for fr in frames_iter:
         frame, t, pos, box = _frame_fields(fr)
         log(f"--> Frame: {frame:<7}  Time Step: {t:.2f} ps")
-        if pending and (pending_box is None or not np.array_equal(box, pending_box)
-                        or len(pending) >= batch):
+        if stager.count and (not np.array_equal(box, stager.box) or stager.count >= batch):
             flush()
-        pending.append(pos)
-        pending_box = box
+        stager.add(pos, box)
         n_done += 1
-    flush()
+    if stager.count:
+        flush()
     torch.cuda.synchronize(dev)
     rows = eng.rows()
     with open(out_path, "w") as fh:
@@ -276,6 +267,80 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     log(timing_banner(cuda_ms, total_ms - cuda_ms, total_ms))
     eng.close()
     return rows
+
+
+class FrameStager:
+    """Double-buffered frame upload (the reference's double buffer,
+    RunSaxs.cu:141-170, without its per-atom host copies): each frame is
+    copied whole into one of two pinned buffers [batch, atoms, 3]; a batch is
+    uploaded to its device buffer on a separate copy stream, and the compute
+    stream waits for that copy only, gathers the atoms into element order
+    (one index_select) and runs the engine.  So batch k+1 is read and uploaded
+    while batch k computes.  Events guard the reuse of each host buffer (its
+    copy has finished) and each device buffer (the kernels reading it have)."""
+
+    def __init__(self, n_atoms: int, batch: int, idx: np.ndarray, device):
+        import torch
+        self.batch = batch
+        self.n_atoms = n_atoms
+        self.idx = np.ascontiguousarray(idx, dtype=np.int64)
+        self.idx_t = torch.from_numpy(self.idx).to(device)
+        self.device = device
+        self.n_frame_atoms = None
+        self.host = self.host_np = self.dev = None
+        self.sel = torch.empty((batch, n_atoms, 3), dtype=torch.float32, device=device)
+        self.copy_stream = torch.cuda.Stream(device=device)
+        self.copied = [None, None]
+        self.consumed = [None, None]
+        self.slot = 0
+        self.count = 0
+        self.box = None
+
+    def _allocate(self, n_frame_atoms: int):
+        import torch
+        if len(self.idx) and int(self.idx.max()) >= n_frame_atoms:
+            raise ValueError(f"frame has {n_frame_atoms} atoms, the topology indexes atom "
+                             f"{int(self.idx.max())}")
+        shape = (self.batch, n_frame_atoms, 3)
+        self.host = [torch.empty(shape, dtype=torch.float32, pin_memory=True) for _ in range(2)]
+        self.host_np = [h.numpy() for h in self.host]
+        self.dev = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        self.n_frame_atoms = n_frame_atoms
+
+    def add(self, pos: np.ndarray, box: np.ndarray):
+        if self.n_frame_atoms is None:
+            self._allocate(len(pos))
+        elif len(pos) != self.n_frame_atoms:
+            raise ValueError(f"frame has {len(pos)} atoms, earlier frames {self.n_frame_atoms}")
+        if self.count == 0 and self.copied[self.slot] is not None:
+            self.copied[self.slot].synchronize()          # host buffer is free again
+        np.copyto(self.host_np[self.slot][self.count], pos)
+        self.count += 1
+        self.box = box
+
+    def launch(self, eng, ids_t):
+        import torch
+        k, s = self.count, self.slot
+        self.count = 0
+        self.slot ^= 1
+        compute = torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(self.copy_stream):
+            if self.consumed[s] is not None:
+                self.copy_stream.wait_event(self.consumed[s])
+            self.dev[s][:k].copy_(self.host[s][:k], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+        self.copied[s] = ev
+        compute.wait_event(ev)
+        try:
+            xyz = self.sel[:k]
+            if self.n_atoms:
+                torch.index_select(self.dev[s][:k], 1, self.idx_t, out=xyz)
+            eng.forward(xyz, ids_t, box=self.box)
+        finally:
+            done = torch.cuda.Event()
+            done.record(compute)
+            self.consumed[s] = done
 
 
 def banner(n, nn, order, sigma, dq, qcut, water_model) -> str:
