@@ -727,8 +727,10 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
 // the drho~ columns 2l and 2l + 1 (line stride = column stride, the threads
 // of a line share one wavefront), z halo repeated after nz.  Then the
 // gather (gather_cols) into the atoms' partial-gradient slots.
-template <int P, int N, bool HALF>
+// EXACT: n_z == N/2, every stage-1 output z = j + 16 rr, rr < R2/2, is in the column.
+template <int P, int N, bool HALF, bool EXACT = false>
 __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffers g) {
+  static_assert(!EXACT || HALF, "EXACT implies HALF");
   constexpr int R2 = N / 16;
   constexpr int ROUT = HALF ? R2 / 2 : R2;             // stage-1 outputs that can be z < nz
   extern __shared__ __align__(16) unsigned char smem[];
@@ -861,7 +863,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
 #pragma unroll
         for (int rr = 0; rr < ROUT; ++rr) {
           const int z = j1 + 16 * rr;
-          if (z < nz) {
+          if (EXACT || z < nz) {
             const float a = t[rr].x, c = ok1 ? -t[rr].y : 0.0f;
             col0[z] = a;
             col1[z] = c;

@@ -515,10 +515,11 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     return fail(SAXS_EUNSUP, "column tile %dx%d exceeds the z transform's line capacity", p.tx, p.ty);
   if (pruned_ok) {
     dim3 ga(p.ntx * p.nty, n_frames);
-    const bool zhalf = 2 * p.n[2] <= Nz;
+    const bool zhalf = 2 * p.n[2] <= Nz, zexact = 2 * p.n[2] == Nz;
 #define SAXS_ZP(PP, NN)                                                                         \
   do {                                                                                          \
-    if (zhalf) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_p, st, p, b); \
+    if (zexact) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, true, true>), ga, dim3(saxs::kWG), lds_p, st, p, b); \
+    else if (zhalf) hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_p, st, p, b); \
     else hipLaunchKernelGGL((saxs::k_spread_zp<PP, NN, false>), ga, dim3(saxs::kWG), lds_p, st, p, b);    \
   } while (0)
     if (p.order == 4) {
@@ -953,10 +954,11 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   if (adj_pruned) {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 ga(p.ntx * p.nty, n_frames);
-    const bool zhalf = 2 * p.n[2] <= Nz;
+    const bool zhalf = 2 * p.n[2] <= Nz, zexact = 2 * p.n[2] == Nz;
 #define SAXS_AZP(PP, NN)                                                                       \
   do {                                                                                         \
-    if (zhalf) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g); \
+    if (zexact) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, true, true>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g); \
+    else if (zhalf) hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, true>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g); \
     else hipLaunchKernelGGL((saxs::k_adj_zp<PP, NN, false>), ga, dim3(saxs::kWG), lds_ap, st, p, b, g);    \
   } while (0)
     if (p.order == 4) {

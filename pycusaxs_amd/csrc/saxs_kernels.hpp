@@ -91,6 +91,18 @@ __device__ __forceinline__ int shell_key(int mx, int my, int mz, const Params& p
          p.kmax[2];
 }
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with its
+// own L2: linear block id `lin` runs on XCD lin % 8.  This bijection maps the
+// blocks of XCD c onto the contiguous range of logical ids
+// [c q + min(c, r), ...) (q = total / 8, r = total % 8), so logical
+// neighbours share an L2.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned lin, unsigned total) {
+  const unsigned q = total / kXcds, r = total % kXcds;
+  const unsigned c = lin % kXcds, i = lin / kXcds;
+  return c * q + min(c, r) + i;
+}
+
 // Bulk-solvent pad of (frame, element) fe in units of 2^-32 atoms per voxel:
 // the border-slab mean (padding::avg, saxsDeviceKernels.cu:446-465) or the
 // caller's constant (padding::given, Options.h:7-11).
@@ -814,8 +826,11 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   return A + B + tw + seg;
 }
 
-template <int P, int N, bool HALF>
+// EXACT: n_z == N/2 (every sigma-2 power-of-two grid), so every stage-0
+// input z = j + R2 r, r < 8, lies in the column and needs no bounds test.
+template <int P, int N, bool HALF, bool EXACT = false>
 __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
+  static_assert(!EXACT || HALF, "EXACT implies HALF");
   constexpr int R2 = N / 16;
   constexpr int RIN = HALF ? 8 : 16;                   // stage-0 inputs that can be nonzero
   extern __shared__ __align__(16) unsigned char smem[];
@@ -924,15 +939,16 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 #pragma unroll
       for (int r = 0; r < RIN; ++r) {
         const int z = j + R2 * r;
-        a0[r] = pad0;
-        a1[r] = pad1;
-        if (z < nz) {
+        if (EXACT || z < nz) {
           a0[r] = col0[z];
           a1[r] = col1[z];
           if (r == 0 && z < P - 1) {                 // fold the z halo back
             a0[r] += col0[nz + z];
             a1[r] += col1[nz + z];
           }
+        } else {
+          a0[r] = pad0;
+          a1[r] = pad1;
         }
         big |= (unsigned)((a0[r] | a1[r]) >> 31);
       }
@@ -1073,10 +1089,13 @@ __global__ void __launch_bounds__(kWG) k_yfft_p(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float2* tw = ybuf + (size_t)LW * LY;                 // W_N^{j s} stored [j][s]
-  const size_t fe = blockIdx.y;
   const int nx = p.n[0], ny = p.n[1], kz = p.kz;
   const int nlines = nx * kz;
-  const int g0 = blockIdx.x * LW;
+  // Neighbouring groups share the 128-byte lines where the kz-long rows of
+  // zbuf straddle group boundaries; give each XCD a contiguous range.
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
+  const int g0 = (int)(grp % gridDim.x) * LW;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[(i >> 4) * (i & 15)];
   const float2* src = b.zbuf + fe * (size_t)nx * ny * kz;
   {
