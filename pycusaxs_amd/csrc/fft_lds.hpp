@@ -252,9 +252,20 @@ __device__ __forceinline__ void dft16(float2* v) {
   float2 u[4][4];
 #pragma unroll
   for (int n1 = 0; n1 < 4; ++n1) {
-    float2 t[4] = {v[n1], v[n1 + 4], HALF ? make_float2(0.0f, 0.0f) : v[n1 + 8],
-                   HALF ? make_float2(0.0f, 0.0f) : v[n1 + 12]};
-    Radix<4>::dft(t);
+    float2 t[4];
+    if constexpr (HALF) {                   // DFT4 of (a, b, 0, 0), no adds of zeros
+      const float2 a = v[n1], b = v[n1 + 4], mib = cmul_mi(b);
+      t[0] = cadd(a, b);
+      t[1] = cadd(a, mib);
+      t[2] = csub(a, b);
+      t[3] = csub(a, mib);
+    } else {
+      t[0] = v[n1];
+      t[1] = v[n1 + 4];
+      t[2] = v[n1 + 8];
+      t[3] = v[n1 + 12];
+      Radix<4>::dft(t);
+    }
 #pragma unroll
     for (int k2 = 0; k2 < 4; ++k2) u[n1][k2] = (n1 * k2 == 0) ? t[k2] : cmul(t[k2], w16(n1 * k2));
   }
@@ -264,6 +275,80 @@ __device__ __forceinline__ void dft16(float2* v) {
     Radix<4>::dft(t);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = t[k1];
+  }
+}
+
+// i * a.
+__device__ __forceinline__ float2 cmul_i(float2 a) { return make_float2(-a.y, a.x); }
+
+// dft16 restricted to the outputs k = k2 + 4 k1 with k1 = 0 (v[0..3]) and
+// k1 = 3 (v[12..15]); the other outputs are left undefined.  Every
+// first-stage DFT4 and twiddle is needed, but each second-stage DFT4 yields
+// two sums: k1 = 0, sum_n1 u, and k1 = 3, sum_n1 W4^{3 n1} u = u0 + i u1 -
+// u2 - i u3.  Outputs the caller never reads are removed by the compiler.
+__device__ __forceinline__ void dft16_ends(float2* v) {
+  float2 u[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    float2 t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    Radix<4>::dft(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) u[n1][k2] = (n1 * k2 == 0) ? t[k2] : cmul(t[k2], w16(n1 * k2));
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    const float2 s02 = cadd(u[0][k2], u[2][k2]), d02 = csub(u[0][k2], u[2][k2]);
+    const float2 s13 = cadd(u[1][k2], u[3][k2]), d13 = csub(u[1][k2], u[3][k2]);
+    v[k2] = cadd(s02, s13);
+    v[k2 + 12] = cadd(d02, cmul_i(d13));
+  }
+}
+
+// dft16 of an input that is zero except at n = 0, 1, 14, 15 (all 16 outputs):
+// with n = n1 + 4 m the first-stage DFT4 over m has one nonzero input, at
+// m = 0 for n1 = 0, 1 and m = 3 for n1 = 2, 3, so its outputs are that input
+// times W4^{m k2} = 1 or i^k2.
+__device__ __forceinline__ void dft16_in_ends(float2* v) {
+  float2 u[4][4];
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    u[0][k2] = v[0];
+    u[1][k2] = k2 == 0 ? v[1] : cmul(v[1], w16(k2));
+    float2 a = v[14], c = v[15];                     // times i^k2
+#pragma unroll
+    for (int q = 0; q < k2; ++q) {
+      a = cmul_i(a);
+      c = cmul_i(c);
+    }
+    u[2][k2] = k2 == 0 ? a : cmul(a, w16(2 * k2));
+    u[3][k2] = k2 == 0 ? c : cmul(c, w16(3 * k2));
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    float2 t[4] = {u[0][k2], u[1][k2], u[2][k2], u[3][k2]};
+    Radix<4>::dft(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = t[k1];
+  }
+}
+
+// dft16 restricted to the outputs k < 8 (k = k2 + 4 k1, k1 = 0, 1): the
+// second-stage DFT4s give sum_n1 u and u0 - i u1 - u2 + i u3 (W4 = -i).
+__device__ __forceinline__ void dft16_lo(float2* v) {
+  float2 u[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    float2 t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    Radix<4>::dft(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) u[n1][k2] = (n1 * k2 == 0) ? t[k2] : cmul(t[k2], w16(n1 * k2));
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    const float2 s02 = cadd(u[0][k2], u[2][k2]), d02 = csub(u[0][k2], u[2][k2]);
+    const float2 s13 = cadd(u[1][k2], u[3][k2]), d13 = csub(u[1][k2], u[3][k2]);
+    v[k2] = cadd(s02, s13);
+    v[k2 + 4] = cadd(d02, cmul_mi(d13));
   }
 }
 
@@ -278,6 +363,49 @@ __device__ __forceinline__ void dft32(float2* v) {
   }
   dft16<false>(a);
   dft16<false>(c);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) {
+    const double ang = -2.0 * 3.14159265358979323846 * k2 / 32.0;
+    const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+    const float2 t = (k2 == 0) ? c[k2] : cmul(c[k2], w);
+    v[k2] = cadd(a[k2], t);
+    v[k2 + 16] = csub(a[k2], t);
+  }
+}
+
+// dft32 restricted to the outputs 0..3 and 28..31 (from the half-DFT16s'
+// outputs 0..3 and 12..15).
+__device__ __forceinline__ void dft32_ends(float2* v) {
+  float2 a[16], c[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 16; ++n2) {
+    a[n2] = v[2 * n2];
+    c[n2] = v[2 * n2 + 1];
+  }
+  dft16_ends(a);
+  dft16_ends(c);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) {
+    if (k2 >= 4 && k2 < 12) continue;
+    const double ang = -2.0 * 3.14159265358979323846 * k2 / 32.0;
+    const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+    const float2 t = (k2 == 0) ? c[k2] : cmul(c[k2], w);
+    if (k2 < 4) v[k2] = cadd(a[k2], t);
+    else v[k2 + 16] = csub(a[k2], t);
+  }
+}
+
+// dft32 of an input that is zero except at n = 0..3 and 28..31 (all 32
+// outputs): the even and odd halves are zero except at n2 = 0, 1, 14, 15.
+__device__ __forceinline__ void dft32_in_ends(float2* v) {
+  float2 a[16], c[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 16; ++n2) {
+    a[n2] = v[2 * n2];
+    c[n2] = v[2 * n2 + 1];
+  }
+  dft16_in_ends(a);
+  dft16_in_ends(c);
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) {
     const double ang = -2.0 * 3.14159265358979323846 * k2 / 32.0;

@@ -302,9 +302,10 @@ __global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers
             t[rr] = make_float2(fv * A[u][o].x, -fv * A[u][o].y);   // conj(f Abar)
           }
         }
+        // nonzero inputs only at the KL = N/128 kept rows of each end
         if constexpr (R2 == 8) Radix<8>::dft(t);
-        else if constexpr (R2 == 16) dft16<false>(t);
-        else dft32(t);
+        else if constexpr (R2 == 16) dft16_in_ends(t);
+        else dft32_in_ends(t);
         float2* dst = lines + k * LY + s;
 #pragma unroll
         for (int j = 0; j < R2; ++j) dst[17 * j] = (j == 0) ? t[0] : cmul(t[j], tw[j * 16 + s]);
@@ -411,6 +412,8 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
       }
     }
     if constexpr (R2 == 8) Radix<8>::dft(t);
+    else if constexpr (ENDS && R2 == 16) dft16_in_ends(t);    // rows rr < 2 or >= 14 only
+    else if constexpr (ENDS && R2 == 32) dft32_in_ends(t);    // rows rr < 4 or >= 28 only
     else if constexpr (R2 == 16) dft16<false>(t);
     else dft32(t);
     float2* dst = tbuf + l * LY + s;
@@ -832,7 +835,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
           }
         }
       }
-      dft16<false>(v);
+      dft16_in_ends(v);                                // nonzero inputs r = 0, 1, 14, 15
       float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll
       for (int r = 0; r < 16; ++r) dst[r] = v[r];
@@ -850,8 +853,10 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
         t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + j1]);
       }
       if constexpr (R2 == 8) Radix<8>::dft(t);
-      else if constexpr (R2 == 16) dft16<false>(t);
-      else dft32(t);
+      else if constexpr (R2 == 16) {
+        if constexpr (HALF) dft16_lo(t);                // outputs rr < 8 only
+        else dft16<false>(t);
+      } else dft32(t);
       const int c0 = 2 * l1;
       const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
       float* col0 = drho + c0 * LY;

@@ -985,8 +985,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
         t[r] = (r == 0) ? y : cmul(y, tw[r * 16 + j]);
       }
       if constexpr (R2 == 8) Radix<8>::dft(t);
-      else if constexpr (R2 == 16) dft16<false>(t);
-      else dft32(t);
+      else if constexpr (R2 == 16) dft16_ends(t);             // outputs 0, 1, 14, 15
+      else dft32_ends(t);                                     // outputs 0..3, 28..31
       // kz <= N / 8 (plan check): at most KL = N / 128 kept outputs at each end
       constexpr int KL = N / 128;
       // Z(N - k) = t[R2 - 1 - rr] of lane (16 - j) mod 16; for j = 0 it is

@@ -274,3 +274,13 @@ def test_dense_tiles_multiple_rounds(gpu):
     H, C = engine_hist([pos], syms, box, 16, 2.0, 0.0, 100.0)
     Ho, Co = oracle_hist([pos], syms, box, 16, 2.0, 0.0, 100.0)
     compare(H, C, Ho, Co, label="dense")
+
+
+def test_pruned_z_n512_exact_half_matches_full_transform(gpu):
+    """N = 512 on a 256 cell (sigma 2, n_z = N/2: the bounds-free stage 0):
+    pruned transforms against the engine's full Stockham transforms."""
+    box = np.diag([160.0] * 3)
+    pos, syms = O.synthetic_system(30000, 160.0, seed=22)
+    H1, C1 = engine_hist([pos], syms, box, 256, 2.0, 0.0, 0.3, tile=(4, 4))
+    H2, C2 = engine_hist([pos], syms, box, 256, 2.0, 0.0, 0.3, tile=(4, 4), flags=2)
+    compare(H1, C1, H2, C2, tol=2e-6, label="pruned vs full, N=512, n_z = N/2")
