@@ -321,27 +321,30 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps):
 
 
 def kernel_bytes(eng, frames_per_launch, n_atoms):
-    """Algorithmic HBM bytes per launch of each kernel (float32 complex = 8 B)."""
+    """Algorithmic bytes per launch of each kernel: the pruned byte model
+    B_fwd = 16N + 8E n^3 + 16E n^2 (M+1) + 16E n (2M+1)(M+1) per frame and its
+    backward B_bwd = 24E n (2M+1)(M+1) + 16E n^2 (M+1) + 8E n^3 + 28N (each
+    array that must round-trip memory between the stages counted once read and
+    once written; float32 grids, complex64 spectra), split over the stages each
+    kernel implements.  The spreading kernels carry the density-grid term
+    8E n^3 although the fused implementation keeps that grid on chip: the
+    measured HBM bytes are reported beside it.  The atom sort (prep, scan,
+    scatter) is not part of the model (None)."""
     kx, ky, kz = eng.kept()
     n = eng.spec.n
     E = len(eng.spec.symbols)
     F = frames_per_launch
-    P = eng.spec.order
-    vis = (4 + P - 1) / 4 * (8 + P - 1) / 8  # tile visits per atom, default 4 x 8 column tile
-    zb = 8.0 * E * n[0] * n[1] * kz          # z-pass output per frame
-    yb = 8.0 * E * n[0] * ky * kz            # y-pass output per frame
+    grid = 8.0 * E * n[0] * n[1] * n[2]      # density grids: spread writes, z pass reads (f32)
+    zb = 8.0 * E * n[0] * n[1] * kz          # z-pass output (complex64), per direction
+    yb = 8.0 * E * n[0] * ky * kz            # y-pass output (complex64), per direction
     return {
-        "prep": F * n_atoms * (12 + 4 + 8),
-        "scatter": F * n_atoms * (12 + 8 + 16),
-        "spread_z": F * (16.0 * n_atoms + zb),
-        "yfft": F * (zb + yb),
-        "xfft_epi": F * yb,
-        "adj_x": F * 2 * yb,
-        "adj_y": F * (yb + zb),
-        # spectrum in, staged records in, one 12 B partial per tile visit
-        "adj_z": F * (zb + 16.0 * n_atoms + vis * 12.0 * n_atoms),
-        # xyz in, visited partials in, grad out
-        "grad_sum": F * n_atoms * (12 + vis * 12 + 12),
+        "spread_z": F * (16.0 * n_atoms + grid + zb),     # atoms in, grid, z output out
+        "yfft": F * (zb + yb),                            # z output in, y output out
+        "xfft_epi": F * yb,                               # y output in (+ the bins)
+        "adj_x": F * 2 * yb,                              # y tiles re-read, adjoint x out
+        "adj_y": F * (yb + zb),                           # adjoint x in, adjoint y out
+        "adj_z": F * (zb + grid + 16.0 * n_atoms),        # adjoint y in, potential grid, atoms in
+        "grad_sum": F * 12.0 * n_atoms,                   # gradient out
     }
 
 
