@@ -98,7 +98,7 @@ def mix_symbols(rng, n_atoms, mix):
 def batched_uneven_workload(count=32, frames=8, seed=3):
     """32 systems, atom counts log-uniform in 10k-300k at 0.1 atoms/A^3,
     cubic grids = smallest 7-smooth >= L/0.75 A clipped to [64, 192],
-    sigma 2, 8 frames each (0.5 A jitter), C1 element mix."""
+    sigma 2, 8 frames each (0.5 A jitter), H/C/O/N/S 50/30/15/4/1 %."""
     out = []
     rng = np.random.default_rng(seed)
     for s_ in range(count):

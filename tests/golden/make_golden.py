@@ -1,7 +1,7 @@
 """Writes the golden fixtures of tests/golden/ (run from the repository root:
 ``python tests/golden/make_golden.py``).
 
-  c1_avg.npz, c1_given.npz
+  atoms5k_box37_avg.npz, atoms5k_box37_given.npz
       The 5,000-atom configuration (37 A box, H/C/O/N/S 50/30/15/4/1 %,
       seed 1, order 4, 64^3 cell, sigma 2 -> 128^3 FFT, 0.01 A^-1 bins
       requested, q <= 0.5): positions, element ids, the FP64 oracle's
@@ -12,7 +12,8 @@
       Outputs of the reference's own host helpers compiled from its sources
       by oracle/build_ref.sh (oracle/_ref/ref_driver): B-spline weights and
       derivatives (BSpline.cpp), modulus tables (BSpmod.cpp), the box
-      inverse (Cell.cpp) and explicit-solvent counts (AtomCounter.cpp).  Kept
+      inverse (Cell.cpp; orthorhombic, and triclinic through the transposed
+      box) and explicit-solvent counts (AtomCounter.cpp).  Kept
       as data so the pins hold where the reference tree is absent.
 
 The oracle is the checker; these files pin it (and through it the engine) on
@@ -32,19 +33,21 @@ sys.path.insert(0, ROOT)
 
 from oracle import saxs_oracle as O  # noqa: E402
 
-C1 = dict(n_atoms=5000, L=37.0, seed=1, n=64, sigma=2.0, dq=0.01, qcut=0.5, order=4)
+ATOMS5K = dict(n_atoms=5000, L=37.0, seed=1, n=64, sigma=2.0, dq=0.01, qcut=0.5, order=4)
 REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 
 BSPLINE_W = (0.0, 0.125, 0.3, 0.5, 0.77, 0.999)
 BSPMOD = ((4, 12, 30, 128), (6, 16, 21, 64))
 CELLS = ((37.0, 37.0, 37.0), (37.0, 41.5, 100.0))
+TRICLINIC = (((30.0, 0.0, 0.0), (7.5, 28.0, 0.0), (-5.0, 6.25, 33.0)),
+             ((61.3, 0.0, 0.0), (-20.4, 57.7, 0.0), (-20.4, -28.9, 50.0)))
 SOLVENT = (((37.0, 37.0, 37.0), (64, 64, 64), "SPCE", {}),
            ((100.0, 100.0, 100.0), (128, 128, 128), "TIP3P", {"Na": 150, "Cl": 150}),
            ((80.5, 61.25, 90.0), (96, 72, 108), "TIP4P", {"K": 7, "Cl": 9}))
 
 
-def c1_fixture(pad_given):
-    c = C1
+def atoms5k_fixture(pad_given):
+    c = ATOMS5K
     pos, syms = O.synthetic_system(c["n_atoms"], c["L"], seed=c["seed"])
     symbols, ids = O.element_ids(syms)
     box = np.diag([c["L"]] * 3)
@@ -66,7 +69,7 @@ def ref(*args):
 
 
 def ref_helpers():
-    d = {"bspline": {}, "bspmod": [], "cell": [], "solvent": []}
+    d = {"bspline": {}, "bspmod": [], "cell": [], "triclinic": [], "solvent": []}
     for p in (4, 6):
         rows = []
         for line in ref("bspline", p, *BSPLINE_W):
@@ -80,6 +83,11 @@ def ref_helpers():
     for lengths in CELLS:
         line = ref("cell", lengths[0], 0, 0, 0, lengths[1], 0, 0, 0, lengths[2])[0]
         d["cell"].append({"lengths": list(lengths), "oc": [float(v) for v in line.split()]})
+    for B in TRICLINIC:
+        # Cell.cpp's upper-triangular inverse applied to B^T (rows a, b, c -> columns)
+        bt = np.asarray(B, dtype=np.float64).T.ravel()
+        line = ref("cell", *bt)[0]
+        d["triclinic"].append({"box": [list(r) for r in B], "inverse": [float(v) for v in line.split()]})
     for lengths, n, model, ions in SOLVENT:
         args = [*lengths, model, *n]
         for k, v in ions.items():
@@ -94,9 +102,9 @@ def ref_helpers():
 
 
 def main():
-    np.savez_compressed(os.path.join(HERE, "c1_avg.npz"), **c1_fixture(None))
-    given = O.solvent_pads((C1["L"],) * 3, (C1["n"],) * 3, "SPCE", {})
-    np.savez_compressed(os.path.join(HERE, "c1_given.npz"), **c1_fixture(given))
+    np.savez_compressed(os.path.join(HERE, "atoms5k_box37_avg.npz"), **atoms5k_fixture(None))
+    given = O.solvent_pads((ATOMS5K["L"],) * 3, (ATOMS5K["n"],) * 3, "SPCE", {})
+    np.savez_compressed(os.path.join(HERE, "atoms5k_box37_given.npz"), **atoms5k_fixture(given))
     if os.path.exists(REF_DRIVER):
         with open(os.path.join(HERE, "ref_helpers.json"), "w") as fh:
             json.dump(ref_helpers(), fh, indent=1)

@@ -28,8 +28,8 @@ def helpers():
         return json.load(fh)
 
 
-@pytest.mark.parametrize("name", ["c1_avg.npz", "c1_given.npz"])
-def test_c1_fixture_reproduced_by_oracle(name):
+@pytest.mark.parametrize("name", ["atoms5k_box37_avg.npz", "atoms5k_box37_given.npz"])
+def test_5k_atom_fixture_reproduced_by_oracle(name):
     d = load(name)
     L, n = float(d["box_len"]), int(d["n"])
     pos, syms = O.synthetic_system(5000, L, seed=1)
@@ -57,7 +57,7 @@ def test_given_pad_moves_only_odd_shells():
     """The two fixtures differ by (rho_border - pad) * FFT(1_cell), which is zero
     wherever one component of m is even and nonzero (nn = 2n): bin 2 holds
     only such shells here (|m|^2 = 4..8) and is identical; bin 1 is not."""
-    a, g = load("c1_avg.npz"), load("c1_given.npz")
+    a, g = load("atoms5k_box37_avg.npz"), load("atoms5k_box37_given.npz")
     assert a["H"][2] == pytest.approx(g["H"][2], rel=1e-12)
     assert abs(a["H"][1] - g["H"][1]) > 0.5 * a["H"][1]
 
@@ -94,3 +94,10 @@ def test_solvent_pads_against_reference_vectors(helpers):
         for s, v in case["pads"].items():
             assert np.float32(mine[s]) == np.float32(v)
             assert np.float32(orc[s]) == np.float32(v)
+
+
+def test_triclinic_inverse_against_reference_vectors(helpers):
+    for case in helpers["triclinic"]:
+        M_ref = np.array(case["inverse"], dtype=np.float64).reshape(3, 3).astype(np.float32)
+        assert np.array_equal(M_ref, G.inverse_box_matrix(case["box"]))
+        assert np.array_equal(M_ref, O.inverse_box_matrix(case["box"]))

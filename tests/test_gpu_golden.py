@@ -13,9 +13,11 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("name,plain", [("c1_avg.npz", False), ("c1_given.npz", False),
-                                        ("c1_avg.npz", True), ("c1_given.npz", True)])
-def test_engine_matches_golden_c1(gpu, name, plain):
+@pytest.mark.parametrize("name,plain", [("atoms5k_box37_avg.npz", False),
+                                        ("atoms5k_box37_given.npz", False),
+                                        ("atoms5k_box37_avg.npz", True),
+                                        ("atoms5k_box37_given.npz", True)])
+def test_engine_matches_golden_5k_atoms(gpu, name, plain):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
     d = np.load(os.path.join(GOLDEN, name), allow_pickle=False)

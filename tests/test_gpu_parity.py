@@ -60,13 +60,13 @@ def test_single_atom_on_node(gpu):
     compare(H, C, Ho, Co, label="single atom")
 
 
-def test_c1_config(gpu):
+def test_5k_atoms_37A_box_64_grid(gpu):
     box = np.diag([37.0] * 3)
     pos, syms = O.synthetic_system(5000, 37.0, seed=1)
     H, C = engine_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
     assert list(C[1:6].astype(int)) == [34, 82, 178, 258, 402]
     Ho, Co = oracle_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
-    compare(H, C, Ho, Co, label="C1")
+    compare(H, C, Ho, Co, label="5k atoms, 37 A, 64 -> 128")
 
 
 def test_full_spectrum_small_grid(gpu):
@@ -134,12 +134,12 @@ def plain_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4):
     return H, C
 
 
-def test_plain_path_c1(gpu):
+def test_plain_path_5k_atoms(gpu):
     box = np.diag([37.0] * 3)
     pos, syms = O.synthetic_system(5000, 37.0, seed=1)
     H, C = plain_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
     Ho, Co = oracle_hist([pos], syms, box, 64, 2.0, 0.01, 0.5)
-    compare(H, C, Ho, Co, label="plain C1")
+    compare(H, C, Ho, Co, label="plain, 5k atoms")
 
 
 @pytest.mark.parametrize("n,sigma,order", [(16, 2.0, 4), (30, 2.0, 4), (32, 2.0, 6)])

@@ -180,14 +180,14 @@ def _count_reference_loop(n_cell, nn, L, qcut, dq):
     return C, plan
 
 
-def test_known_shell_counts_c1():
+def test_known_shell_counts_64_grid_37A():
     C, plan = _count_reference_loop(64, 128, 37.0, 0.5, 0.01)
     assert plan.nn == (128, 128, 128)
     assert plan.hp.num_bins == 6
     assert C[1:] == [34, 82, 178, 258, 402]
 
 
-def test_known_shell_counts_c2():
+def test_known_shell_counts_128_grid_100A():
     C, plan = _count_reference_loop(128, 256, 100.0, 0.6, 0.005)
     assert plan.hp.num_bins == 20
     assert C[1:] == [34, 82, 178, 258, 450, 506, 786, 922, 1226, 1426, 1690, 2050, 2458,
