@@ -78,10 +78,15 @@ typedef struct saxs_plan saxs_plan;
  *  flags        SAXS_FLAG_PER_ELEMENT_SPREAD: one spreading workgroup per
  *               (tile, element) instead of one per tile for all elements;
  *               SAXS_FLAG_NO_PRUNED_Z: full Stockham z and y transforms even
- *               where the output-pruned radix-16 ones apply.
+ *               where the output-pruned radix-16 ones apply;
+ *               SAXS_FLAG_WIDE_RADIX: Stockham transforms factor nn with the
+ *               widest register radices first (12, 10, 9, 8, ...; fewer
+ *               stages) instead of 8/4/2 then 3, 5, 7.  The autotuner picks
+ *               among these per grid.
  */
 #define SAXS_FLAG_PER_ELEMENT_SPREAD 1
 #define SAXS_FLAG_NO_PRUNED_Z 2
+#define SAXS_FLAG_WIDE_RADIX 4
 typedef struct saxs_desc {
   int32_t n[3];
   int32_t nn[3];

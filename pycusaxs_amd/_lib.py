@@ -51,6 +51,7 @@ class SaxsDesc(C.Structure):
 
 FLAG_PER_ELEMENT_SPREAD = 1
 FLAG_NO_PRUNED_Z = 2
+FLAG_WIDE_RADIX = 4
 BWD_REUSE_FORWARD = 1
 
 

@@ -134,6 +134,46 @@ struct Radix<5> : RadixOdd<5> {};
 template <>
 struct Radix<7> : RadixOdd<7> {};
 
+// Composite radix A*B in registers (Cooley-Tukey): n = B n1 + n2, k = k1 + A k2,
+// X[k] = sum_n2 W_B^{n2 k2} W_{AB}^{n2 k1} sum_n1 x[B n1 + n2] W_A^{n1 k1};
+// natural order in and out.  Wider Stockham stages mean fewer LDS passes.
+template <int A, int B>
+struct RadixCT {
+  static __device__ __forceinline__ void dft(float2* v) {
+    constexpr int N = A * B;
+    float2 y[B][A];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) y[n2][n1] = v[B * n1 + n2];
+      Radix<A>::dft(y[n2]);
+#pragma unroll
+      for (int k1 = 1; k1 < A; ++k1) {
+        if (n2 == 0) continue;
+        const double ang = -2.0 * 3.14159265358979323846 * (double)(n2 * k1) / (double)N;
+        y[n2][k1] = cmul(y[n2][k1], make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang)));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      float2 z[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) z[n2] = y[n2][k1];
+      Radix<B>::dft(z);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = z[k2];
+    }
+  }
+};
+template <>
+struct Radix<6> : RadixCT<2, 3> {};
+template <>
+struct Radix<9> : RadixCT<3, 3> {};
+template <>
+struct Radix<10> : RadixCT<2, 5> {};
+template <>
+struct Radix<12> : RadixCT<4, 3> {};
+
 // One Stockham stage over nlines lines (line stride ls) of length N.
 template <int R, int MAXC>
 __device__ __forceinline__ void fft_stage(float2* buf, int ls, int nlines, int N, int Ns,
@@ -196,10 +236,12 @@ __device__ __forceinline__ void fft_stage0(float2* buf, int ls, int nlines, int 
 }
 
 // Forward FFT of nlines lines of length d.N in LDS, from stage s0 on (the
-// sub-transform size before stage s0 is Ns0).  POW2 restricts the code to
-// radices 8/4/2 (fewer registers).  The caller must have synchronised after
+// sub-transform size before stage s0 is Ns0).  The radix set RS (named POW2
+// in the kernels) bounds the code, and so the registers, of one instance:
+// 1 radices 8/4/2, 0 also 3/5/7, 2 also the composite radices 6, 9, 10 and
+// 12 (fft_radix_set).  The caller must have synchronised after
 // filling buf; the routine ends synchronised.
-template <int MAXC, bool POW2 = false>
+template <int MAXC, int POW2 = 0>
 __device__ void fft_lines(float2* buf, int ls, int nlines, const FftDesc& d,
                           const float2* __restrict__ tw, int s0 = 0, int Ns0 = 1) {
   int Ns = Ns0;
@@ -211,27 +253,39 @@ __device__ void fft_lines(float2* buf, int ls, int nlines, const FftDesc& d,
       fft_stage<4, MAXC>(buf, ls, nlines, d.N, Ns, tw);
     } else if (R == 2) {
       fft_stage<2, MAXC>(buf, ls, nlines, d.N, Ns, tw);
-    } else if constexpr (!POW2) {
+    } else if constexpr (POW2 != 1) {
       if (R == 3) fft_stage<3, MAXC>(buf, ls, nlines, d.N, Ns, tw);
       else if (R == 5) fft_stage<5, MAXC>(buf, ls, nlines, d.N, Ns, tw);
       else if (R == 7) fft_stage<7, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+      else if constexpr (POW2 == 2) {
+        if (R == 6) fft_stage<6, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+        else if (R == 9) fft_stage<9, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+        else if (R == 10) fft_stage<10, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+        else if (R == 12) fft_stage<12, MAXC>(buf, ls, nlines, d.N, Ns, tw);
+      }
     }
     Ns *= R;
   }
 }
 
 // Whole transform with the first stage fed by a loader.
-template <int MAXC, bool POW2, class Load>
+template <int MAXC, int POW2, class Load>
 __device__ void fft_lines_loaded(float2* buf, int ls, int nlines, const FftDesc& d,
                                  const float2* __restrict__ tw, Load ld) {
   const int R = d.radix[0];
   if (R == 8) fft_stage0<8>(buf, ls, nlines, d.N, ld);
   else if (R == 4) fft_stage0<4>(buf, ls, nlines, d.N, ld);
   else if (R == 2) fft_stage0<2>(buf, ls, nlines, d.N, ld);
-  else if constexpr (!POW2) {
+  else if constexpr (POW2 != 1) {
     if (R == 3) fft_stage0<3>(buf, ls, nlines, d.N, ld);
     else if (R == 5) fft_stage0<5>(buf, ls, nlines, d.N, ld);
     else if (R == 7) fft_stage0<7>(buf, ls, nlines, d.N, ld);
+    else if constexpr (POW2 == 2) {
+      if (R == 6) fft_stage0<6>(buf, ls, nlines, d.N, ld);
+      else if (R == 9) fft_stage0<9>(buf, ls, nlines, d.N, ld);
+      else if (R == 10) fft_stage0<10>(buf, ls, nlines, d.N, ld);
+      else if (R == 12) fft_stage0<12>(buf, ls, nlines, d.N, ld);
+    }
   }
   fft_lines<MAXC, POW2>(buf, ls, nlines, d, tw, 1, R);
 }
@@ -414,6 +468,17 @@ __device__ __forceinline__ void dft32_in_ends(float2* v) {
     v[k2] = cadd(a[k2], t);
     v[k2 + 16] = csub(a[k2], t);
   }
+}
+
+// Radix set of a factorisation: 1 only 8/4/2, 0 also 3/5/7, 2 composite radices.
+__host__ __device__ inline int fft_radix_set(const FftDesc& d) {
+  int rs = 1;
+  for (int s = 0; s < d.nst; ++s) {
+    const int r = d.radix[s];
+    if (r == 6 || r == 9 || r == 10 || r == 12) return 2;
+    if (r != 2 && r != 4 && r != 8) rs = 0;
+  }
+  return rs;
 }
 
 __host__ __device__ inline bool fft_is_pow2(const FftDesc& d) {

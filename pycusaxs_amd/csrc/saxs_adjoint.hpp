@@ -58,7 +58,7 @@ __global__ void k_fill_zero(float* a, size_t n, double* d, int nd) {
 
 // x-pass adjoint: per (kept m_y, kz chunk, frame).  Recomputes A from Y,
 // forms Abar, and for every element writes Ybar_e over ybuf in place.
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_adj_x(Params p, Buffers b, AdjBuffers g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
@@ -336,7 +336,7 @@ __host__ __device__ inline size_t adjx_lds_bytes(const Params& p) {
 
 // y-pass adjoint: per (x, kz chunk, frame*E + e): Zbar(x, y, kz) for y < n_y
 // from Ybar over the kept m_y; writes zbuf.
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_adj_y(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int x = blockIdx.x, ch = blockIdx.y;
@@ -452,7 +452,7 @@ __host__ __device__ inline size_t adjz_lds_bytes(const Params& p) {
 }
 
 // z-pass adjoint + B-spline gradient gather: per (column tile, e, f).
-template <int P, bool POW2>
+template <int P, int POW2>
 __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;

@@ -40,9 +40,30 @@ int fail(int code, const char* fmt, ...) {
 
 constexpr size_t kLdsLimit = 64 * 1024;
 
-bool factor_radices(int N, saxs::FftDesc& d) {
+// Stockham stage radices of N.  Default: radix 8/4/2 for the power of two,
+// then 3, 5, 7.  wide: greedy with the widest register radix first (12, 10,
+// 9, 8, 7, 6, 5, 4, 3, 2), fewer stages and LDS passes for sizes with
+// several odd factors, at fewer lines per call.
+bool factor_radices(int N, saxs::FftDesc& d, bool wide = false) {
   d.N = N;
   d.nst = 0;
+  if (wide) {
+    int m = N, nst = 0;
+    while (m > 1) {
+      int r = 0;
+      for (int c : {12, 10, 9, 8, 7, 6, 5, 4, 3, 2})
+        if (m % c == 0) {
+          r = c;
+          break;
+        }
+      if (r == 0 || nst == saxs::kMaxStages) return false;
+      d.radix[nst++] = r;
+      m /= r;
+    }
+    if (nst == 0) d.radix[nst++] = 1;  // N == 1 is rejected by the caller
+    d.nst = nst;
+    return true;
+  }
   int m = N, twos = 0;
   while (m % 2 == 0) {
     m /= 2;
@@ -504,7 +525,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   dim3 gs(p.ntx * p.nty, p.E, n_frames);
   const size_t lds_s = saxs::spread_lds_bytes(p);
   { Probe pr(pl, SAXS_K_SPREAD_Z, st);
-  const bool pow2 = saxs::fft_is_pow2(p.fz);
+  const int rsz = saxs::fft_radix_set(p.fz);
   const size_t lds_all = saxs::spread_all_lds_bytes(p);
   const size_t lds_p = saxs::spread_p_lds_bytes(p);
   const int Nz = p.nn[2];
@@ -530,22 +551,22 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 #undef SAXS_ZP
   } else if (pl->spread_all && lds_all <= kLdsLimit) {
     dim3 ga(p.ntx * p.nty, n_frames);
-    if (p.order == 4 && pow2)
-      hipLaunchKernelGGL((saxs::k_spread_zall<4, true>), ga, dim3(saxs::kWG), lds_all, st, p, b);
-    else if (p.order == 4)
-      hipLaunchKernelGGL((saxs::k_spread_zall<4, false>), ga, dim3(saxs::kWG), lds_all, st, p, b);
-    else if (pow2)
-      hipLaunchKernelGGL((saxs::k_spread_zall<6, true>), ga, dim3(saxs::kWG), lds_all, st, p, b);
-    else
-      hipLaunchKernelGGL((saxs::k_spread_zall<6, false>), ga, dim3(saxs::kWG), lds_all, st, p, b);
-  } else if (p.order == 4 && pow2)
-    hipLaunchKernelGGL((saxs::k_spread_z<4, true>), gs, dim3(saxs::kWG), lds_s, st, p, b);
-  else if (p.order == 4)
-    hipLaunchKernelGGL((saxs::k_spread_z<4, false>), gs, dim3(saxs::kWG), lds_s, st, p, b);
-  else if (pow2)
-    hipLaunchKernelGGL((saxs::k_spread_z<6, true>), gs, dim3(saxs::kWG), lds_s, st, p, b);
-  else
-    hipLaunchKernelGGL((saxs::k_spread_z<6, false>), gs, dim3(saxs::kWG), lds_s, st, p, b); }
+#define SAXS_ZA(PP, RS) hipLaunchKernelGGL((saxs::k_spread_zall<PP, RS>), ga, dim3(saxs::kWG), lds_all, st, p, b)
+    if (p.order == 4) {
+      if (rsz == 1) SAXS_ZA(4, 1); else if (rsz == 0) SAXS_ZA(4, 0); else SAXS_ZA(4, 2);
+    } else {
+      if (rsz == 1) SAXS_ZA(6, 1); else if (rsz == 0) SAXS_ZA(6, 0); else SAXS_ZA(6, 2);
+    }
+#undef SAXS_ZA
+  } else {
+#define SAXS_ZS(PP, RS) hipLaunchKernelGGL((saxs::k_spread_z<PP, RS>), gs, dim3(saxs::kWG), lds_s, st, p, b)
+    if (p.order == 4) {
+      if (rsz == 1) SAXS_ZS(4, 1); else if (rsz == 0) SAXS_ZS(4, 0); else SAXS_ZS(4, 2);
+    } else {
+      if (rsz == 1) SAXS_ZS(6, 1); else if (rsz == 0) SAXS_ZS(6, 0); else SAXS_ZS(6, 2);
+    }
+#undef SAXS_ZS
+  } }
   { Probe pr(pl, SAXS_K_YFFT, st);
   const int Ny = p.nn[1];
   const unsigned nfe = (unsigned)(n_frames * p.E);
@@ -566,12 +587,12 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
 #undef SAXS_YP
-  } else if (saxs::fft_is_pow2(p.fy))
-    hipLaunchKernelGGL(saxs::k_yfft<true>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG),
-                       saxs::yfft_lds_bytes(p), st, p, b);
-  else
-    hipLaunchKernelGGL(saxs::k_yfft<false>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG),
-                       saxs::yfft_lds_bytes(p), st, p, b); }
+  } else {
+    const int rsy = saxs::fft_radix_set(p.fy);
+#define SAXS_Y(RS) hipLaunchKernelGGL(saxs::k_yfft<RS>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b)
+    if (rsy == 1) SAXS_Y(1); else if (rsy == 0) SAXS_Y(0); else SAXS_Y(2);
+#undef SAXS_Y
+  } }
   if (!with_x) {
     HIP_TRY(hipGetLastError());
     return SAXS_OK;
@@ -589,12 +610,12 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Nx == 128) SAXS_XP(128); else if (Nx == 256) SAXS_XP(256); else SAXS_XP(512);
 #undef SAXS_XP
-  } else if (saxs::fft_is_pow2(p.fx))
-    hipLaunchKernelGGL(saxs::k_xfft_epi<true>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
-                       saxs::xfft_lds_bytes(p), st, p, b);
-  else
-    hipLaunchKernelGGL(saxs::k_xfft_epi<false>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG),
-                       saxs::xfft_lds_bytes(p), st, p, b); }
+  } else {
+    const int rsx = saxs::fft_radix_set(p.fx);
+#define SAXS_X(RS) hipLaunchKernelGGL(saxs::k_xfft_epi<RS>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG), saxs::xfft_lds_bytes(p), st, p, b)
+    if (rsx == 1) SAXS_X(1); else if (rsx == 0) SAXS_X(0); else SAXS_X(2);
+#undef SAXS_X
+  } }
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
@@ -657,8 +678,9 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   for (int a = 0; a < 3; ++a) p.kmax[a] = point_keys ? d->kmax[a] : 0;
   set_cell(p, d->cell);
   p.inv_count = 1.0 / ((double)d->n[0] * d->n[1] * (2 * d->border + 1));
-  if (!factor_radices(p.nn[2], p.fz) || !factor_radices(p.nn[1], p.fy) ||
-      !factor_radices(p.nn[0], p.fx) || p.nn[0] < 2 || p.nn[1] < 2 || p.nn[2] < 2) {
+  const bool wide = (d->flags & SAXS_FLAG_WIDE_RADIX) != 0;
+  if (!factor_radices(p.nn[2], p.fz, wide) || !factor_radices(p.nn[1], p.fy, wide) ||
+      !factor_radices(p.nn[0], p.fx, wide) || p.nn[0] < 2 || p.nn[1] < 2 || p.nn[2] < 2) {
     free_plan(pl);
     return fail(SAXS_EUNSUP, "FFT sizes must be 2,3,5,7-smooth and >= 2");
   }
@@ -895,8 +917,8 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   const int npadg = n_frames * p.E;
   hipLaunchKernelGGL(saxs::k_fill_zero, dim3((unsigned)((npadg + 255) / 256)), dim3(256), 0, st,
                      grad, (size_t)0, pl->padg, npadg);
-  const bool px = saxs::fft_is_pow2(p.fx), py = saxs::fft_is_pow2(p.fy),
-             pz = saxs::fft_is_pow2(p.fz);
+  const int px = saxs::fft_radix_set(p.fx), py = saxs::fft_radix_set(p.fy),
+            pz = saxs::fft_radix_set(p.fz);
   {
     Probe pr(pl, SAXS_K_ADJ_X, st);
     dim3 gx(p.ky, p.nchx, n_frames);
@@ -911,10 +933,11 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } while (0)
       if (Nx == 128) SAXS_AXP(128); else if (Nx == 256) SAXS_AXP(256); else SAXS_AXP(512);
 #undef SAXS_AXP
-    } else if (px)
-      hipLaunchKernelGGL(saxs::k_adj_x<true>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
-    else
-      hipLaunchKernelGGL(saxs::k_adj_x<false>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g);
+    } else {
+#define SAXS_AX(RS) hipLaunchKernelGGL(saxs::k_adj_x<RS>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g)
+      if (px == 1) SAXS_AX(1); else if (px == 0) SAXS_AX(0); else SAXS_AX(2);
+#undef SAXS_AX
+    }
   }
   const int Ny = p.nn[1];
   if (pl->pruned_z && (Ny == 128 || Ny == 256 || Ny == 512)) {
@@ -939,10 +962,9 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } else {
     Probe pr(pl, SAXS_K_ADJ_Y, st);
     dim3 gy(p.n[0], p.nchy, n_frames * p.E);
-    if (py)
-      hipLaunchKernelGGL(saxs::k_adj_y<true>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b);
-    else
-      hipLaunchKernelGGL(saxs::k_adj_y<false>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b);
+#define SAXS_AY(RS) hipLaunchKernelGGL(saxs::k_adj_y<RS>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b)
+    if (py == 1) SAXS_AY(1); else if (py == 0) SAXS_AY(0); else SAXS_AY(2);
+#undef SAXS_AY
   }
   const int Nz = p.nn[2];
   const size_t lds_ap = saxs::adj_p_lds_bytes(p);
@@ -971,14 +993,13 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 gz(p.ntx * p.nty, p.E, n_frames);
     const size_t lds = saxs::adjz_lds_bytes(p);
-    if (p.order == 4 && pz)
-      hipLaunchKernelGGL((saxs::k_adj_z<4, true>), gz, dim3(saxs::kWG), lds, st, p, b, g);
-    else if (p.order == 4)
-      hipLaunchKernelGGL((saxs::k_adj_z<4, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
-    else if (pz)
-      hipLaunchKernelGGL((saxs::k_adj_z<6, true>), gz, dim3(saxs::kWG), lds, st, p, b, g);
-    else
-      hipLaunchKernelGGL((saxs::k_adj_z<6, false>), gz, dim3(saxs::kWG), lds, st, p, b, g);
+#define SAXS_AZ(PP, RS) hipLaunchKernelGGL((saxs::k_adj_z<PP, RS>), gz, dim3(saxs::kWG), lds, st, p, b, g)
+    if (p.order == 4) {
+      if (pz == 1) SAXS_AZ(4, 1); else if (pz == 0) SAXS_AZ(4, 0); else SAXS_AZ(4, 2);
+    } else {
+      if (pz == 1) SAXS_AZ(6, 1); else if (pz == 0) SAXS_AZ(6, 0); else SAXS_AZ(6, 2);
+    }
+#undef SAXS_AZ
   }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_GRAD_SUM, st);
@@ -1063,8 +1084,8 @@ extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t
   const size_t n3 = (size_t)p.n[0] * p.n[1] * p.n[2];
   const size_t nn3 = (size_t)q.nn[0] * q.nn[1] * q.nn[2];
   const size_t nh = (size_t)q.nn[0] * q.nn[1] * h;
-  const bool pz = saxs::fft_is_pow2(q.fz), py = saxs::fft_is_pow2(q.fy),
-             px = saxs::fft_is_pow2(q.fx);
+  const int pz = saxs::fft_radix_set(q.fz), py = saxs::fft_radix_set(q.fy),
+            px = saxs::fft_radix_set(q.fx);
   Probe pr(pl, SAXS_K_PLAIN, st);
   hipLaunchKernelGGL(saxs::k_plain_fill, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st,
                      (float*)nullptr, (size_t)0, pl->acc, nh, (double*)nullptr);
@@ -1088,25 +1109,18 @@ extern "C" int saxs_forward_plain(saxs_plan* pl, const float* xyz, const int32_t
     const size_t npairs = (size_t)q.nn[0] * (q.nn[1] / 2);
     const size_t lds_z = (size_t)np * q.lsz * sizeof(float2) + q.nn[2] * sizeof(float2);
     dim3 gz((unsigned)((npairs + np - 1) / np));
-    if (pz)
-      hipLaunchKernelGGL(saxs::k_plain_zfft<true>, gz, dim3(saxs::kWG), lds_z, st, q, b, pl->sup,
-                         pl->spec, np);
-    else
-      hipLaunchKernelGGL(saxs::k_plain_zfft<false>, gz, dim3(saxs::kWG), lds_z, st, q, b, pl->sup,
-                         pl->spec, np);
+#define SAXS_PZ(RS) hipLaunchKernelGGL(saxs::k_plain_zfft<RS>, gz, dim3(saxs::kWG), lds_z, st, q, b, pl->sup, pl->spec, np)
+    if (pz == 1) SAXS_PZ(1); else if (pz == 0) SAXS_PZ(0); else SAXS_PZ(2);
+#undef SAXS_PZ
     dim3 gy(q.nn[0], q.nchy, 1);
-    if (py)
-      hipLaunchKernelGGL(saxs::k_yfft<true>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(q), st, q, bq);
-    else
-      hipLaunchKernelGGL(saxs::k_yfft<false>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(q), st, q, bq);
+#define SAXS_PY(RS) hipLaunchKernelGGL(saxs::k_yfft<RS>, gy, dim3(saxs::kWG), saxs::yfft_lds_bytes(q), st, q, bq)
+    if (py == 1) SAXS_PY(1); else if (py == 0) SAXS_PY(0); else SAXS_PY(2);
+#undef SAXS_PY
     dim3 gx(q.nn[1], q.nchx, 1);
     const size_t lds_x = (size_t)q.kcx * q.lsx * sizeof(float2) + q.nn[0] * sizeof(float2);
-    if (px)
-      hipLaunchKernelGGL(saxs::k_plain_xfft_scatter<true>, gx, dim3(saxs::kWG), lds_x, st, q, b,
-                         pl->yspec, pl->acc, e);
-    else
-      hipLaunchKernelGGL(saxs::k_plain_xfft_scatter<false>, gx, dim3(saxs::kWG), lds_x, st, q, b,
-                         pl->yspec, pl->acc, e);
+#define SAXS_PX(RS) hipLaunchKernelGGL(saxs::k_plain_xfft_scatter<RS>, gx, dim3(saxs::kWG), lds_x, st, q, b, pl->yspec, pl->acc, e)
+    if (px == 1) SAXS_PX(1); else if (px == 0) SAXS_PX(0); else SAXS_PX(2);
+#undef SAXS_PX
   }
   hipLaunchKernelGGL(saxs::k_plain_modulus, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, q,
                      b, pl->acc);

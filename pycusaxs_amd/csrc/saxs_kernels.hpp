@@ -530,7 +530,7 @@ __host__ __device__ inline size_t spread_lds_bytes(const Params& p) {
   return rho + buf + tw;
 }
 
-template <int P, bool POW2>
+template <int P, int POW2>
 __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, e = blockIdx.y, f = blockIdx.z;
@@ -664,7 +664,7 @@ __host__ __device__ inline size_t spread_all_lds_bytes(const Params& p) {
 // round trip, and while element e is transformed the first records of
 // element e+1 are already in flight (loaded into registers, stored to LDS
 // after the transform).
-template <int P, bool POW2>
+template <int P, int POW2>
 __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x, f = blockIdx.y;
@@ -1034,7 +1034,7 @@ __host__ __device__ inline size_t yfft_lds_bytes(const Params& p) {
   return (size_t)p.kcy * p.lsy * sizeof(float2) + (size_t)p.nn[1] * sizeof(float2);
 }
 
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_yfft(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int x = blockIdx.x, ch = blockIdx.y;
@@ -1165,7 +1165,7 @@ __host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
          (size_t)p.nn[0] * sizeof(float2);
 }
 
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;

@@ -104,7 +104,7 @@ __global__ void k_plain_embed(Params p, const float* grid, float* sup, const dou
 }
 
 // R2C along z over all nn_x * nn_y lines of sup; line pairs (y, y+1).
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_plain_zfft(Params p, Buffers b, const float* sup, float2* zspec,
                                                     int pairs_per_wg) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kWG) k_plain_zfft(Params p, Buffers b, const f
 
 // FFT along x, then acc += f_e(q) F at q <= qcut (scatterKernel).  Input
 // yspec layout [nn_y][nn_x][h] (from k_yfft on the full grid).
-template <bool POW2>
+template <int POW2>
 __global__ void __launch_bounds__(kWG) k_plain_xfft_scatter(Params p, Buffers b, const float2* yspec,
                                                             float2* acc, int e) {
   extern __shared__ __align__(16) unsigned char smem[];

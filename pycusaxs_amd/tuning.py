@@ -1,9 +1,10 @@
 """Autotuner for the engine's launch shapes.
 
 For one system (cell grid, FFT grid, order, element count, atom density)
-it times every candidate spreading tile and z-transform variant on
-synthetic atoms of the same density with the engine's hipEvent hook, and
-keeps the fastest.  Results are cached per device name in a JSON file
+it times every candidate spreading tile, z-transform variant (pruned radix-16
+or full Stockham) and Stockham radix factorization (8/4/2 then 3, 5, 7, or
+the widest register radices first) on synthetic atoms of the same density
+with the engine's hipEvent hook, and keeps the fastest.  Results are cached per device name in a JSON file
 (``PYCUSAXS_AMD_TUNE_CACHE``, default ~/.cache/pycusaxs_amd/tune.json) so
 a shape is tuned once.
 """
@@ -20,7 +21,7 @@ import numpy as np
 from . import _lib
 
 TILES = ((4, 8), (8, 4), (4, 4), (2, 8))
-VARIANTS = (0, _lib.FLAG_NO_PRUNED_Z)
+VARIANTS = (0, _lib.FLAG_NO_PRUNED_Z, _lib.FLAG_WIDE_RADIX)
 TIMED = ("prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce")
 
 

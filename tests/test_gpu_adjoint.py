@@ -197,3 +197,9 @@ def test_adjoint_pruned_n512_matches_full_transform(gpu, n, L, sigma, qcut):
     rel = np.linalg.norm(g1 - g2) / np.linalg.norm(g2)
     print(f"[adjoint N=512 n={n}] pruned vs full rel-L2 {rel:.2e}", flush=True)
     assert rel <= 1e-5
+
+
+@pytest.mark.parametrize("n,qcut", [(27, 100.0), (36, 1.2), (40, 1.0)])
+def test_adjoint_wide_radix(gpu, n, qcut):
+    """The mixed-radix adjoint with the wide-radix factorisation."""
+    run_case(n, 30.0, 1500, qcut=qcut, seed=n, flags=4 | 2)

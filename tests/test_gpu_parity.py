@@ -120,13 +120,13 @@ def test_atoms_outside_box_wrap(gpu):
     compare(H, C, Ho, Co, label="wrapped")
 
 
-def plain_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4):
+def plain_hist(frames, syms, box, n, sigma=2.0, dq=0.0, qcut=0.0, order=4, flags=0):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
     symbols, ids = O.element_ids(syms)
     n3, nn = resolve_grid((n,) * 3, sigma)
     eng = SaxsEngine(SystemSpec(n3, nn, order, tuple(symbols), sigma, dq, qcut), box,
-                     max_atoms=len(frames[0]), max_frames=1)
+                     max_atoms=len(frames[0]), max_frames=1, flags=flags)
     eng.forward_plain(torch.from_numpy(np.stack(frames)), torch.from_numpy(ids))
     torch.cuda.synchronize()
     H, C = eng.H.cpu().numpy(), eng.C.cpu().numpy()
@@ -284,3 +284,23 @@ def test_pruned_z_n512_exact_half_matches_full_transform(gpu):
     H1, C1 = engine_hist([pos], syms, box, 256, 2.0, 0.0, 0.3, tile=(4, 4))
     H2, C2 = engine_hist([pos], syms, box, 256, 2.0, 0.0, 0.3, tile=(4, 4), flags=2)
     compare(H1, C1, H2, C2, tol=2e-6, label="pruned vs full, N=512, n_z = N/2")
+
+
+@pytest.mark.parametrize("n,sigma,qcut", [(27, 2.0, 100.0),    # nn 54 = 9 x 6
+                                          (30, 2.0, 100.0),    # 60 = 15 x 4
+                                          (36, 2.0, 1.2),      # 72 = 12 x 6
+                                          (35, 2.0, 100.0),    # 70 = 10 x 7
+                                          (40, 2.0, 1.0),      # 80 = 16 x 5
+                                          (45, 2.0, 1.0),      # 90 = 15 x 6
+                                          (32, 2.0, 100.0)])   # 64 = 16 x 4 (pow2)
+def test_wide_radix_factorisation(gpu, n, sigma, qcut):
+    """Stockham transforms factored with the widest register radices
+    (16, 15, 12, 10, 9, 6 as composite register DFTs) against the oracle,
+    forward and the plain per-frame path."""
+    box = np.diag([30.0] * 3)
+    pos, syms = O.synthetic_system(1500, 30.0, seed=n + 50)
+    H, C = engine_hist([pos], syms, box, n, sigma, 0.0, qcut, flags=4 | 2)
+    Ho, Co = oracle_hist([pos], syms, box, n, sigma, 0.0, qcut)
+    compare(H, C, Ho, Co, label=f"wide radix n={n}")
+    Hp, Cp = plain_hist([pos], syms, box, n, sigma, 0.0, qcut, flags=4)
+    compare(Hp, Cp, Ho, Co, label=f"plain n={n}")
