@@ -71,6 +71,11 @@ def load(path: Path = LIB_PATH):
         raise ImportError(
             f"MI355X SAXS engine library not built: {path} is missing "
             "(build it with: python -m pycusaxs_amd._build)")
+    # torch first: its HIP runtime (libtorch_hip asks for libamdhip64.so by
+    # its own rpath) then also serves this library's libamdhip64.so.7.  The
+    # other order loads a second runtime, on which torch's streams and
+    # pointers are meaningless.
+    import torch  # noqa: F401
     lib = C.CDLL(str(path))
     vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
     lib.saxs_plan_create.argtypes = [C.POINTER(SaxsDesc), C.POINTER(vp)]

@@ -156,6 +156,31 @@ int kz_of(const saxs::Params& p, int lut_len) {
   return std::min(K[2] + 1, p.nn[2] / 2 + 1);
 }
 
+// Whether the radix-16 output-pruned spreading kernel (k_spread_zp) applies.
+bool zp_applies(bool spread_all, bool pruned_z, const saxs::Params& p) {
+  const int Nz = p.nn[2], nl = p.tx * p.ty / 2;
+  return spread_all && pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
+         2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 && nl * (Nz / 16) <= saxs::kWG &&
+         saxs::spread_p_lds_bytes(p) <= kLdsLimit;
+}
+
+// Half-length H of the first stage of the generic output-pruned spreading
+// kernel (k_spread_zg, N = 2H R2), or 0 where it does not apply: it needs
+// n_z <= N/2, an even R1 = 2H dividing N with order <= R2 <= 64 (a
+// wavefront transforms whole lines), and the LDS.  The widest first stage
+// is taken: stage 1 then sums over the fewest terms.
+int zg_half(bool spread_all, bool pruned_z, const saxs::Params& p) {
+  const int Nz = p.nn[2];
+  if (!spread_all || !pruned_z || 2 * p.n[2] > Nz) return 0;
+  for (int H : {10, 9, 8, 7, 6, 5}) {
+    const int R1 = 2 * H;
+    if (Nz % R1 != 0 || Nz / R1 < p.order || Nz / R1 > 64) continue;
+    if (saxs::spread_g_lds_bytes(p, R1) > kLdsLimit) continue;
+    return H;
+  }
+  return 0;
+}
+
 // Whether the spreading kernel and the z adjoint that will run for column
 // tile (p.tx, p.ty) fit: the pruned kernels where they apply, else the
 // all-element or per-element Stockham ones (the launch picks in that order).
@@ -166,6 +191,7 @@ bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len) {
   const bool pruned = spread_all && pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
                       2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 && nl * (Nz / 16) <= saxs::kWG;
   const bool fwd = (pruned && saxs::spread_p_lds_bytes(p) <= kLdsLimit) ||
+                   zg_half(spread_all, pruned_z, p) > 0 ||
                    (line_ok && spread_all && saxs::spread_all_lds_bytes(p) <= kLdsLimit) ||
                    (line_ok && saxs::spread_lds_bytes(p) <= kLdsLimit);
   const bool bwd = (pruned && saxs::adj_p_lds_bytes(p) <= kLdsLimit) ||
@@ -529,10 +555,9 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const size_t lds_all = saxs::spread_all_lds_bytes(p);
   const size_t lds_p = saxs::spread_p_lds_bytes(p);
   const int Nz = p.nn[2];
-  const bool pruned_ok = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
-                         2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
-                         (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_p <= kLdsLimit;
-  if (!pruned_ok && p.tx * p.ty / 2 > fft_line_cap(p.fz))
+  const bool pruned_ok = zp_applies(pl->spread_all, pl->pruned_z, p);
+  const int zg_h = pruned_ok ? 0 : zg_half(pl->spread_all, pl->pruned_z, p);
+  if (!pruned_ok && !zg_h && p.tx * p.ty / 2 > fft_line_cap(p.fz))
     return fail(SAXS_EUNSUP, "column tile %dx%d exceeds the z transform's line capacity", p.tx, p.ty);
   if (pruned_ok) {
     dim3 ga(p.ntx * p.nty, n_frames);
@@ -549,6 +574,24 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
       if (Nz == 128) SAXS_ZP(6, 128); else if (Nz == 256) SAXS_ZP(6, 256); else SAXS_ZP(6, 512);
     }
 #undef SAXS_ZP
+  } else if (zg_h) {
+    dim3 ga(p.ntx * p.nty, n_frames);
+    const size_t lds_g = saxs::spread_g_lds_bytes(p, 2 * zg_h);
+#define SAXS_ZG(PP, HH) hipLaunchKernelGGL((saxs::k_spread_zg<PP, HH>), ga, dim3(saxs::kWG), lds_g, st, p, b)
+#define SAXS_ZGH(PP)                                                                   \
+  do {                                                                                 \
+    switch (zg_h) {                                                                    \
+      case 10: SAXS_ZG(PP, 10); break;                                                 \
+      case 9: SAXS_ZG(PP, 9); break;                                                   \
+      case 8: SAXS_ZG(PP, 8); break;                                                   \
+      case 7: SAXS_ZG(PP, 7); break;                                                   \
+      case 6: SAXS_ZG(PP, 6); break;                                                   \
+      default: SAXS_ZG(PP, 5); break;                                                  \
+    }                                                                                  \
+  } while (0)
+    if (p.order == 4) SAXS_ZGH(4); else SAXS_ZGH(6);
+#undef SAXS_ZGH
+#undef SAXS_ZG
   } else if (pl->spread_all && lds_all <= kLdsLimit) {
     dim3 ga(p.ntx * p.nty, n_frames);
 #define SAXS_ZA(PP, RS) hipLaunchKernelGGL((saxs::k_spread_zall<PP, RS>), ga, dim3(saxs::kWG), lds_all, st, p, b)

@@ -1030,6 +1030,260 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   }
 }
 
+// ---- output-pruned z transform for any N = 2H R2 with n_z <= N/2 --------
+//
+// Stage 0 (radix R1 = 2H): butterfly j < R2 takes x[j + R2 r]; only r < H
+// can be nonzero (z < n_z <= N/2), so its length-R1 DFT is two length-H
+// DFTs, Y_j[2t] = DFT_H(x_r)[t] and Y_j[2t+1] = DFT_H(x_r W_R1^r)[t].
+// Stage 1 forms only the kept outputs as direct sums over j,
+//   Z(k)     = sum_j Y_j[k mod R1]  W_N^{jk}
+//   Z(N - k) = sum_j Y_j[-k mod R1] conj(W_N^{jk}),   k < kz,
+// which costs 2 kz R2 products per line against the N log N of a full
+// transform whose outputs are mostly discarded (kz is about N/30 here).
+// Y_j[s] sits at (R1 + 1) j + s of its line, and line l occupies the bytes
+// of columns 2l and 2l + 1 as in k_spread_zp.  Each wavefront owns a block
+// of lines and takes them through both stages and the clear for the next
+// element on its own: a stage-0 pass holds whole lines (64 / R2 of them),
+// and a wavefront's LDS operations execute in order, so no workgroup
+// barrier separates the stages.  Stage 1 splits each output's sum over up
+// to 8 lanes, as many as keep the wavefront's lanes busy, and adds the
+// parts with lane shuffles.
+
+__host__ __device__ inline int zg_col_stride(int nz, int P, int N, int R1) {
+  const int a = nz + P - 1, b = (N / R1 * (R1 + 1) + 1) / 2;
+  return ((a > b ? a : b) + 1) & ~1;
+}
+
+__host__ __device__ inline size_t spread_g_lds_bytes(const Params& p, int R1) {
+  const int N = p.nn[2];
+  const int cs = zg_col_stride(p.n[2], p.order, N, R1);
+  const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
+  const size_t B = (size_t)kZpRecs * sizeof(float4);
+  const size_t tw = (size_t)N * sizeof(float2);          // W_N^m
+  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  return A + B + tw + seg;
+}
+
+// The plan guarantees R2 <= 64 (a stage-0 pass holds a whole line) and
+// R2 >= P (the z halo folds onto butterfly inputs with r = 0 only).
+template <int P, int H>
+__global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
+  constexpr int R1 = 2 * H, YS = R1 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, f = blockIdx.y;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], N = p.nn[2], R2 = N / R1;
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int E = p.E, kz = p.kz;
+  const int cs = zg_col_stride(nz, P, N, R1);
+  const int LY = 2 * cs;                               // line stride (float2)
+  const size_t A_b = (size_t)ncol * cs * sizeof(unsigned long long);
+  const size_t B_b = (size_t)kZpRecs * sizeof(float4);
+  unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
+  float2* ybuf = reinterpret_cast<float2*>(smem);
+  float4* recs = reinterpret_cast<float4*>(smem + A_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + B_b);
+  int* s_start = reinterpret_cast<int*>(tw + N);
+  int* s_cnt = s_start + 9 * E;
+  int* s_pref = s_cnt + 9 * E;
+
+  for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
+    const int e = t / 9, sg = t - e * 9;
+    const int4 sx = b.srcx[bx], sy = b.srcy[by];
+    const int ix = sg % 3, iy = sg / 3;
+    int st = 0, cnt = 0;
+    if (ix < sx.x && iy < sy.x) {
+      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
+      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
+      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[t] = st;
+    s_cnt[t] = cnt;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[i];
+  for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
+    reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[e * 10 + k] = acc;
+      acc += s_cnt[e * 9 + k];
+    }
+    s_pref[e * 10 + 9] = acc;
+  }
+  __syncthreads();
+  auto fetch = [&](int e, int a) -> float4 {
+    const int* pref = s_pref + e * 10;
+    int sgm = 0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
+    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+  };
+  {
+    const int tot0 = min(s_pref[9], kZpRecs);
+#pragma unroll
+    for (int u = 0; u < kZpPre; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot0) recs[a] = fetch(0, a);
+    }
+  }
+  const float inv_ty = 1.0f / (float)p.ty;
+  const float inv_r2 = 1.0f / (float)R2;
+  const int g0 = 64 / R2;                            // lines per stage-0 pass
+  const int lpw = (nl + kWG / 64 - 1) / (kWG / 64);  // lines per wavefront
+  const int lw0 = min(nl, (int)(threadIdx.x >> 6) * lpw), lw1 = min(nl, lw0 + lpw);
+  // stage 1: nq output pairs of this wavefront, each summed by split lanes
+  const int nq = (lw1 - lw0) * kz;
+  int lgs = 0;
+  while (lgs < 3 && (nq << (lgs + 1)) <= 64) ++lgs;
+  const int split = 1 << lgs;
+  const float inv_kz = 1.0f / (float)kz;
+  float4 pre[kZpPre];
+  for (int e = 0; e < E; ++e) {
+    const int total = s_pref[e * 10 + 9];
+    __syncthreads();                                 // records staged, columns clear
+    spread_cols<P, kSpreadLanes>(recs, min(total, kZpRecs), rho, p, X0, Y0, wx, wy, cs);
+    for (int base = kZpRecs; base < total; base += kZpRecs) {
+      const int chunk = min(kZpRecs, total - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
+      __syncthreads();
+      spread_cols<P, kSpreadLanes>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
+    }
+    const int tot_next = min((e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0, kZpRecs);
+#pragma unroll
+    for (int u = 0; u < kZpPre; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) pre[u] = fetch(e + 1, a);
+    }
+    __syncthreads();                                 // columns complete
+
+    const size_t fe = (size_t)f * E + e;
+    float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
+    const unsigned long long pad = pad_units(b, p, fe, e);
+    const int lane = threadIdx.x & 63;
+    // stage 0, a pass of g0 whole lines: lane -> (line, butterfly j)
+    {
+      const int li = div_small(lane, R2, inv_r2), j = lane - li * R2;
+      for (int lg = lw0; lg < lw1; lg += g0) {
+        const int l = lg + li;
+        if (li < g0 && l < lw1) {
+          const int c0 = 2 * l;
+          const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+          const unsigned long long pad0 = (lx < wx && ly < wy) ? pad : 0ull;
+          const unsigned long long pad1 = (lx < wx && ly + 1 < wy) ? pad : 0ull;
+          const unsigned long long* col0 = rho + c0 * cs;
+          const unsigned long long* col1 = col0 + cs;
+          unsigned long long a0[H], a1[H];
+          unsigned big = (unsigned)(pad >> 31);
+#pragma unroll
+          for (int r = 0; r < H; ++r) {
+            const int z = j + R2 * r;
+            if (z < nz) {
+              a0[r] = col0[z];
+              a1[r] = col1[z];
+              if (r == 0 && z < P - 1) {             // fold the z halo back
+                a0[r] += col0[nz + z];
+                a1[r] += col1[nz + z];
+              }
+            } else {
+              a0[r] = pad0;
+              a1[r] = pad1;
+            }
+            big |= (unsigned)((a0[r] | a1[r]) >> 31);
+          }
+          float2 xe[H], xo[H];
+          if (big == 0u) {
+            // every value and the pad below 2^31 units: exact 32-bit difference
+#pragma unroll
+            for (int r = 0; r < H; ++r)
+              xe[r] = make_float2((float)(int)((unsigned)a0[r] - (unsigned)pad0),
+                                  (float)(int)((unsigned)a1[r] - (unsigned)pad1));
+          } else {
+#pragma unroll
+            for (int r = 0; r < H; ++r)
+              xe[r] = make_float2(fixed_minus_units(a0[r], pad0), fixed_minus_units(a1[r], pad1));
+          }
+          xo[0] = xe[0];
+#pragma unroll
+          for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);   // W_R1^r = W_N^{r R2}
+          Radix<H>::dft(xe);
+          Radix<H>::dft(xo);
+          __builtin_amdgcn_wave_barrier();
+          float2* dst = ybuf + l * LY + YS * j;
+#pragma unroll
+          for (int t = 0; t < H; ++t) {
+            dst[2 * t] = xe[t];
+            dst[2 * t + 1] = xo[t];
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    // stage 1 over the wavefront's lines at once: lane -> (line, output pair
+    // k, part of the sum over j), the parts added with lane shuffles; then
+    // the two real columns of each line, X_a(k) = (Z(k) + conj Z(N-k))/2 and
+    // X_b(k) = (Z(k) - conj Z(N-k))/(2i); then clear the lines
+    for (int q0 = 0; q0 < nq; q0 += 64 >> lgs) {
+      const int q = q0 + (lane >> lgs), part = lane & (split - 1);
+      float2 Z = make_float2(0.0f, 0.0f), Zm = make_float2(0.0f, 0.0f);
+      int l = 0, k = 0;
+      if (q < nq) {
+        l = div_small(q, kz, inv_kz);
+        k = q - l * kz;
+        l += lw0;
+        const int s = k % R1, sm = s == 0 ? 0 : R1 - s;
+        int idx = (part * k) % N;                    // j k mod N
+        const int step = (split * k) % N;
+        const float2* yl = ybuf + l * LY;
+        for (int jj = part; jj < R2; jj += split) {
+          const float2 w = tw[idx];
+          idx += step;
+          if (idx >= N) idx -= N;
+          const float2 y1 = yl[YS * jj + s], y2 = yl[YS * jj + sm];
+          Z.x += y1.x * w.x - y1.y * w.y;
+          Z.y += y1.x * w.y + y1.y * w.x;
+          Zm.x += y2.x * w.x + y2.y * w.y;
+          Zm.y += y2.y * w.x - y2.x * w.y;
+        }
+      }
+      for (int m = 1; m < split; m <<= 1) {
+        Z.x += __shfl_xor(Z.x, m, 64);
+        Z.y += __shfl_xor(Z.y, m, 64);
+        Zm.x += __shfl_xor(Zm.x, m, 64);
+        Zm.y += __shfl_xor(Zm.y, m, 64);
+      }
+      if (part == 0 && q < nq) {
+        const int c0 = 2 * l;
+        const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+        if (lx < wx && ly < wy) {
+          float2* dst = zt + (lx * p.n[1] + ly) * kz;
+          constexpr float h = 0.5f / 4294967296.0f; // 1/2 and the 2^-32 unit
+          dst[k] = make_float2(h * (Z.x + Zm.x), h * (Z.y - Zm.y));
+          if (ly + 1 < wy) dst[kz + k] = make_float2(h * (Z.y + Zm.y), -h * (Z.x - Zm.x));
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    {
+      uint4* lines = reinterpret_cast<uint4*>(ybuf + lw0 * LY);   // LY / 2 uint4 a line
+      for (int i = lane; i < (lw1 - lw0) * (LY / 2); i += 64) lines[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kZpPre; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < tot_next) recs[a] = pre[u];
+    }
+  }
+}
+
 __host__ __device__ inline size_t yfft_lds_bytes(const Params& p) {
   return (size_t)p.kcy * p.lsy * sizeof(float2) + (size_t)p.nn[1] * sizeof(float2);
 }

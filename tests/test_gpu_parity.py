@@ -304,3 +304,36 @@ def test_wide_radix_factorisation(gpu, n, sigma, qcut):
     compare(H, C, Ho, Co, label=f"wide radix n={n}")
     Hp, Cp = plain_hist([pos], syms, box, n, sigma, 0.0, qcut, flags=4)
     compare(Hp, Cp, Ho, Co, label=f"plain n={n}")
+
+
+@pytest.mark.parametrize("n,sigma,qcut,order", [(35, 2.0, 100.0, 4),   # N 70 = 14 x 5, all outputs
+                                                (45, 2.0, 1.0, 4),     # 90 = 18 x 5
+                                                (50, 2.0, 0.8, 4),     # 100 = 20 x 5
+                                                (70, 2.0, 0.5, 4),     # 140 = 20 x 7
+                                                (81, 2.0, 0.5, 4),     # 162 = 18 x 9
+                                                (30, 2.0, 100.0, 6),   # 60 = 10 x 6, order 6
+                                                (48, 2.0, 0.9, 6),     # 96 = 16 x 6, order 6
+                                                (28, 2.5, 100.0, 4),   # 70, n_z < N/2
+                                                (40, 2.5, 0.7, 4)])    # 100, n_z < N/2
+def test_generic_pruned_z_against_oracle(gpu, n, sigma, qcut, order):
+    """The output-pruned z transform for N = 2H R2 (H = 5..10): half-input
+    first stage of radix 2H, kept outputs as direct sums over R2."""
+    L = 0.75 * n
+    box = np.diag([L] * 3)
+    pos, syms = O.synthetic_system(max(500, int(0.1 * L ** 3)), L, seed=n + 70)
+    H, C = engine_hist([pos], syms, box, n, sigma, 0.0, qcut, order=order)
+    Ho, Co = oracle_hist([pos], syms, box, n, sigma, 0.0, qcut, order=order)
+    compare(H, C, Ho, Co, label=f"generic pruned z n={n} sigma={sigma} order={order}")
+
+
+@pytest.mark.parametrize("n,tile", [(189, (4, 8)), (105, (4, 4)), (144, (2, 8))])
+def test_generic_pruned_z_matches_full_transform(gpu, n, tile):
+    """Grids of the uneven-systems workload (N = 378 = 18 x 21, 210 = 14 x 15,
+    288 = 18 x 16): the generic pruned z kernel against the full Stockham z
+    transform (flag 2) at 0.1 atoms/A^3."""
+    L = 0.75 * n
+    box = np.diag([L] * 3)
+    pos, syms = O.synthetic_system(int(0.1 * L ** 3), L, seed=n + 90)
+    H1, C1 = engine_hist([pos], syms, box, n, 2.0, 0.0, 0.5, tile=tile)
+    H2, C2 = engine_hist([pos], syms, box, n, 2.0, 0.0, 0.5, tile=tile, flags=2)
+    compare(H1, C1, H2, C2, tol=2e-6, label=f"generic pruned vs full, n={n}")
