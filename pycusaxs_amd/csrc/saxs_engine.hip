@@ -167,15 +167,33 @@ bool zp_applies(bool spread_all, bool pruned_z, const saxs::Params& p) {
 // Half-length H of the first stage of the generic output-pruned spreading
 // kernel (k_spread_zg, N = 2H R2), or 0 where it does not apply: it needs
 // n_z <= N/2, an even R1 = 2H dividing N with order <= R2 <= 64 (a
-// wavefront transforms whole lines), and the LDS.  The widest first stage
-// is taken: stage 1 then sums over the fewest terms.
+// wavefront transforms whole lines), and the LDS.  H = 8, 7, 6, 5 come
+// before 9 and 10: the length-9 and length-10 register DFTs need about 40
+// more VGPRs, which costs a wave per SIMD and more time than the shorter
+// stage-1 sums save.
 int zg_half(bool spread_all, bool pruned_z, const saxs::Params& p) {
   const int Nz = p.nn[2];
   if (!spread_all || !pruned_z || 2 * p.n[2] > Nz) return 0;
-  for (int H : {10, 9, 8, 7, 6, 5}) {
+  for (int H : {8, 7, 6, 5, 9, 10}) {
     const int R1 = 2 * H;
     if (Nz % R1 != 0 || Nz / R1 < p.order || Nz / R1 > 64) continue;
     if (saxs::spread_g_lds_bytes(p, R1) > kLdsLimit) continue;
+    return H;
+  }
+  return 0;
+}
+
+// Half-length H of the first stage of the generic output-pruned y transform
+// (k_yfft_g, N = 2H R2), or 0 where it does not apply: it needs n_y <= N/2,
+// few kept outputs (Ky < N/8), an even R1 = 2H dividing N with R2 <= 64,
+// and the LDS (the same order of preference as zg_half).
+int yg_half(bool pruned_z, const saxs::Params& p) {
+  const int Ny = p.nn[1];
+  if (!pruned_z || 2 * p.n[1] > Ny || p.ky >= Ny || p.Ky >= Ny / 8) return 0;
+  for (int H : {8, 7, 6, 5, 9, 10}) {
+    const int R1 = 2 * H;
+    if (Ny % R1 != 0 || Ny / R1 > 64) continue;
+    if (saxs::yfft_g_lds_bytes(Ny, R1) > kLdsLimit) continue;
     return H;
   }
   return 0;
@@ -301,6 +319,7 @@ struct saxs_plan {
   int* segoff = nullptr;       // [(512 + max_frames) * keys per frame] segmented sort
   bool seg_sort = false;       // one frame's keys fit the LDS histogram of k_prep_seg
   bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
+  int x_gen_h = 0;             // forward x pass by k_xfft_g<x_gen_h> (0: Stockham)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
 };
 
@@ -350,6 +369,19 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
       pl->x_pruned = true;
       p.nchx = (p.kz + kc - 1) / kc;
       p.kcx = (p.kz + p.nchx - 1) / p.nchx;
+    }
+  }
+  // generic pruned x pass (forward only; the adjoint keeps the Stockham
+  // transform over the same z chunks): n_x <= N/2, few kept m_x, an even
+  // R1 = 2H dividing N, every kept pair of a chunk in kXgItems per thread
+  pl->x_gen_h = 0;
+  if (pl->pruned_z && !pl->x_pruned && 2 * p.n[0] <= Nx && p.kx < Nx && p.Kx < Nx / 8 &&
+      p.kcx * (p.Kx + 1) <= saxs::kXgItems * saxs::kWG) {
+    for (int H : {8, 7, 6, 5, 9, 10}) {
+      if (Nx % (2 * H) != 0 || Nx / (2 * H) > 64) continue;
+      if (saxs::xfft_g_lds_bytes(p, 2 * H) > kLdsLimit) continue;
+      pl->x_gen_h = H;
+      break;
     }
   }
   if (saxs::yfft_lds_bytes(p) > kLdsLimit || (!pl->x_pruned && saxs::xfft_lds_bytes(p) > kLdsLimit))
@@ -630,6 +662,20 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
 #undef SAXS_YP
+  } else if (const int yh = yg_half(pl->pruned_z, p)) {
+    const int R2 = Ny / (2 * yh), LW = saxs::kWG / R2;
+    const dim3 gyg((p.n[0] * p.kz + LW - 1) / LW, nfe);
+    const size_t lq = saxs::yfft_g_lds_bytes(Ny, 2 * yh);
+#define SAXS_YG(HH) hipLaunchKernelGGL(saxs::k_yfft_g<HH>, gyg, dim3(saxs::kWG), lq, st, p, b)
+    switch (yh) {
+      case 10: SAXS_YG(10); break;
+      case 9: SAXS_YG(9); break;
+      case 8: SAXS_YG(8); break;
+      case 7: SAXS_YG(7); break;
+      case 6: SAXS_YG(6); break;
+      default: SAXS_YG(5); break;
+    }
+#undef SAXS_YG
   } else {
     const int rsy = saxs::fft_radix_set(p.fy);
 #define SAXS_Y(RS) hipLaunchKernelGGL(saxs::k_yfft<RS>, dim3(p.n[0], p.nchy, nfe), dim3(saxs::kWG), saxs::yfft_lds_bytes(p), st, p, b)
@@ -653,6 +699,19 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Nx == 128) SAXS_XP(128); else if (Nx == 256) SAXS_XP(256); else SAXS_XP(512);
 #undef SAXS_XP
+  } else if (pl->x_gen_h) {
+    const dim3 gx(p.ky, p.nchx, n_frames);
+    const size_t lq = saxs::xfft_g_lds_bytes(p, 2 * pl->x_gen_h);
+#define SAXS_XG(HH) hipLaunchKernelGGL(saxs::k_xfft_g<HH>, gx, dim3(saxs::kWG), lq, st, p, b)
+    switch (pl->x_gen_h) {
+      case 10: SAXS_XG(10); break;
+      case 9: SAXS_XG(9); break;
+      case 8: SAXS_XG(8); break;
+      case 7: SAXS_XG(7); break;
+      case 6: SAXS_XG(6); break;
+      default: SAXS_XG(5); break;
+    }
+#undef SAXS_XG
   } else {
     const int rsx = saxs::fft_radix_set(p.fx);
 #define SAXS_X(RS) hipLaunchKernelGGL(saxs::k_xfft_epi<RS>, dim3(p.ky, p.nchx, n_frames), dim3(saxs::kWG), saxs::xfft_lds_bytes(p), st, p, b)

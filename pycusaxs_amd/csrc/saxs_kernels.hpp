@@ -1419,6 +1419,116 @@ __host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
          (size_t)p.nn[0] * sizeof(float2);
 }
 
+// ---- output-pruned y transform for any N = 2H R2 with n_y <= N/2 ---------
+//
+// The z-transformed rows of zbuf are transformed along y keeping only m_y in
+// [0, Ky] and (N - Ky, N).  Stage 0 as in k_spread_zg: butterfly j < R2 of
+// line (x, kz) takes the inputs y = j + R2 r, r < H (the rest is zero
+// padding), as two length-H DFTs.  Stage 1 as direct sums over j, the pair
+// m, N - m sharing its twiddles:
+//   Y(m) = sum_j T_j[m mod R1] W_N^{jm},  Y(N-m) = sum_j T_j[-m mod R1] conj(W_N^{jm}).
+// A workgroup takes LW = kWG / R2 lines with consecutive kz, so loads and
+// stores are coalesced along kz.  The plan guarantees 2 Ky + 1 < N.
+
+__host__ __device__ inline int yg_line_stride(int R1, int R2) { return (R2 * (R1 + 1)) | 1; }
+
+__host__ __device__ inline size_t yfft_g_lds_bytes(int N, int R1) {
+  const int R2 = N / R1;
+  return (size_t)(kWG / R2) * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H>
+__global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
+  constexpr int R1 = 2 * H, YS = R1 + 1;
+  const int N = p.nn[1], R2 = N / R1;
+  const int LW = kWG / R2, LY = yg_line_stride(R1, R2);
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tbuf = reinterpret_cast<float2*>(smem);
+  float2* tw = tbuf + (size_t)LW * LY;
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
+  const int g0 = (int)(grp % gridDim.x) * LW;
+  const float inv_lw = 1.0f / (float)LW, inv_kz = 1.0f / (float)kz;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[i];
+  const float2* src = b.zbuf + fe * (size_t)nx * ny * kz;
+  // stage 0 (item = butterfly j, line l; l fastest)
+  const bool act0 = (int)threadIdx.x < LW * R2;
+  const int j0 = div_small(threadIdx.x, LW, inv_lw), l0 = threadIdx.x - j0 * LW;
+  float2 xe[H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) xe[r] = make_float2(0.0f, 0.0f);
+  if (act0 && g0 + l0 < nlines) {
+    const int g = g0 + l0;
+    const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+    const float2* col = src + (size_t)x * ny * kz + k;
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const int y = j0 + R2 * r;
+      if (y < ny) xe[r] = col[(size_t)y * kz];
+    }
+  }
+  __syncthreads();                                   // twiddles
+  if (act0) {
+    float2 xo[H];
+    xo[0] = xe[0];
+#pragma unroll
+    for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);   // W_R1^r = W_N^{r R2}
+    Radix<H>::dft(xe);
+    Radix<H>::dft(xo);
+    float2* dst = tbuf + l0 * LY + YS * j0;
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+      dst[2 * t] = xe[t];
+      dst[2 * t + 1] = xo[t];
+    }
+  }
+  __syncthreads();                                   // lines complete
+  // stage 1 (item = output pair m <= Ky, line l, part of the sum over j)
+  const int Ky = p.Ky, ky = p.ky;
+  const int nq = LW * (Ky + 1);
+  int lgs = 0;
+  while (lgs < 3 && (nq << (lgs + 1)) <= kWG) ++lgs;
+  const int split = 1 << lgs;
+  for (int t0 = 0; t0 < (nq << lgs); t0 += kWG) {
+    const int t = t0 + threadIdx.x;
+    const int part = t & (split - 1), qq = t >> lgs;
+    const int m = div_small(qq, LW, inv_lw), l = qq - m * LW;
+    const bool act = qq < nq && g0 + l < nlines;
+    float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+    if (act) {
+      const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
+      int idx = (part * m) % N;                      // j m mod N
+      const int step = (split * m) % N;
+      const float2* yl = tbuf + l * LY;
+      for (int jj = part; jj < R2; jj += split) {
+        const float2 w = tw[idx];
+        idx += step;
+        if (idx >= N) idx -= N;
+        const float2 y1 = yl[YS * jj + s], y2 = yl[YS * jj + sm];
+        A.x += y1.x * w.x - y1.y * w.y;
+        A.y += y1.x * w.y + y1.y * w.x;
+        B.x += y2.x * w.x + y2.y * w.y;
+        B.y += y2.y * w.x - y2.x * w.y;
+      }
+    }
+    for (int sh = 1; sh < split; sh <<= 1) {
+      A.x += __shfl_xor(A.x, sh, 64);
+      A.y += __shfl_xor(A.y, sh, 64);
+      B.x += __shfl_xor(B.x, sh, 64);
+      B.y += __shfl_xor(B.y, sh, 64);
+    }
+    if (act && part == 0) {
+      const int g = g0 + l;
+      const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+      float2* out = b.ybuf + (fe * ky * nx + x) * kz + k;   // + jy nx kz
+      out[(size_t)m * nx * kz] = A;
+      if (m > 0) out[(size_t)(ky - m) * nx * kz] = B;
+    }
+  }
+}
+
 template <int POW2>
 __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1622,6 +1732,144 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
               atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
               atomicAdd(&Cs[bin], origin ? 1u : 2u);
             }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = b.partial + (((size_t)f * p.ky + jy) * p.nchx + ch) * 2 * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    out[i] = Hs[i];
+    out[nb + i] = (double)Cs[i];
+  }
+}
+
+// ---- output-pruned x transform for any N = 2H R2 with n_x <= N/2, fused
+// with the form-factor sum, moduli and histogram ----------------------------
+//
+// Same result and accounting as k_xfft_epi over the same (jy, z chunk,
+// frame) workgroups: stage 0 (butterfly j < R2 of line k: inputs x = j + R2 r,
+// r < H, two length-H DFTs) per element, then stage 1 only for the kept
+// pairs m, N - m (m <= Kx) as direct sums over j, each thread accumulating
+// A = sum_e f_e(q) F_e for its pairs in registers; then |A|^2 x moduli into
+// the LDS histogram (2 I, or I at the origin).  The plan guarantees
+// Kx < N/8, kx < N and kcx (Kx + 1) <= kXgItems kWG.
+
+constexpr int kXgItems = 4;          // stage-1 output pairs per thread
+
+__host__ __device__ inline size_t xfft_g_lds_bytes(const Params& p, int R1) {
+  const int N = p.nn[0], R2 = N / R1;
+  return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) +
+         (size_t)p.kcx * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H>
+__global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
+  constexpr int R1 = 2 * H, YS = R1 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], N = p.nn[0], nb = p.num_bins, Kx = p.Kx;
+  const int R2 = N / R1, LY = yg_line_stride(R1, R2);
+  double* Hs = reinterpret_cast<double*>(smem);
+  unsigned int* Cs = reinterpret_cast<unsigned int*>(Hs + nb);
+  float2* lines = reinterpret_cast<float2*>(smem + (((size_t)nb * 12 + 15) & ~size_t(15)));
+  float2* tw = lines + (size_t)p.kcx * LY;           // W_N^m
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    Hs[i] = 0.0;
+    Cs[i] = 0u;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = (p.ky == p.nn[1]) ? jy : (jy <= p.Ky ? jy : p.nn[1] - (p.ky - jy));
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const float inv_kc = 1.0f / (float)kc;
+  const int nitems = kc * (Kx + 1);                  // (pair m, line k), k fastest
+  float2 A[kXgItems][2];
+#pragma unroll
+  for (int u = 0; u < kXgItems; ++u) A[u][0] = A[u][1] = make_float2(0.0f, 0.0f);
+  __syncthreads();                                   // twiddles
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = div_small(item, kc, inv_kc), k = item - j * kc;
+      float2 xe[H], xo[H];
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int x = j + R2 * r;
+        xe[r] = x < nx ? src[(size_t)x * kz + k] : make_float2(0.0f, 0.0f);
+      }
+      xo[0] = xe[0];
+#pragma unroll
+      for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);   // W_R1^r = W_N^{r R2}
+      Radix<H>::dft(xe);
+      Radix<H>::dft(xo);
+      float2* dst = lines + k * LY + YS * j;
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        dst[2 * t] = xe[t];
+        dst[2 * t + 1] = xo[t];
+      }
+    }
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXgItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < nitems) {
+        const int m = div_small(item, kc, inv_kc), k = item - m * kc;
+        const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
+        const float2* yl = lines + k * LY;
+        float2 Z = make_float2(0.0f, 0.0f), Zm = make_float2(0.0f, 0.0f);
+        int idx = 0;                                 // j m mod N
+        for (int j = 0; j < R2; ++j) {
+          const float2 w = tw[idx];
+          idx += m;
+          if (idx >= N) idx -= N;
+          const float2 y1 = yl[YS * j + s], y2 = yl[YS * j + sm];
+          Z.x += y1.x * w.x - y1.y * w.y;
+          Z.y += y1.x * w.y + y1.y * w.x;
+          Zm.x += y2.x * w.x + y2.y * w.y;
+          Zm.y += y2.y * w.x - y2.x * w.y;
+        }
+        const int mz = k0 + k;
+        const int n0 = shell_key(m, my, mz, p);
+        if (n0 < p.lut_len) {
+          const float fv = ff[n0];
+          A[u][0].x += fv * Z.x;
+          A[u][0].y += fv * Z.y;
+        }
+        if (m > 0) {
+          const int n1 = shell_key(-m, my, mz, p);
+          if (n1 < p.lut_len) {
+            const float fv = ff[n1];
+            A[u][1].x += fv * Zm.x;
+            A[u][1].y += fv * Zm.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kXgItems; ++u) {
+    const int item = threadIdx.x + u * blockDim.x;
+    if (item < nitems) {
+      const int m = div_small(item, kc, inv_kc), k = item - m * kc;
+      const int mz = k0 + k;
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        if (o == 1 && m == 0) continue;
+        const int ix = o == 0 ? m : N - m, mx = o == 0 ? m : -m;
+        const int nsq = shell_key(mx, my, mz, p);
+        if (nsq < p.lut_len) {
+          const int bin = b.lut_bin[nsq];
+          if (bin >= 0 && bin < nb) {
+            const float mod = b.modx[ix] * b.mody[iy] * b.modz[mz];
+            const float I = (A[u][o].x * A[u][o].x + A[u][o].y * A[u][o].y) * mod;
+            const bool origin = (ix == 0) && (iy == 0) && (mz == 0);
+            atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
+            atomicAdd(&Cs[bin], origin ? 1u : 2u);
           }
         }
       }

@@ -337,3 +337,37 @@ def test_generic_pruned_z_matches_full_transform(gpu, n, tile):
     H1, C1 = engine_hist([pos], syms, box, n, 2.0, 0.0, 0.5, tile=tile)
     H2, C2 = engine_hist([pos], syms, box, n, 2.0, 0.0, 0.5, tile=tile, flags=2)
     compare(H1, C1, H2, C2, tol=2e-6, label=f"generic pruned vs full, n={n}")
+
+
+@pytest.mark.parametrize("n,qcut", [((48, 70, 64), 0.6),     # y: N 140 = 20 x 7; z 128 radix 16
+                                    ((40, 81, 45), 0.5),     # y: 162 = 18 x 9; z: 90 = 18 x 5
+                                    ((36, 105, 50), 0.45)])  # y: 210 = 14 x 15
+def test_generic_pruned_y_against_oracle(gpu, n, qcut):
+    """The output-pruned y transform for N = 2H R2 on non-cubic cells (each
+    axis its own transform length), against the oracle."""
+    lengths = [0.75 * v for v in n]
+    box = np.diag(lengths)
+    natoms = int(0.1 * np.prod(lengths))
+    rng = np.random.default_rng(sum(n))
+    pos = (rng.random((natoms, 3)) * np.array(lengths)).astype(np.float32)
+    syms = list(rng.choice(["C", "N", "O", "H", "S"], size=natoms, p=[0.3, 0.04, 0.15, 0.5, 0.01]))
+    H, C = engine_hist([pos], syms, box, n, 2.0, 0.0, qcut)
+    Ho, Co = oracle_hist([pos], syms, box, n, 2.0, 0.0, qcut)
+    compare(H, C, Ho, Co, label=f"generic pruned y n={n}")
+
+
+@pytest.mark.parametrize("n,qcut", [((70, 64, 48), 0.6),     # x: N 140 = 14 x 10
+                                    ((81, 40, 45), 0.5),     # x: 162 = 18 x 9
+                                    ((105, 36, 50), 0.45)])  # x: 210 = 14 x 15
+def test_generic_pruned_x_against_oracle(gpu, n, qcut):
+    """The output-pruned x transform fused with the form-factor sum, moduli
+    and histogram, for N = 2H R2 on non-cubic cells, against the oracle."""
+    lengths = [0.75 * v for v in n]
+    box = np.diag(lengths)
+    natoms = int(0.1 * np.prod(lengths))
+    rng = np.random.default_rng(sum(n) + 1)
+    pos = (rng.random((natoms, 3)) * np.array(lengths)).astype(np.float32)
+    syms = list(rng.choice(["C", "N", "O", "H", "S"], size=natoms, p=[0.3, 0.04, 0.15, 0.5, 0.01]))
+    H, C = engine_hist([pos], syms, box, n, 2.0, 0.0, qcut)
+    Ho, Co = oracle_hist([pos], syms, box, n, 2.0, 0.0, qcut)
+    compare(H, C, Ho, Co, label=f"generic pruned x n={n}")
