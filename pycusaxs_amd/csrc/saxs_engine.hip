@@ -541,7 +541,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
   const bool seg = pl->seg_sort && n_atoms > 0;
   if (seg) {  // ~512 workgroups, >= 4096 atoms per segment, <= 65536 (key, segment) runs
-             // per frame for k_scan_seg; nseg * F <= 512 + F - 1
+             // per frame; nseg * F <= 512 + F - 1
     b.nseg = std::max(1, std::min({(512 + n_frames - 1) / n_frames, (n_atoms + 4095) / 4096,
                                    65536 / kframe}));
     b.seglen = (n_atoms + b.nseg - 1) / b.nseg;
@@ -568,10 +568,10 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   }
   { Probe pr(pl, SAXS_K_SCAN, st);
   if (seg)
-    hipLaunchKernelGGL(saxs::k_scan_seg, dim3(n_frames), dim3(1024), 0, st, p, b);
-  else
-    hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
-                       kframe, n_atoms); }
+    hipLaunchKernelGGL(saxs::k_seg_keys, dim3((kframe + saxs::kWG - 1) / saxs::kWG, n_frames),
+                       dim3(saxs::kWG), 0, st, p, b);
+  hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
+                     kframe, n_atoms); }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_SCATTER, st);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);

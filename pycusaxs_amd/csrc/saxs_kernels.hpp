@@ -4,6 +4,7 @@
 //   k_clear         zero bin counters and pad sums
 //   k_prep<P>       per atom: grid cell and fraction (saxsDeviceKernels.cu:352-368),
 //                   column-tile key, slot in its bin, border-slab weight sum
+//   k_seg_keys      per key: segment starts within its run and its total
 //   k_scan          exclusive scan of the per-(frame, element, tile) counts
 //   k_scatter<P>    counting-sort the atoms by (frame, element, tile)
 //   k_spread_z<P>   per (frame, element, column tile): gather-spread the atoms
@@ -296,9 +297,9 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
 // k_prep_seg: workgroup (segment s, frame f) takes atoms [s L, (s+1) L) and
 // counts them per key in an LDS histogram (the slot within (key, segment)
 // is the LDS atomic's old value); the histogram goes to segoff[f][s][k].
-// k_scan_seg turns those counts into the starting position of each
-// (key, segment) run and writes the per-key counts and offsets the spread
-// kernels read.  No global atomics on the counters.
+// k_seg_keys turns those counts into each (key, segment) run's start within
+// the key and writes the per-key counts; k_scan then gives the per-key
+// offsets the spread kernels read.  No global atomics on the counters.
 constexpr int kSortWG = 1024;
 
 __host__ __device__ inline size_t prep_seg_lds_bytes(int K, int E) {
@@ -339,64 +340,67 @@ __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
   }
 }
 
-// Per frame (blockIdx.x): key totals over segments, exclusive scan over
-// keys from f N, then each (key, segment) run's start in place of its count.
-__global__ void __launch_bounds__(1024) k_scan_seg(Params p, Buffers b) {
-  __shared__ int sh[1024];
-  const int f = blockIdx.x, t = threadIdx.x;
-  const int K = p.E * p.nty * p.ntx, S = b.nseg;
-  int* seg = b.segoff + (size_t)f * S * K;
-  const int chunk = (K + 1023) / 1024;
-  const int beg = min(K, t * chunk), end = min(K, beg + chunk);
-  int sum = 0;
-  for (int k = beg; k < end; ++k)
-    for (int q = 0; q < S; ++q) sum += seg[(size_t)q * K + k];
-  sh[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
+// Per key of frame blockIdx.y: the exclusive prefix of its per-segment
+// counts (in place: segoff then holds each segment's start within the key's
+// run) and the key's total.  One thread per key, loads and stores coalesced
+// along the keys.
+__global__ void __launch_bounds__(kWG) k_seg_keys(Params p, Buffers b) {
+  const int K = p.E * p.nty * p.ntx;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
+  if (k >= K) return;
+  int* seg = b.segoff + (size_t)f * b.nseg * K + k;
+  int run = 0;
+  for (int q = 0; q < b.nseg; ++q) {
+    const int c = seg[(size_t)q * K];
+    seg[(size_t)q * K] = run;
+    run += c;
   }
-  int run = f * p.N + sh[t] - sum;
-  for (int k = beg; k < end; ++k) {
-    b.offsets[(size_t)f * K + k] = run;
-    int tot = 0;
-    for (int q = 0; q < S; ++q) {
-      const int c = seg[(size_t)q * K + k];
-      seg[(size_t)q * K + k] = run + tot;
-      tot += c;
-    }
-    b.counts[(size_t)f * K + k] = tot;
-    run += tot;
-  }
+  b.counts[(size_t)f * K + k] = run;
 }
 
-// Exclusive scan of one frame's counts (blockIdx.x = frame, K keys per
-// frame) into offsets, starting at frame * N: every frame owns the sorted
-// range [f*N, (f+1)*N).
+// Exclusive scan of the key counts of frame blockIdx.x from f N (the
+// frame's first slot of the sorted array): one workgroup of 1024 threads,
+// keys in tiles of 1024, one key per thread (coalesced); a round's loads are
+// all issued before its tiles are scanned (wavefront shuffles, then the
+// sixteen wavefront totals through LDS), with a running carry.
+constexpr int kScanTiles = 16;
+
 __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int* offsets, int K,
                                                int N) {
-  __shared__ int s[1024];
-  const int t = threadIdx.x;
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const size_t off0 = (size_t)blockIdx.x * K;
-  const int chunk = (K + 1023) / 1024;
-  const int beg = min(K, t * chunk), end = min(K, beg + chunk);
-  int sum = 0;
-  for (int i = beg; i < end; ++i) sum += counts[off0 + i];
-  s[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  int run = (int)blockIdx.x * N + s[t] - sum;
-  for (int i = beg; i < end; ++i) {
-    offsets[off0 + i] = run;
-    run += counts[off0 + i];
+  int carry = (int)blockIdx.x * N;
+  for (int base = 0; base < K; base += 1024 * kScanTiles) {
+    int v[kScanTiles];
+#pragma unroll
+    for (int i = 0; i < kScanTiles; ++i) {
+      const int k = base + i * 1024 + t;
+      v[i] = k < K ? counts[off0 + k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kScanTiles; ++i) {
+      if (base + i * 1024 >= K) break;               // uniform
+      int x = v[i];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63) wsum[w] = x;
+      __syncthreads();
+      int before = 0, total = 0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int sq = wsum[q];
+        before += q < w ? sq : 0;
+        total += sq;
+      }
+      const int k = base + i * 1024 + t;
+      if (k < K) offsets[off0 + k] = carry + before + x - v[i];
+      carry += total;
+      __syncthreads();                               // wsum is rewritten
+    }
   }
 }
 
@@ -415,10 +419,9 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   rec.y = rr[1];
   rec.z = rr[2];
   rec.w = __int_as_float(i);
-  const int pos = b.nseg > 0
-                      ? b.segoff[((size_t)f * b.nseg + i / b.seglen) * (p.E * p.nty * p.ntx) +
-                                 (ks.x - f * p.E * p.nty * p.ntx)] + ks.y
-                      : b.offsets[ks.x] + ks.y;
+  const int K = p.E * p.nty * p.ntx;
+  int pos = b.offsets[ks.x] + ks.y;
+  if (b.nseg > 0) pos += b.segoff[((size_t)f * b.nseg + i / b.seglen) * K + (ks.x - f * K)];
   b.sorted[pos] = rec;
 }
 
