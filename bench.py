@@ -392,6 +392,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="with --profile-only: also run one elementwise copy of this many MiB "
                          "(known HBM bytes, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6"),
+                    help="run only this secondary workload (for rocprofv3) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this engine's "
@@ -399,6 +401,11 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.leg:
+        legs = {"batched_uneven": leg_batched_uneven, "large_1M_512": leg_large,
+                "restraint_50k_order6": lambda t, n: leg_restraint(t, n, {})}
+        print(json.dumps({"leg": args.leg, **legs[args.leg](torch, max(1, args.steps))}), flush=True)
+        return
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
 
     xyz_h, symbols, ids_h, _ = make_inputs(frames=args.frames)

@@ -83,10 +83,16 @@ typedef struct saxs_plan saxs_plan;
  *               widest register radices first (12, 10, 9, 8, ...; fewer
  *               stages) instead of 8/4/2 then 3, 5, 7.  The autotuner picks
  *               among these per grid.
+ *               SAXS_FLAG_FORWARD_ONLY: the column tile is fitted to the
+ *               forward kernels alone; saxs_backward / saxs_backward_ex on
+ *               such a plan return SAXS_EUNSUP when the z adjoint does not
+ *               fit that tile (no counterpart in the reference, whose plans
+ *               have no backward; for forward-only batches).
  */
 #define SAXS_FLAG_PER_ELEMENT_SPREAD 1
 #define SAXS_FLAG_NO_PRUNED_Z 2
 #define SAXS_FLAG_WIDE_RADIX 4
+#define SAXS_FLAG_FORWARD_ONLY 8
 typedef struct saxs_desc {
   int32_t n[3];
   int32_t nn[3];

@@ -52,6 +52,7 @@ class SaxsDesc(C.Structure):
 FLAG_PER_ELEMENT_SPREAD = 1
 FLAG_NO_PRUNED_Z = 2
 FLAG_WIDE_RADIX = 4
+FLAG_FORWARD_ONLY = 8     # tile fitted to the forward kernels alone (no backward)
 BWD_REUSE_FORWARD = 1
 
 

@@ -14,6 +14,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 import torch
 
+from . import _lib
 from .engine import SaxsEngine, SystemSpec
 
 
@@ -36,9 +37,10 @@ class BatchedSystems:
         self.elem: List[torch.Tensor] = []
         self.shapes = []
         for s in self.systems:
-            tile, flags = (0, 0), 0
+            # the graphs only run forward: tiles fitted to the forward kernels
+            tile, flags = (0, 0), _lib.FLAG_FORWARD_ONLY
             if tuner is not None:
-                tile, flags = tuner.tune(s.spec, s.n_atoms, s.box)
+                tile, flags = tuner.tune(s.spec, s.n_atoms, s.box, forward_only=True)
             eng = SaxsEngine(s.spec, s.box, max_atoms=s.n_atoms, max_frames=s.frames,
                              tile=tile, flags=flags)
             self.engines.append(eng)

@@ -202,7 +202,7 @@ int yg_half(bool pruned_z, const saxs::Params& p) {
 // Whether the spreading kernel and the z adjoint that will run for column
 // tile (p.tx, p.ty) fit: the pruned kernels where they apply, else the
 // all-element or per-element Stockham ones (the launch picks in that order).
-bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len) {
+bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len, bool fwd_only) {
   p.kz = kz_of(p, lut_len);
   const int Nz = p.nn[2], nl = p.tx * p.ty / 2;
   const bool line_ok = nl <= fft_line_cap(p.fz);
@@ -214,7 +214,7 @@ bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len) {
                    (line_ok && saxs::spread_lds_bytes(p) <= kLdsLimit);
   const bool bwd = (pruned && saxs::adj_p_lds_bytes(p) <= kLdsLimit) ||
                    (line_ok && saxs::adjz_lds_bytes(p) <= kLdsLimit);
-  return fwd && bwd;
+  return fwd && (bwd || fwd_only);
 }
 
 // Number of tile offsets (tile - home tile, mod the tile count) that one
@@ -540,10 +540,12 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const int nkeys = n_frames * kframe;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
   const bool seg = pl->seg_sort && n_atoms > 0;
-  if (seg) {  // ~512 workgroups, >= 4096 atoms per segment, <= 65536 (key, segment) runs
-             // per frame; nseg * F <= 512 + F - 1
+  if (seg) {  // ~512 workgroups (nseg * F <= 512 + F - 1, the segoff allocation),
+             // >= 4096 atoms per segment, and at least K/2 atoms per segment so
+             // that the per-segment key histograms (K ints each, written by
+             // k_prep_seg and rewritten by k_seg_keys) stay below the atom data
     b.nseg = std::max(1, std::min({(512 + n_frames - 1) / n_frames, (n_atoms + 4095) / 4096,
-                                   65536 / kframe}));
+                                   std::max(1, 2 * n_atoms / kframe)}));
     b.seglen = (n_atoms + b.nseg - 1) / b.nseg;
   }
   { Probe pr(pl, SAXS_K_CLEAR, st);
@@ -803,7 +805,9 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   for (;;) {
     p.tx = tx;
     p.ty = ty;
-    if (tile_fits(pl->spread_all, pl->pruned_z, p, d->lut_len)) break;
+    if (tile_fits(pl->spread_all, pl->pruned_z, p, d->lut_len,
+                  (d->flags & SAXS_FLAG_FORWARD_ONLY) != 0))
+      break;
     if (tx >= ty && tx > 1)
       tx = (tx + 1) / 2;
     else if (ty > 2)
@@ -998,6 +1002,17 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   int rc = check_call(pl, xyz, elem, n_atoms, n_frames, oc_diag);
   if (rc) return rc;
   if (!gH || (n_atoms > 0 && !grad)) return fail(SAXS_EINVAL, "gH/grad pointer is NULL");
+  {  // a forward-only plan's tile may not fit the z adjoint: refuse before any launch
+    saxs::Params q = pl->p;
+    const int Nz = q.nn[2];
+    const bool pruned = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
+                        2 * q.kz - 1 <= Nz / 4 && q.kz <= Nz / 8 &&
+                        (q.tx * q.ty / 2) * (Nz / 16) <= saxs::kWG &&
+                        saxs::adj_p_lds_bytes(q) <= kLdsLimit;
+    if (!pruned && (q.tx * q.ty / 2 > fft_line_cap(q.fz) || saxs::adjz_lds_bytes(q) > kLdsLimit))
+      return fail(SAXS_EUNSUP, "column tile %dx%d does not fit the z adjoint (plan created with "
+                  "SAXS_FLAG_FORWARD_ONLY)", q.tx, q.ty);
+  }
   hipStream_t st = (hipStream_t)stream;
   if (!pl->gpart && pl->max_atoms > 0) {   // partial-gradient slots, on first use
     rc = dev_alloc(&pl->gpart, (size_t)pl->max_frames * pl->nsx * pl->nsy * pl->max_atoms * 3,

@@ -32,11 +32,11 @@ def _cache_path() -> Path:
     return Path.home() / ".cache" / "pycusaxs_amd" / "tune.json"
 
 
-def shape_key(spec, n_atoms: int, box_volume: float) -> str:
+def shape_key(spec, n_atoms: int, box_volume: float, forward_only: bool = False) -> str:
     density = n_atoms / max(box_volume, 1e-9)
     dbucket = int(round(math.log2(max(density, 1e-6)) * 2))        # half-octaves
     return (f"n{'x'.join(map(str, spec.n))}-nn{'x'.join(map(str, spec.nn))}-p{spec.order}"
-            f"-e{len(spec.symbols)}-q{spec.qcut_req:g}-d{dbucket}")
+            f"-e{len(spec.symbols)}-q{spec.qcut_req:g}-d{dbucket}" + ("-fwd" if forward_only else ""))
 
 
 class Autotuner:
@@ -55,19 +55,22 @@ class Autotuner:
         import torch
         return torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
 
-    def lookup(self, spec, n_atoms: int, box) -> Optional[Tuple[Tuple[int, int], int]]:
+    def lookup(self, spec, n_atoms: int, box,
+               forward_only: bool = False) -> Optional[Tuple[Tuple[int, int], int]]:
         vol = float(np.prod(np.diag(np.asarray(box, dtype=np.float64).reshape(3, 3))))
-        ent = self._cache.get(self._device(), {}).get(shape_key(spec, n_atoms, vol))
+        ent = self._cache.get(self._device(), {}).get(shape_key(spec, n_atoms, vol, forward_only))
         if ent is None:
             return None
         return tuple(ent["tile"]), int(ent["flags"])
 
     def tune(self, spec, n_atoms: int, box, candidates: Optional[Iterable] = None,
-             seed: int = 0) -> Tuple[Tuple[int, int], int]:
-        """Times the candidates (tile, flags) and returns the fastest."""
+             seed: int = 0, forward_only: bool = False) -> Tuple[Tuple[int, int], int]:
+        """Times the candidates (tile, flags) and returns the fastest.  With
+        forward_only the plans are made with FLAG_FORWARD_ONLY (tiles fitted
+        to the forward kernels alone; such plans refuse backward calls)."""
         import torch
         from .engine import SaxsEngine
-        hit = self.lookup(spec, n_atoms, box)
+        hit = self.lookup(spec, n_atoms, box, forward_only)
         if hit is not None:
             return hit
         b = np.diag(np.asarray(box, dtype=np.float64).reshape(3, 3))
@@ -78,6 +81,8 @@ class Autotuner:
                                 .astype(np.int32)).cuda()
         cands = list(candidates) if candidates is not None else \
             [(t, f) for t in TILES for f in VARIANTS]
+        if forward_only:
+            cands = [(t, f | _lib.FLAG_FORWARD_ONLY) for t, f in cands]
         results = []
         for tile, flags in cands:
             try:
@@ -101,7 +106,7 @@ class Autotuner:
         best_ms, tile, flags = results[0]
         vol = float(np.prod(b))
         dev = self._cache.setdefault(self._device(), {})
-        dev[shape_key(spec, n_atoms, vol)] = {
+        dev[shape_key(spec, n_atoms, vol, forward_only)] = {
             "tile": list(tile), "flags": int(flags), "ms_per_frame": best_ms / (self.repeats * self.frames),
             "candidates": [[list(t), int(f), ms / (self.repeats * self.frames)] for ms, t, f in results]}
         try:

@@ -371,3 +371,35 @@ def test_generic_pruned_x_against_oracle(gpu, n, qcut):
     H, C = engine_hist([pos], syms, box, n, 2.0, 0.0, qcut)
     Ho, Co = oracle_hist([pos], syms, box, n, 2.0, 0.0, qcut)
     compare(H, C, Ho, Co, label=f"generic pruned x n={n}")
+
+
+def test_forward_only_plan_tile_and_backward_refusal(gpu):
+    """A forward-only plan (FLAG_FORWARD_ONLY) may take a column tile that
+    the z adjoint does not fit (4 x 8 on a 189^3 cell, 378^3 FFT): its
+    forward equals the default plan's bit for bit (the fixed-point columns
+    do not depend on the tile), and a backward call is refused before any
+    launch with a ValueError."""
+    import torch
+    from pycusaxs_amd import _lib
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    n, L = 189, 141.75
+    box = np.diag([L] * 3)
+    pos, syms = O.synthetic_system(60000, L, seed=189)
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.5)
+    xyz = torch.from_numpy(pos[None])
+    idt = torch.from_numpy(ids)
+    out = []
+    for flags in (0, _lib.FLAG_FORWARD_ONLY):
+        eng = SaxsEngine(spec, box, max_atoms=len(pos), max_frames=1, tile=(4, 8), flags=flags)
+        eng.forward(xyz, idt)
+        torch.cuda.synchronize()
+        out.append((eng.H.cpu().numpy(), eng.C.cpu().numpy()))
+        if flags:
+            gH = torch.zeros((1, eng.hist.num_bins), dtype=torch.float64, device=eng.device)
+            with pytest.raises(ValueError, match="does not fit the z adjoint"):
+                eng.backward(xyz, idt, gH)
+        eng.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
