@@ -1,7 +1,10 @@
 """Many independent systems (uneven atom counts, boxes and grids) computed
 together: one engine plan per system, inputs in static device buffers, and
 the launch chains of every system captured once into a single hipGraph that
-is replayed per batch.
+is replayed per batch.  The systems are independent, so their chains are
+captured on parallel branches (a few default-priority streams forked from
+and joined to the capture stream): the small kernels of one system overlap
+the others' work instead of running one after another.
 
 The reference has no batching: it computes one frame of one system at a
 time (RunSaxs.cu:154-203).
@@ -30,7 +33,7 @@ class BatchedSystems:
     """forward() replays one hipGraph covering every system's frames and
     leaves per-system (H, C) in histograms()."""
 
-    def __init__(self, systems: Sequence[SystemInput], tuner=None):
+    def __init__(self, systems: Sequence[SystemInput], tuner=None, branches: int = 4):
         self.systems = list(systems)
         self.engines: List[SaxsEngine] = []
         self.xyz: List[torch.Tensor] = []
@@ -48,8 +51,30 @@ class BatchedSystems:
             self.xyz.append(torch.zeros((s.frames, s.n_atoms, 3), dtype=torch.float32,
                                         device=eng.device))
             self.elem.append(torch.zeros(s.n_atoms, dtype=torch.int32, device=eng.device))
+        # every system's H and C as views of one buffer: one fill kernel per
+        # replay clears them all
+        sizes = [eng.hist.num_bins for eng in self.engines]
+        dev = self.engines[0].device if self.engines else torch.device("cuda", 0)
+        self._hist = torch.zeros(2 * sum(sizes), dtype=torch.float64, device=dev)
+        o = 0
+        for eng, nb in zip(self.engines, sizes):
+            eng.H = self._hist[o:o + nb]
+            eng.C = self._hist[o + nb:o + 2 * nb]
+            o += 2 * nb
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self._stream = torch.cuda.Stream()
+        # independent systems on parallel graph branches (default-priority
+        # streams joined by events): longest first onto the least-loaded branch
+        nbr = max(1, min(branches, len(self.engines)))
+        self._branches = [torch.cuda.Stream() for _ in range(nbr)]
+        load = [0.0] * nbr
+        self._branch_of = [0] * len(self.engines)
+        cost = [float(np.prod(s.spec.nn)) * s.frames + 50.0 * s.n_atoms * s.frames
+                for s in self.systems]
+        for i in sorted(range(len(cost)), key=lambda j: -cost[j]):
+            b = min(range(nbr), key=lambda k: load[k])
+            self._branch_of[i] = b
+            load[b] += cost[i]
 
     def load(self, i: int, xyz, elem):
         """Copies system i's frames [frames, atoms, 3] and element ids into
@@ -62,10 +87,15 @@ class BatchedSystems:
         self.elem[i].copy_(e)
 
     def _enqueue(self):
-        for eng, x, e in zip(self.engines, self.xyz, self.elem):
-            eng.H.zero_()
-            eng.C.zero_()
-            eng.forward(x, e)
+        main = torch.cuda.current_stream()
+        self._hist.zero_()
+        for st in self._branches:
+            st.wait_stream(main)
+        for i, (eng, x, e) in enumerate(zip(self.engines, self.xyz, self.elem)):
+            with torch.cuda.stream(self._branches[self._branch_of[i]]):
+                eng.forward(x, e)
+        for st in self._branches:
+            main.wait_stream(st)
 
     def forward_eager(self):
         self._enqueue()
