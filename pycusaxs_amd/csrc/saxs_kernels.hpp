@@ -639,10 +639,16 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   }
 }
 
+// Lanes per record in the pruned kernels' spread_cols.  A column tile and
+// element holds ~100 records at 0.05 atoms per cell, so one lane per record
+// leaves most of a workgroup's 256 lanes idle in the spreading loop; two
+// lanes (x planes o = s, s + 2) measured 1.89 ms against 1.98 per 100
+// frames of the 128^3 trajectory, four 2.02 (every lane of a record
+// computes its B-spline weights).
 #ifndef SAXS_SPREAD_LANES
-#define SAXS_SPREAD_LANES 1
+#define SAXS_SPREAD_LANES 2
 #endif
-constexpr int kSpreadLanes = SAXS_SPREAD_LANES;    // lanes per record in spread_cols
+constexpr int kSpreadLanes = SAXS_SPREAD_LANES;
 constexpr int kPrefetch = 2;                       // records per thread prefetched
 constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
 // Records staged per round by the pruned z kernels: fewer, so that four
