@@ -383,10 +383,13 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tbuf = reinterpret_cast<float2*>(smem);
   float2* tw = tbuf + (size_t)LW * LY;                 // W_N^{j s} stored [j][s]
-  const size_t fe = blockIdx.y;
+  // neighbouring groups share the 128-byte lines where the kz-long rows of
+  // zbuf straddle group boundaries: each XCD (own L2) takes a contiguous range
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
   const int nx = p.n[0], ny = p.n[1], kz = p.kz;
   const int nlines = nx * kz;
-  const int g0 = blockIdx.x * LW;
+  const int g0 = (int)(grp % gridDim.x) * LW;
   const int Ky = p.Ky, ky = p.ky;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[(i >> 4) * (i & 15)];
   __syncthreads();
