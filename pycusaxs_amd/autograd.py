@@ -48,7 +48,15 @@ def saxs_profile(xyz: torch.Tensor, engine, elem: torch.Tensor):
 
 
 def chi2(I: torch.Tensor, target: torch.Tensor, sigma: torch.Tensor, bins=None) -> torch.Tensor:
-    """chi^2 = sum_b ((I_b - T_b) / s_b)^2 over the selected bins."""
-    if bins is not None:
-        I, target, sigma = I[..., bins], target[..., bins], sigma[..., bins]
+    """chi^2 = sum_b ((I_b - T_b) / s_b)^2 over the selected bins (a boolean
+    mask or an index list).  A mask is applied with torch.where, so the loss
+    keeps static shapes (no host synchronisation, an elementwise backward);
+    unselected bins get s = 1 so that their zero gradient stays finite."""
+    if bins is None:
+        return (((I - target) / sigma) ** 2).sum()
+    if isinstance(bins, torch.Tensor) and bins.dtype == torch.bool:
+        s = torch.where(bins, sigma, torch.ones_like(sigma))
+        r = torch.where(bins, (I - target) / s, torch.zeros_like(I))
+        return (r ** 2).sum()
+    I, target, sigma = I[..., bins], target[..., bins], sigma[..., bins]
     return (((I - target) / sigma) ** 2).sum()
