@@ -318,6 +318,8 @@ struct saxs_plan {
   float* gpart = nullptr;      // [F][nsx*nsy][N][3] adjoint partials (first backward)
   int* segoff = nullptr;       // [(512 + max_frames) * keys per frame] segmented sort
   bool seg_sort = false;       // one frame's keys fit the LDS histogram of k_prep_seg
+  bool seg16 = false;          // ... with 16-bit counters (segments <= kSeg16MaxLen atoms)
+  int seg_kr = 0;              // keys per k_prep_seg pass (a frame's keys in <= 4 passes)
   bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
   int x_gen_h = 0;             // forward x pass by k_xfft_g<x_gen_h> (0: Stockham)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
@@ -499,6 +501,7 @@ saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem)
   b.segoff = pl->segoff;
   b.nseg = 0;
   b.seglen = 1;
+  b.segkr = 0;
   return b;
 }
 
@@ -539,14 +542,21 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   const int kframe = p.E * p.nty * p.ntx;               // keys per frame
   const int nkeys = n_frames * kframe;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
-  const bool seg = pl->seg_sort && n_atoms > 0;
+  // 16-bit segment counters need segments of at most kSeg16MaxLen atoms
+  // within the segment table's nseg * F <= 512 + F - 1
+  const bool seg = pl->seg_sort && n_atoms > 0 &&
+                   (!pl->seg16 || (n_atoms + saxs::kSeg16MaxLen - 1) / saxs::kSeg16MaxLen <=
+                                      (512 + n_frames - 1) / n_frames);
   if (seg) {  // ~512 workgroups (nseg * F <= 512 + F - 1, the segoff allocation),
              // >= 4096 atoms per segment, and at least K/2 atoms per segment so
              // that the per-segment key histograms (K ints each, written by
              // k_prep_seg and rewritten by k_seg_keys) stay below the atom data
     b.nseg = std::max(1, std::min({(512 + n_frames - 1) / n_frames, (n_atoms + 4095) / 4096,
-                                   std::max(1, 2 * n_atoms / kframe)}));
+                                   (int)std::max(1LL, 2LL * n_atoms / kframe)}));
+    if (pl->seg16)
+      b.nseg = std::max(b.nseg, (n_atoms + saxs::kSeg16MaxLen - 1) / saxs::kSeg16MaxLen);
     b.seglen = (n_atoms + b.nseg - 1) / b.nseg;
+    b.segkr = pl->seg_kr;
   }
   { Probe pr(pl, SAXS_K_CLEAR, st);
   hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, pl->counts,
@@ -555,11 +565,18 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     Probe pr(pl, SAXS_K_PREP, st);
     if (seg) {
       dim3 gq(b.nseg, n_frames);
-      const size_t lq = saxs::prep_seg_lds_bytes(kframe, p.E);
-      if (p.order == 4)
-        hipLaunchKernelGGL(saxs::k_prep_seg<4>, gq, dim3(saxs::kSortWG), lq, st, p, b);
-      else
-        hipLaunchKernelGGL(saxs::k_prep_seg<6>, gq, dim3(saxs::kSortWG), lq, st, p, b);
+      const size_t lq = saxs::prep_seg_lds_bytes(pl->seg_kr, p.E, pl->seg16);
+      if (pl->seg16) {
+        if (p.order == 4)
+          hipLaunchKernelGGL((saxs::k_prep_seg<4, true>), gq, dim3(saxs::kSortWG), lq, st, p, b);
+        else
+          hipLaunchKernelGGL((saxs::k_prep_seg<6, true>), gq, dim3(saxs::kSortWG), lq, st, p, b);
+      } else {
+        if (p.order == 4)
+          hipLaunchKernelGGL((saxs::k_prep_seg<4, false>), gq, dim3(saxs::kSortWG), lq, st, p, b);
+        else
+          hipLaunchKernelGGL((saxs::k_prep_seg<6, false>), gq, dim3(saxs::kSortWG), lq, st, p, b);
+      }
     } else {
       dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
       if (p.order == 4)
@@ -878,7 +895,21 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->keyslot, F * N);
   {
     const int kframe = E * p.nty * p.ntx;
-    pl->seg_sort = saxs::prep_seg_lds_bytes(kframe, E) <= 48 * 1024;
+    // LDS histogram of a frame's keys: 32-bit counters in one pass, else
+    // 16-bit counters in passes over key ranges (up to 4), else the global-
+    // atomic sort (k_prep)
+    const size_t budget = 48 * 1024;
+    if (saxs::prep_seg_lds_bytes(kframe, E) <= budget) {
+      pl->seg_sort = true;
+      pl->seg_kr = kframe;
+    } else {
+      int kr = 2 * (int)((budget - 16 - E * sizeof(unsigned long long)) / sizeof(unsigned));
+      while (kr > 2 && saxs::prep_seg_lds_bytes(kr, E, true) > budget) kr -= 2;
+      if ((kframe + kr - 1) / kr <= 4) {
+        pl->seg_sort = pl->seg16 = true;
+        pl->seg_kr = std::min(kr, kframe);
+      }
+    }
     if (pl->seg_sort) ALLOC(pl->segoff, (size_t)(512 + F) * kframe);
   }
   ALLOC(pl->sorted, F * N);

@@ -75,6 +75,7 @@ struct Buffers {
   double* partial;       // [F][ky][nchx][2*num_bins]
   int* segoff;           // [F][nseg][keys per frame] segmented sort (nseg > 0)
   int nseg, seglen;      // segments per frame and atoms per segment
+  int segkr;             // keys per k_prep_seg pass
 };
 
 // Index of frequency m in the q tables (bin and form factor): the shell
@@ -302,38 +303,63 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
 // offsets the spread kernels read.  No global atomics on the counters.
 constexpr int kSortWG = 1024;
 
-__host__ __device__ inline size_t prep_seg_lds_bytes(int K, int E) {
-  return (((size_t)K * sizeof(int) + 15) & ~size_t(15)) + (size_t)E * sizeof(unsigned long long);
+// LDS of k_prep_seg for K keys a pass: the key histogram (32-bit counters,
+// or with HW 16-bit counters two to a word: a (key, segment) count never
+// exceeds the segment length, which the plan then keeps below 65536) and the
+// per-element border sums.
+__host__ __device__ inline size_t prep_seg_lds_bytes(int K, int E, bool hw = false) {
+  const size_t hist = hw ? (size_t)((K + 1) / 2) * sizeof(unsigned) : (size_t)K * sizeof(int);
+  return ((hist + 15) & ~size_t(15)) + (size_t)E * sizeof(unsigned long long);
 }
 
-template <int P>
+constexpr int kSeg16MaxLen = 65535;   // atoms per segment with 16-bit counters
+
+template <int P, bool HW>
 __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int K = p.E * p.nty * p.ntx;
-  int* hist = reinterpret_cast<int*>(smem);
+  const int KR = b.segkr;                            // keys per pass (LDS)
+  const int KRW = HW ? (KR + 1) / 2 : KR;
+  unsigned* hist = reinterpret_cast<unsigned*>(smem);
   unsigned long long* padacc =
-      reinterpret_cast<unsigned long long*>(smem + (((size_t)K * sizeof(int) + 15) & ~size_t(15)));
+      reinterpret_cast<unsigned long long*>(smem + (((size_t)KRW * sizeof(unsigned) + 15) & ~size_t(15)));
   const int s = blockIdx.x, f = blockIdx.y;
-  for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) padacc[e] = 0ull;
-  __syncthreads();
   const int i0 = s * b.seglen, i1 = min(p.N, i0 + b.seglen);
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-    const int e = b.elem[i];
-    if (e < 0 || e >= p.E) {
-      b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
-      continue;
-    }
-    int key;
-    unsigned long long border;
-    atom_key_border<P>(p, b, f, i, e, key, border);
-    const int slot = atomicAdd(&hist[key], 1);
-    b.keyslot[(size_t)f * p.N + i] = make_int2(f * K + key, slot);
-    atomicAdd(&padacc[e], border);
-  }
-  __syncthreads();
   int* out = b.segoff + ((size_t)f * b.nseg + s) * K;
-  for (int k = threadIdx.x; k < K; k += blockDim.x) out[k] = hist[k];
+  // keys in passes of KR (one pass when a frame's keys fit the LDS); the
+  // segment's atoms are read once per pass, the border sums added in the first
+  for (int kb = 0; kb < K; kb += KR) {
+    const int kn = min(KR, K - kb);
+    const int knw = HW ? (kn + 1) / 2 : kn;
+    for (int k = threadIdx.x; k < knw; k += blockDim.x) hist[k] = 0u;
+    __syncthreads();
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const int e = b.elem[i];
+      if (e < 0 || e >= p.E) {
+        if (kb == 0) b.keyslot[(size_t)f * p.N + i] = make_int2(-1, -1);
+        continue;
+      }
+      int key;
+      unsigned long long border;
+      atom_key_border<P>(p, b, f, i, e, key, border);
+      if (kb == 0) atomicAdd(&padacc[e], border);
+      const int kl = key - kb;
+      if (kl < 0 || kl >= kn) continue;
+      int slot;
+      if (HW) {
+        const unsigned sh = (unsigned)(kl & 1) * 16u;
+        slot = (int)((atomicAdd(&hist[kl >> 1], 1u << sh) >> sh) & 0xffffu);
+      } else {
+        slot = (int)atomicAdd(&hist[kl], 1u);
+      }
+      b.keyslot[(size_t)f * p.N + i] = make_int2(f * K + key, slot);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kn; k += blockDim.x)
+      out[kb + k] = HW ? (int)((hist[k >> 1] >> ((k & 1) * 16)) & 0xffffu) : (int)hist[k];
+    __syncthreads();                                 // hist is cleared for the next pass
+  }
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
     const unsigned long long v = padacc[e];
     if (v != 0ull) atomicAdd(&b.pad_sum[f * p.E + e], v);
@@ -1055,8 +1081,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 // element on its own: a stage-0 pass holds whole lines (64 / R2 of them),
 // and a wavefront's LDS operations execute in order, so no workgroup
 // barrier separates the stages.  Stage 1 splits each output's sum over up
-// to 8 lanes, as many as keep the wavefront's lanes busy, and adds the
-// parts with lane shuffles.
+// to 8 lanes (as many as fit kz pairs in a wavefront) and adds the parts
+// with lane shuffles.
 
 __host__ __device__ inline int zg_col_stride(int nz, int P, int N, int R1) {
   const int a = nz + P - 1, b = (N / R1 * (R1 + 1) + 1) / 2;
@@ -1147,10 +1173,12 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   const int g0 = 64 / R2;                            // lines per stage-0 pass
   const int lpw = (nl + kWG / 64 - 1) / (kWG / 64);  // lines per wavefront
   const int lw0 = min(nl, (int)(threadIdx.x >> 6) * lpw), lw1 = min(nl, lw0 + lpw);
-  // stage 1: nq output pairs of this wavefront, each summed by split lanes
+  // stage 1: nq output pairs of this wavefront, each summed by split lanes;
+  // split depends on kz alone, so every output is summed in the same order
+  // whatever the tile (and the spectra do not depend on it)
   const int nq = (lw1 - lw0) * kz;
   int lgs = 0;
-  while (lgs < 3 && (nq << (lgs + 1)) <= 64) ++lgs;
+  while (lgs < 3 && (kz << (lgs + 1)) <= 64) ++lgs;
   const int split = 1 << lgs;
   const float inv_kz = 1.0f / (float)kz;
   float4 pre[kZpPre];

@@ -376,9 +376,9 @@ def test_generic_pruned_x_against_oracle(gpu, n, qcut):
 def test_forward_only_plan_tile_and_backward_refusal(gpu):
     """A forward-only plan (FLAG_FORWARD_ONLY) may take a column tile that
     the z adjoint does not fit (4 x 8 on a 189^3 cell, 378^3 FFT): its
-    forward equals the default plan's bit for bit (the fixed-point columns
-    do not depend on the tile), and a backward call is refused before any
-    launch with a ValueError."""
+    forward equals the default plan's (the fixed-point columns do not depend
+    on the tile: counts exact, H to FP64 summation order), and a backward
+    call is refused before any launch with a ValueError."""
     import torch
     from pycusaxs_amd import _lib
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
@@ -402,4 +402,70 @@ def test_forward_only_plan_tile_and_backward_refusal(gpu):
                 eng.backward(xyz, idt, gH)
         eng.close()
     assert np.array_equal(out[0][1], out[1][1])
-    assert np.array_equal(out[0][0], out[1][0])
+    # the spread columns agree bit for bit; the histogram's FP64 LDS sums may
+    # add in another order
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-12, atol=0)
+
+
+def test_segmented_sort_with_16bit_counters_matches_32bit(gpu):
+    """A 200^3 cell with 5 elements in 4 x 4 column tiles has 12,500 keys a
+    frame: the segment histogram only fits the LDS with 16-bit counters
+    (segments of at most 65,535 atoms).  The same system in 4 x 8 tiles
+    (6,250 keys) sorts with 32-bit counters; the fixed-point columns agree
+    bit for bit, so the counts are equal and H agrees to FP64 summation order."""
+    import torch
+    from pycusaxs_amd import _lib
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    n, L, F = 200, 150.0, 2
+    box = np.diag([L] * 3)
+    pos, syms = O.synthetic_system(150000, L, seed=200)
+    symbols, ids = O.element_ids(syms)
+    frames = np.stack([O.frame_positions(pos, 200, f, 0.5) for f in range(F)]).astype(np.float32)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.3)
+    out = []
+    for tile in ((4, 4), (4, 8)):
+        eng = SaxsEngine(spec, box, max_atoms=len(pos), max_frames=F, tile=tile,
+                         flags=_lib.FLAG_FORWARD_ONLY)
+        eng.forward(torch.from_numpy(frames), torch.from_numpy(ids))
+        torch.cuda.synchronize()
+        out.append((eng.H.cpu().numpy(), eng.C.cpu().numpy()))
+        eng.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    # the spread columns agree bit for bit; the histogram's FP64 LDS sums may
+    # add in another order
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-12, atol=0)
+
+
+def test_segmented_sort_in_key_passes_matches_single_pass(gpu):
+    """32 element types on a 64^3 cell in 2 x 2 column tiles: 32,768 keys a
+    frame, counted in two passes over key ranges with 16-bit counters; in
+    4 x 8 tiles the 4,096 keys fit one pass of 32-bit counters.  Same
+    fixed-point columns, so equal counts and H to FP64 summation order."""
+    import json
+    import os
+    import torch
+    from pycusaxs_amd import _lib
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    table = json.load(open(os.path.join(os.path.dirname(_lib.__file__), "data",
+                                        "cromer_mann_it1992.json")))
+    elems = [s for s in table if s.isalpha() and s != "Du"][:32]
+    n, L, F = 64, 48.0, 2
+    rng = np.random.default_rng(64)
+    pos = rng.uniform(0.0, L, size=(40000, 3)).astype(np.float32)
+    syms = list(rng.choice(elems, size=len(pos)))
+    symbols, ids = O.element_ids(syms)
+    assert len(symbols) == 32
+    frames = np.stack([O.frame_positions(pos, 64, f, 0.5) for f in range(F)]).astype(np.float32)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.8)
+    out = []
+    for tile in ((2, 2), (4, 8)):
+        eng = SaxsEngine(spec, np.diag([L] * 3), max_atoms=len(pos), max_frames=F, tile=tile,
+                         flags=_lib.FLAG_FORWARD_ONLY)
+        eng.forward(torch.from_numpy(frames), torch.from_numpy(ids))
+        torch.cuda.synchronize()
+        out.append((eng.H.cpu().numpy(), eng.C.cpu().numpy()))
+        eng.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-12, atol=0)
