@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0
 # TCC_EA0_RDREQ_128B + 64 B x TCC_EA0_RDREQ_64B, writes = WRITE_SIZE; the same
 # counters give a 512 MiB elementwise copy exactly.  Commands and the
 # per-kernel breakdown: profiles/hbm_traffic_trajectory_100k_128.md.
-MEASURED_TRAFFIC = {"prep": 211.6e6, "scatter": 561.7e6, "spread_z": 1675.8e6, "yfft": 1710.1e6,
-                    "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1710.1e6, "adj_z": 2718.9e6,
+MEASURED_TRAFFIC = {"prep": 211.2e6, "scatter": 572.5e6, "spread_z": 1675.8e6, "yfft": 1710.1e6,
+                    "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1710.2e6, "adj_z": 2718.9e6,
                     "grad_sum": 755.9e6}
 
 

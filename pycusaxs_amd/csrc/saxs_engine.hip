@@ -897,7 +897,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
     const int kframe = E * p.nty * p.ntx;
     // LDS histogram of a frame's keys: 32-bit counters in one pass, else
     // 16-bit counters in passes over key ranges (up to 4), else the global-
-    // atomic sort (k_prep)
+    // atomic sort (k_prep); k_prep_seg<P, false> always runs a single pass
     const size_t budget = 48 * 1024;
     if (saxs::prep_seg_lds_bytes(kframe, E) <= budget) {
       pl->seg_sort = true;

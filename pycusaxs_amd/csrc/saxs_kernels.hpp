@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
     key += f * p.E * p.nty * p.ntx;
     const int slot = atomicAdd(&b.counts[key], 1);
     b.keyslot[(size_t)f * p.N + i] = make_int2(key, slot);
-    atomicAdd(&padacc[e], border);
+    if (border != 0ull) atomicAdd(&padacc[e], border);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < p.E; e += blockDim.x) {
@@ -318,7 +318,7 @@ template <int P, bool HW>
 __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int K = p.E * p.nty * p.ntx;
-  const int KR = b.segkr;                            // keys per pass (LDS)
+  const int KR = HW ? b.segkr : K;                   // keys per pass (LDS): 32-bit, one pass
   const int KRW = HW ? (KR + 1) / 2 : KR;
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
   unsigned long long* padacc =
@@ -343,17 +343,20 @@ __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
       int key;
       unsigned long long border;
       atom_key_border<P>(p, b, f, i, e, key, border);
-      if (kb == 0) atomicAdd(&padacc[e], border);
       const int kl = key - kb;
-      if (kl < 0 || kl >= kn) continue;
-      int slot;
-      if (HW) {
-        const unsigned sh = (unsigned)(kl & 1) * 16u;
-        slot = (int)((atomicAdd(&hist[kl >> 1], 1u << sh) >> sh) & 0xffffu);
-      } else {
-        slot = (int)atomicAdd(&hist[kl], 1u);
+      if (kl >= 0 && kl < kn) {
+        int slot;
+        if (HW) {
+          const unsigned sh = (unsigned)(kl & 1) * 16u;
+          slot = (int)((atomicAdd(&hist[kl >> 1], 1u << sh) >> sh) & 0xffffu);
+        } else {
+          slot = (int)atomicAdd(&hist[kl], 1u);
+        }
+        b.keyslot[(size_t)f * p.N + i] = make_int2(f * K + key, slot);
       }
-      b.keyslot[(size_t)f * p.N + i] = make_int2(f * K + key, slot);
+      // border-slab share: nonzero for the few atoms near the slab only (the
+      // E accumulators are contended)
+      if (kb == 0 && border != 0ull) atomicAdd(&padacc[e], border);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kn; k += blockDim.x)
