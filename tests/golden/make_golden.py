@@ -8,6 +8,11 @@
       histogram H_b, counts C_b and output rows, once with the border-mean
       pad (padding::avg) and once with the explicit-water pad of SPC/E
       (padding::given, 0.334 e/A^3).
+  reference_solvents_db.json
+      The layout of the reference's solvent-profile database
+      (pycusaxs/data/reference_solvents.db, read with sqlite3 in read-only
+      mode) and, per stored profile, its key parameters, its profile_hash, its
+      point count and first points: pins the profile store's format.
   ref_helpers.json
       Outputs of the reference's own host helpers compiled from its sources
       by oracle/build_ref.sh (oracle/_ref/ref_driver): B-spline weights and
@@ -101,10 +106,42 @@ def ref_helpers():
     return d
 
 
+REF_DB = "/root/reference/pycusaxs/data/reference_solvents.db"
+
+
+def reference_db_fixture():
+    """The reference's solvent-profile database, opened read-only: its table
+    layout (PRAGMA table_info), and per row the key parameters, the stored
+    profile_hash, the point count and the first ten (q, I) points."""
+    import sqlite3
+    con = sqlite3.connect(f"file:{REF_DB}?mode=ro", uri=True)
+    cols = [list(r) for r in con.execute("PRAGMA table_info(saxs_profiles)")]
+    rows = []
+    for r in con.execute("SELECT profile_hash, water_model, n_water_molecules, ion_counts, "
+                         "other_molecules, box_x, box_y, box_z, box_volume, supercell_scale, "
+                         "supercell_volume, simulation_time_ps, n_frames_analyzed, frame_stride, "
+                         "grid_size, bin_size, qcut, spline_order, density_g_cm3, n_atoms, "
+                         "profile_data FROM saxs_profiles ORDER BY id"):
+        data = json.loads(r[20])
+        rows.append({"profile_hash": r[0], "water_model": r[1], "n_water_molecules": r[2],
+                     "ion_counts": json.loads(r[3]), "other_molecules": json.loads(r[4] or "{}"),
+                     "box": [r[5], r[6], r[7]], "box_volume": r[8], "supercell_scale": r[9],
+                     "supercell_volume": r[10], "simulation_time_ps": r[11],
+                     "n_frames_analyzed": r[12], "frame_stride": r[13],
+                     "grid_size": json.loads(r[14]), "bin_size": r[15], "qcut": r[16],
+                     "spline_order": r[17], "density_g_cm3": r[18], "n_atoms": r[19],
+                     "n_points": len(data), "first_points": data[:10]})
+    con.close()
+    return {"columns": cols, "rows": rows}
+
+
 def main():
     np.savez_compressed(os.path.join(HERE, "atoms5k_box37_avg.npz"), **atoms5k_fixture(None))
     given = O.solvent_pads((ATOMS5K["L"],) * 3, (ATOMS5K["n"],) * 3, "SPCE", {})
     np.savez_compressed(os.path.join(HERE, "atoms5k_box37_given.npz"), **atoms5k_fixture(given))
+    if os.path.exists(REF_DB):
+        with open(os.path.join(HERE, "reference_solvents_db.json"), "w") as fh:
+            json.dump(reference_db_fixture(), fh, indent=1)
     if os.path.exists(REF_DRIVER):
         with open(os.path.join(HERE, "ref_helpers.json"), "w") as fh:
             json.dump(ref_helpers(), fh, indent=1)
