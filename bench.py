@@ -392,6 +392,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="with --profile-only: also run one elementwise copy of this many MiB "
                          "(known HBM bytes, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the forward as eager launches only (no hipGraph replay)")
     ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6"),
                     help="run only this secondary workload (for rocprofv3) and print its JSON")
     args = ap.parse_args()
@@ -423,16 +425,46 @@ def main():
     for _ in range(args.warmup):
         eng.forward(xyz, ids)
     torch.cuda.synchronize()
+    frames_done = args.steps * args.frames
+    # eager launch chain (one host call per step)
     eng.reset()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.forward(xyz, ids)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    frames_done = args.steps * args.frames
-    value = frames_done / elapsed
-    print(f"# forward: {value:.1f} frames/s ({elapsed * 1e3 / args.steps:.3f} ms/step)",
-          file=sys.stderr, flush=True)
+    eager_value = frames_done / elapsed
+    print(f"# forward, eager launches: {eager_value:.1f} frames/s "
+          f"({elapsed * 1e3 / args.steps:.3f} ms/step)", file=sys.stderr, flush=True)
+    value = eager_value
+    graph_used = False
+    if not args.no_graph and not args.profile_only:
+        # the same launch chain captured once as a hipGraph and replayed per
+        # step: the same kernels on the same inputs without the host launch
+        # gaps between them
+        cap = torch.cuda.Stream()
+        cap.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(cap):
+            eng.forward(xyz, ids)
+        torch.cuda.current_stream().wait_stream(cap)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            eng.forward(xyz, ids)
+        for _ in range(args.warmup):
+            graph.replay()
+        torch.cuda.synchronize()
+        eng.reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        value = frames_done / elapsed
+        graph_used = True
+        del graph
+        print(f"# forward, hipGraph replay: {value:.1f} frames/s "
+              f"({elapsed * 1e3 / args.steps:.3f} ms/step)", file=sys.stderr, flush=True)
 
     line = {
         "metric": "forward frames/s, 100k atoms, 128^3 grid, sigma=2 (256^3 FFT), batched",
@@ -442,6 +474,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "forward_launch": "hipGraph replay of the launch chain" if graph_used else "eager",
+        "eager_frames_per_s": round(eager_value, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -479,6 +513,30 @@ def main():
         eng.backward(xyz, ids, gH, generation=eng.generation)
     torch.cuda.synchronize()
     fb = frames_done / (time.perf_counter() - t0)
+    line["fwd_bwd_eager_frames_per_s"] = round(fb, 2)
+    if graph_used:
+        # the same forward + backward chain as one replayed hipGraph (the
+        # partial-gradient slots were allocated by the warm-up above)
+        cap = torch.cuda.Stream()
+        cap.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(cap):
+            eng.forward_frames(xyz, ids)
+            eng.backward(xyz, ids, gH, generation=eng.generation)
+        torch.cuda.current_stream().wait_stream(cap)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            eng.forward_frames(xyz, ids)
+            eng.backward(xyz, ids, gH, generation=eng.generation)
+        for _ in range(max(1, args.warmup)):
+            graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        fb = frames_done / (time.perf_counter() - t0)
+        del graph
     line["fwd_bwd_frames_per_s"] = round(fb, 2)
     line["fwd_bwd_time_over_fwd"] = round(value / fb, 3)
     print(f"# fwd+bwd: {fb:.1f} frames/s ({value / fb:.2f}x forward time)", file=sys.stderr,
