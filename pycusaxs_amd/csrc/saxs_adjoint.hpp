@@ -725,6 +725,211 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
   }
 }
 
+// ---- output-pruned z adjoint for any N = 2H R2 with n_z <= N/2 ----------
+//
+// The transpose of k_spread_zg's two stages.  The kept spectrum gradient of
+// a line (column pair) is first formed as in k_adj_zp into v(k), nonzero for
+// k < kz and N - k < kz (staged in LDS); then one thread per butterfly j < R2
+// of the line computes
+//   Ybar_j[s] = sum_{k = s mod 2H} v(k) W_N^{jk}       (one or two terms a s
+//                                                     when kz <= 2H)
+//   out(j + R2 r) = sum_s Ybar_j[s] W_2H^{rs},  r < H  (two DFTs of length H:
+//                   E_r + W_2H^r O_r over the even and odd s)
+// which is out(z) = sum_k v(k) W_N^{kz} for every z < N/2, and writes the
+// drho~ columns (Re, -Im as k_adj_zp) with their z halo straight to LDS: no
+// exchange between the stages.  The pad-term sum and the gather follow
+// k_adj_zp.  The plan guarantees kz <= kAdjZgMaxKz.
+
+constexpr int kAdjZgMaxKz = 64;
+
+__host__ __device__ inline int adjg_col_stride(int nz, int P) { return (nz + P - 1 + 1) & ~1; }
+
+__host__ __device__ inline size_t adj_g_lds_bytes(const Params& p) {
+  const int cs = adjg_col_stride(p.n[2], p.order);
+  const size_t A = ((size_t)p.tx * p.ty * cs * sizeof(float) + 15) & ~size_t(15);
+  const size_t V = ((size_t)(p.tx * p.ty / 2) * 2 * p.kz * sizeof(float2) + 15) & ~size_t(15);
+  const size_t B = ((size_t)kZpRecs * sizeof(float4) + 15) & ~size_t(15);
+  const size_t tw = (size_t)p.nn[2] * sizeof(float2);
+  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  return A + V + B + tw + seg;
+}
+
+template <int P, int H>
+__global__ void __launch_bounds__(kWG) k_adj_zg(Params p, Buffers b, AdjBuffers g) {
+  constexpr int R1 = 2 * H;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x, f = blockIdx.y;
+  const int bx = tile % p.ntx, by = tile / p.ntx;
+  const int X0 = bx * p.tx, Y0 = by * p.ty;
+  const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
+  const int nz = p.n[2], N = p.nn[2], R2 = N / R1;
+  const int ncol = p.tx * p.ty;
+  const int nl = ncol / 2;
+  const int E = p.E, kz = p.kz;
+  const int cs = adjg_col_stride(nz, P);
+  const size_t A_b = ((size_t)ncol * cs * sizeof(float) + 15) & ~size_t(15);
+  const size_t V_b = ((size_t)nl * 2 * kz * sizeof(float2) + 15) & ~size_t(15);
+  const size_t B_b = ((size_t)kZpRecs * sizeof(float4) + 15) & ~size_t(15);
+  float* drho = reinterpret_cast<float*>(smem);
+  float2* vbuf = reinterpret_cast<float2*>(smem + A_b);   // [l][v_lo(0..kz) | v_hi(0..kz)]
+  float4* recs = reinterpret_cast<float4*>(smem + A_b + V_b);
+  float2* tw = reinterpret_cast<float2*>(smem + A_b + V_b + B_b);   // W_N^m
+  int* s_start = reinterpret_cast<int*>(tw + N);
+  int* s_cnt = s_start + 9 * E;
+  int* s_pref = s_cnt + 9 * E;
+  __shared__ double s_red[kWG / 64];
+
+  for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
+    const int e = t / 9, sg = t - e * 9;
+    const int4 sxv = b.srcx[bx], syv = b.srcy[by];
+    const int ix = sg % 3, iy = sg / 3;
+    int st = 0, cnt = 0;
+    if (ix < sxv.x && iy < syv.x) {
+      const int kx = ix == 0 ? sxv.y : (ix == 1 ? sxv.z : sxv.w);
+      const int ky = iy == 0 ? syv.y : (iy == 1 ? syv.z : syv.w);
+      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
+      st = b.offsets[key];
+      cnt = b.counts[key];
+    }
+    s_start[t] = st;
+    s_cnt[t] = cnt;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int acc = 0;
+    for (int k = 0; k < 9; ++k) {
+      s_pref[e * 10 + k] = acc;
+      acc += s_cnt[e * 9 + k];
+    }
+    s_pref[e * 10 + 9] = acc;
+  }
+  __syncthreads();
+  auto fetch = [&](int e, int a) -> float4 {
+    const int* pref = s_pref + e * 10;
+    int sgm = 0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
+    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+  };
+  const float inv_ty = 1.0f / (float)p.ty;
+  const float inv_r2 = 1.0f / (float)R2;
+  const float inv_kz = 1.0f / (float)kz;
+
+  for (int e = 0; e < E; ++e) {
+    const size_t fe = (size_t)f * E + e;
+    const int total = s_pref[e * 10 + 9];
+    const int staged = min(total, kZpRecs);
+    float4 pre[kZpPre];
+#pragma unroll
+    for (int u = 0; u < kZpPre; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < staged) pre[u] = fetch(e, a);
+    }
+    // v(k) (k < kz) and v(N - k) (0 < k < kz) of every line, from zbuf
+    for (int item = threadIdx.x; item < nl * kz; item += blockDim.x) {
+      const int l = div_small(item, kz, inv_kz), k = item - l * kz;
+      const int c0 = 2 * l;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+      float2 lo = make_float2(0.0f, 0.0f), hi = make_float2(0.0f, 0.0f);
+      if (lx < wx && ly < wy) {
+        const float2* za = b.zbuf + ((fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly)) * kz;
+        const float2 a = za[k];
+        const float2 c = (ly + 1 < wy) ? za[kz + k] : make_float2(0.0f, 0.0f);
+        lo = k == 0 ? make_float2(a.x, -c.x) : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
+        hi = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+      }
+      vbuf[l * 2 * kz + k] = lo;
+      vbuf[l * 2 * kz + kz + k] = hi;
+    }
+    __syncthreads();                                  // v staged
+    double part = 0.0;
+    for (int item = threadIdx.x; item < nl * R2; item += blockDim.x) {
+      const int l = div_small(item, R2, inv_r2), j = item - l * R2;
+      const int c0 = 2 * l;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
+      if (lx >= wx || ly >= wy) continue;            // never gathered
+      const float2* vlo = vbuf + l * 2 * kz;
+      const float2* vhi = vlo + kz;
+      const int stepR = (R1 * j) % N;                 // (2H j) mod N
+      float2 Y[R1];
+      int base = 0;                                   // (s j) mod N
+#pragma unroll
+      for (int s = 0; s < R1; ++s) {
+        float2 acc = make_float2(0.0f, 0.0f);
+        int idx = base;
+        for (int k = s; k < kz; k += R1) {            // v(k) W_N^{jk}
+          const float2 w = tw[idx], v = vlo[k];
+          acc.x += v.x * w.x - v.y * w.y;
+          acc.y += v.x * w.y + v.y * w.x;
+          idx += stepR;
+          if (idx >= N) idx -= N;
+        }
+        int m = s == 0 ? R1 : R1 - s;                 // N - m = s mod 2H, m >= 1
+        int idh = stepR - base;                       // (m j) mod N
+        if (idh < 0) idh += N;
+        for (; m < kz; m += R1) {                     // v(N - m) conj(W_N^{jm})
+          const float2 w = tw[idh], v = vhi[m];
+          acc.x += v.x * w.x + v.y * w.y;
+          acc.y += v.y * w.x - v.x * w.y;
+          idh += stepR;
+          if (idh >= N) idh -= N;
+        }
+        Y[s] = acc;
+        base += j;
+        if (base >= N) base -= N;
+      }
+      float2 ev[H], od[H];
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        ev[t] = Y[2 * t];
+        od[t] = Y[2 * t + 1];
+      }
+      Radix<H>::dft(ev);
+      Radix<H>::dft(od);
+      const bool ok1 = ly + 1 < wy;
+      float* col0 = drho + c0 * cs;
+      float* col1 = col0 + cs;
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int z = j + R2 * r;
+        if (z >= nz) continue;
+        const float2 x = r == 0 ? cadd(ev[0], od[0]) : cadd(ev[r], cmul(od[r], tw[r * R2]));
+        const float a = x.x, c = ok1 ? -x.y : 0.0f;
+        col0[z] = a;
+        col1[z] = c;
+        if (r == 0 && z < P - 1) {                    // z halo
+          col0[nz + z] = a;
+          col1[nz + z] = c;
+        }
+        part += (double)a + (double)c;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
+#pragma unroll
+    for (int u = 0; u < kZpPre; ++u) {
+      const int a = threadIdx.x + u * blockDim.x;
+      if (a < staged) recs[a] = pre[u];
+    }
+    __syncthreads();                                  // drho~ and records ready
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
+      atomicAdd(&g.padg[fe], sum);
+    }
+    gather_cols<P>(recs, staged, drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
+    for (int base = kZpRecs; base < total; base += kZpRecs) {
+      const int chunk = min(kZpRecs, total - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
+      __syncthreads();
+      gather_cols<P>(recs, chunk, drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
+    }
+    __syncthreads();                                  // gather done: columns reused
+  }
+}
+
 // Pruned z-pass adjoint + gradient gather, all elements of one column tile
 // per workgroup.  Stage 0: the kept spectrum of a column pair (kz <= N / 8,
 // so only inputs r = 0, 1 of the low and r = 14, 15 of the mirrored

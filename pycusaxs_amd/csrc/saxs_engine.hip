@@ -183,6 +183,19 @@ int zg_half(bool spread_all, bool pruned_z, const saxs::Params& p) {
   return 0;
 }
 
+// Half-length H of the generic output-pruned z adjoint (k_adj_zg, N = 2H R2),
+// or 0 where it does not apply: n_z <= N/2, kz <= kAdjZgMaxKz, an even
+// R1 = 2H dividing N with order <= R2, and the LDS (same preference as
+// zg_half).
+int adjg_half(bool spread_all, bool pruned_z, const saxs::Params& p) {
+  const int Nz = p.nn[2];
+  if (!spread_all || !pruned_z || 2 * p.n[2] > Nz || p.kz > saxs::kAdjZgMaxKz) return 0;
+  if (saxs::adj_g_lds_bytes(p) > kLdsLimit) return 0;
+  for (int H : {8, 7, 6, 5, 9, 10})
+    if (Nz % (2 * H) == 0 && Nz / (2 * H) >= p.order) return H;
+  return 0;
+}
+
 // Half-length H of the first stage of the generic output-pruned y transform
 // (k_yfft_g, N = 2H R2), or 0 where it does not apply: it needs n_y <= N/2,
 // few kept outputs (Ky < N/8), an even R1 = 2H dividing N with R2 <= 64,
@@ -212,7 +225,10 @@ bool tile_fits(bool spread_all, bool pruned_z, saxs::Params p, int lut_len, bool
                    zg_half(spread_all, pruned_z, p) > 0 ||
                    (line_ok && spread_all && saxs::spread_all_lds_bytes(p) <= kLdsLimit) ||
                    (line_ok && saxs::spread_lds_bytes(p) <= kLdsLimit);
+  saxs::Params q = p;                      // room for one more kept z frequency
+  q.kz = std::min(p.kz + 1, p.nn[2] / 2 + 1);
   const bool bwd = (pruned && saxs::adj_p_lds_bytes(p) <= kLdsLimit) ||
+                   adjg_half(spread_all, pruned_z, q) > 0 ||
                    (line_ok && saxs::adjz_lds_bytes(p) <= kLdsLimit);
   return fwd && (bwd || fwd_only);
 }
@@ -819,6 +835,15 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   if (ty + d->order - 1 > p.n[1]) ty = (p.n[1] - d->order + 1) & ~1;
   pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
   pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
+  if (d->tile_x <= 0 && d->tile_y <= 0 && tx == 4 && ty == 8) {
+    // default tile on long generic columns: 4 x 4 once the generic pruned
+    // spreading kernel's LDS passes 48 KiB at 4 x 8 (two workgroups a CU;
+    // measured faster from 160^3 cells up, slower below)
+    p.tx = tx;
+    p.ty = ty;
+    const int h = zg_half(pl->spread_all, pl->pruned_z, p);
+    if (h && saxs::spread_g_lds_bytes(p, 2 * h) > 48 * 1024) ty = 4;
+  }
   for (;;) {
     p.tx = tx;
     p.ty = ty;
@@ -1040,7 +1065,8 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
                         2 * q.kz - 1 <= Nz / 4 && q.kz <= Nz / 8 &&
                         (q.tx * q.ty / 2) * (Nz / 16) <= saxs::kWG &&
                         saxs::adj_p_lds_bytes(q) <= kLdsLimit;
-    if (!pruned && (q.tx * q.ty / 2 > fft_line_cap(q.fz) || saxs::adjz_lds_bytes(q) > kLdsLimit))
+    if (!pruned && !adjg_half(pl->spread_all, pl->pruned_z, q) &&
+        (q.tx * q.ty / 2 > fft_line_cap(q.fz) || saxs::adjz_lds_bytes(q) > kLdsLimit))
       return fail(SAXS_EUNSUP, "column tile %dx%d does not fit the z adjoint (plan created with "
                   "SAXS_FLAG_FORWARD_ONLY)", q.tx, q.ty);
   }
@@ -1119,9 +1145,30 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   const bool adj_pruned = pl->spread_all && pl->pruned_z && (Nz == 128 || Nz == 256 || Nz == 512) &&
                           2 * p.kz - 1 <= Nz / 4 && p.kz <= Nz / 8 &&
                           (p.tx * p.ty / 2) * (Nz / 16) <= saxs::kWG && lds_ap <= kLdsLimit;
-  if (!adj_pruned && (p.tx * p.ty / 2 > fft_line_cap(p.fz) || saxs::adjz_lds_bytes(p) > kLdsLimit))
+  const int adj_h = adj_pruned ? 0 : adjg_half(pl->spread_all, pl->pruned_z, p);
+  if (!adj_pruned && !adj_h &&
+      (p.tx * p.ty / 2 > fft_line_cap(p.fz) || saxs::adjz_lds_bytes(p) > kLdsLimit))
     return fail(SAXS_EUNSUP, "column tile %dx%d does not fit the z adjoint", p.tx, p.ty);
-  if (adj_pruned) {
+  if (adj_h) {
+    Probe pr(pl, SAXS_K_ADJ_Z, st);
+    dim3 ga(p.ntx * p.nty, n_frames);
+    const size_t lds = saxs::adj_g_lds_bytes(p);
+#define SAXS_AZG(PP, HH) hipLaunchKernelGGL((saxs::k_adj_zg<PP, HH>), ga, dim3(saxs::kWG), lds, st, p, b, g)
+#define SAXS_AZGH(PP)                                                                  \
+  do {                                                                                 \
+    switch (adj_h) {                                                                   \
+      case 10: SAXS_AZG(PP, 10); break;                                                \
+      case 9: SAXS_AZG(PP, 9); break;                                                  \
+      case 8: SAXS_AZG(PP, 8); break;                                                  \
+      case 7: SAXS_AZG(PP, 7); break;                                                  \
+      case 6: SAXS_AZG(PP, 6); break;                                                  \
+      default: SAXS_AZG(PP, 5); break;                                                 \
+    }                                                                                  \
+  } while (0)
+    if (p.order == 4) SAXS_AZGH(4); else SAXS_AZGH(6);
+#undef SAXS_AZGH
+#undef SAXS_AZG
+  } else if (adj_pruned) {
     Probe pr(pl, SAXS_K_ADJ_Z, st);
     dim3 ga(p.ntx * p.nty, n_frames);
     const bool zhalf = 2 * p.n[2] <= Nz, zexact = 2 * p.n[2] == Nz;

@@ -375,10 +375,12 @@ def test_generic_pruned_x_against_oracle(gpu, n, qcut):
 
 def test_forward_only_plan_tile_and_backward_refusal(gpu):
     """A forward-only plan (FLAG_FORWARD_ONLY) may take a column tile that
-    the z adjoint does not fit (4 x 8 on a 189^3 cell, 378^3 FFT): its
-    forward equals the default plan's (the fixed-point columns do not depend
-    on the tile: counts exact, H to FP64 summation order), and a backward
-    call is refused before any launch with a ValueError."""
+    no z adjoint fits (4 x 8 on a 189^3 cell, 378^3 FFT, with q <= 3.0 so
+    that kz = 68 exceeds the generic pruned adjoint's 64 and only the
+    Stockham adjoint, too large for that tile, would remain): its forward
+    equals the default plan's (the fixed-point columns do not depend on the
+    tile: counts exact, H to FP64 summation order), and a backward call is
+    refused before any launch with a ValueError."""
     import torch
     from pycusaxs_amd import _lib
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
@@ -387,7 +389,7 @@ def test_forward_only_plan_tile_and_backward_refusal(gpu):
     pos, syms = O.synthetic_system(60000, L, seed=189)
     symbols, ids = O.element_ids(syms)
     n3, nn = resolve_grid((n,) * 3, 2.0)
-    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.5)
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 3.0)
     xyz = torch.from_numpy(pos[None])
     idt = torch.from_numpy(ids)
     out = []
