@@ -252,3 +252,25 @@ def test_forward_backward_chain_replays_as_graph(gpu):
         assert torch.allclose(out["g"], ref["g"], rtol=1e-6, atol=1e-6 * ref["g"].abs().max().item()), rep
     del g
     eng.close()
+
+
+@pytest.mark.parametrize("n,L,order,qcut", [(35, 26.0, 4, 100.0),   # N 70 = 14 x 5, kz = N/2 + 1
+                                            (45, 34.0, 4, 1.0),     # 90 = 18 x 5
+                                            (50, 37.5, 6, 0.9),     # 100 = 20 x 5, order 6
+                                            (70, 52.5, 4, 0.5)])    # 140 = 14 x 10
+def test_adjoint_generic_pruned_z_against_oracle(gpu, n, L, order, qcut):
+    """The output-pruned z adjoint for N = 2H R2 (k_adj_zg) against the FP64
+    oracle gradient, through full-spectrum and pruned kept ranges."""
+    run_case(n, L, max(600, int(0.1 * L ** 3)), order=order, qcut=qcut, seed=n + 5)
+
+
+def test_adjoint_generic_pruned_z_matches_stockham(gpu):
+    """189^3 cell (N = 378 = 14 x 27): the generic pruned z adjoint against
+    the Stockham adjoint (flag 2) on the same inputs and dL/dH."""
+    L, n = 141.75, 189
+    pos, syms = O.synthetic_system(60000, L, seed=3)
+    g1, _ = _engine_grad(n, L, pos, syms, 0.5, 2.0, 0, tile=(4, 4))
+    g2, _ = _engine_grad(n, L, pos, syms, 0.5, 2.0, 2, tile=(4, 4))
+    rel = np.linalg.norm(g1 - g2) / np.linalg.norm(g2)
+    print(f"[generic pruned vs Stockham adjoint, n=189] rel-L2 {rel:.2e}", flush=True)
+    assert rel <= 2e-5
