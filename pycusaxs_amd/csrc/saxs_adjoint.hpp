@@ -648,6 +648,100 @@ __host__ __device__ inline size_t adj_p_lds_bytes(const Params& p) {
   return A + B + tw + seg;
 }
 
+// ---- output-pruned y adjoint for any N = 2H R2 with n_y <= N/2 -----------
+//
+// The same map as k_adj_y, out(y) = conj(sum_m conj(Ybar(m)) W_N^{my}) over
+// the kept m_y (m <= Ky or N - m <= Ky) for y < n_y, as the transpose of
+// k_yfft_g's stages: v(m) = conj(Ybar(m)) of each line staged in LDS, then
+// one thread per (line, butterfly j) forms Tbar_j[s] = sum_{m = s mod 2H}
+// v(m) W_N^{jm} and the half-output DFT of length 2H by two length-H DFTs
+// (as k_adj_zg).  Lines of a workgroup have consecutive kz (coalesced reads
+// of ybuf and writes of zbuf); groups map to XCDs contiguously.  The plan
+// guarantees 2 Ky + 1 < N.
+
+__host__ __device__ inline size_t adj_y_g_lds_bytes(const Params& p, int R1) {
+  const int N = p.nn[1];
+  return (size_t)(kWG / (N / R1)) * p.ky * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H>
+__global__ void __launch_bounds__(kWG) k_adj_y_g(Params p, Buffers b) {
+  constexpr int R1 = 2 * H;
+  const int N = p.nn[1], R2 = N / R1, LW = kWG / R2;
+  const int Ky = p.Ky, ky = p.ky;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* vbuf = reinterpret_cast<float2*>(smem);          // [LW][ky] in ybuf's jy order
+  float2* tw = vbuf + (size_t)LW * ky;                     // W_N^m
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
+  const int g0 = (int)(grp % gridDim.x) * LW;
+  const float inv_lw = 1.0f / (float)LW, inv_kz = 1.0f / (float)kz;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[i];
+  for (int item = threadIdx.x; item < LW * ky; item += blockDim.x) {
+    const int jy = div_small(item, LW, inv_lw), l = item - jy * LW;
+    const int g = g0 + l;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (g < nlines) {
+      const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+      v = b.ybuf[((fe * ky + jy) * nx + x) * kz + k];
+    }
+    vbuf[l * ky + jy] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  if ((int)threadIdx.x >= LW * R2) return;
+  const int j = div_small(threadIdx.x, LW, inv_lw), l = threadIdx.x - j * LW;
+  const int g = g0 + l;
+  if (g >= nlines) return;
+  const float2* vl = vbuf + l * ky;                        // v(m) = vl[m], v(N - m) = vl[ky - m]
+  const int stepR = (R1 * j) % N;
+  float2 Y[R1];
+  int base = 0;                                            // (s j) mod N
+#pragma unroll
+  for (int s = 0; s < R1; ++s) {
+    float2 acc = make_float2(0.0f, 0.0f);
+    int idx = base;
+    for (int m = s; m <= Ky; m += R1) {                    // v(m) W_N^{jm}
+      const float2 w = tw[idx], v = vl[m];
+      acc.x += v.x * w.x - v.y * w.y;
+      acc.y += v.x * w.y + v.y * w.x;
+      idx += stepR;
+      if (idx >= N) idx -= N;
+    }
+    int m = s == 0 ? R1 : R1 - s;                          // N - m = s mod 2H, m >= 1
+    int idh = stepR - base;                                // (m j) mod N
+    if (idh < 0) idh += N;
+    for (; m <= Ky; m += R1) {                             // v(N - m) conj(W_N^{jm})
+      const float2 w = tw[idh], v = vl[ky - m];
+      acc.x += v.x * w.x + v.y * w.y;
+      acc.y += v.y * w.x - v.x * w.y;
+      idh += stepR;
+      if (idh >= N) idh -= N;
+    }
+    Y[s] = acc;
+    base += j;
+    if (base >= N) base -= N;
+  }
+  float2 ev[H], od[H];
+#pragma unroll
+  for (int t = 0; t < H; ++t) {
+    ev[t] = Y[2 * t];
+    od[t] = Y[2 * t + 1];
+  }
+  Radix<H>::dft(ev);
+  Radix<H>::dft(od);
+  const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+  float2* dst = b.zbuf + ((fe * nx + x) * (size_t)ny) * kz + k;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const int y = j + R2 * r;
+    if (y >= ny) continue;
+    const float2 o = r == 0 ? cadd(ev[0], od[0]) : cadd(ev[r], cmul(od[r], tw[r * R2]));
+    dst[(size_t)y * kz] = make_float2(o.x, -o.y);
+  }
+}
+
 // Adjoint of spread_cols: for each staged atom, the in-tile part of
 // sum_stencil drho~ grad(th_x th_y th_z), with drho~ in columns of stride cs
 // whose z halo repeats z < P - 1 after nz.  The stencil is unrolled and

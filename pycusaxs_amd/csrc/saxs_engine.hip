@@ -1133,6 +1133,21 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } while (0)
     if (Ny == 128) SAXS_AYP(128); else if (Ny == 256) SAXS_AYP(256); else SAXS_AYP(512);
 #undef SAXS_AYP
+  } else if (const int yh = yg_half(pl->pruned_z, p)) {
+    Probe pr(pl, SAXS_K_ADJ_Y, st);
+    const int R2 = Ny / (2 * yh), LW = saxs::kWG / R2;
+    const dim3 gay((p.n[0] * p.kz + LW - 1) / LW, (unsigned)(n_frames * p.E));
+    const size_t lq = saxs::adj_y_g_lds_bytes(p, 2 * yh);
+#define SAXS_AYG(HH) hipLaunchKernelGGL(saxs::k_adj_y_g<HH>, gay, dim3(saxs::kWG), lq, st, p, b)
+    switch (yh) {
+      case 10: SAXS_AYG(10); break;
+      case 9: SAXS_AYG(9); break;
+      case 8: SAXS_AYG(8); break;
+      case 7: SAXS_AYG(7); break;
+      case 6: SAXS_AYG(6); break;
+      default: SAXS_AYG(5); break;
+    }
+#undef SAXS_AYG
   } else {
     Probe pr(pl, SAXS_K_ADJ_Y, st);
     dim3 gy(p.n[0], p.nchy, n_frames * p.E);
