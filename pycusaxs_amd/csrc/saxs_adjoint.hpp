@@ -742,6 +742,206 @@ __global__ void __launch_bounds__(kWG) k_adj_y_g(Params p, Buffers b) {
   }
 }
 
+// ---- output-pruned x adjoint for any N = 2H R2 with n_x <= N/2 -----------
+//
+// k_adj_x over the same (jy, z chunk, frame) workgroups with k_xfft_g's
+// transforms: A = sum_e f_e F_e at the kept pairs m, N - m (k_xfft_g's two
+// stages per element, each thread holding its pairs), Abar = 2 mod w gH_b A
+// (the histogram's weights), then per element Ybar_e(x) =
+// conj(sum_m conj(f_e Abar(m)) W_N^{mx}) for x < n_x by the transposed
+// stages of k_adj_y_g (the kept values of each z line staged in LDS),
+// written over Y_e.  The plan guarantees what k_xfft_g needs.
+
+__host__ __device__ inline size_t adj_x_g_lds_bytes(const Params& p, int R1) {
+  const int N = p.nn[0];
+  return (size_t)p.kcx * yg_line_stride(R1, N / R1) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H>
+__global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers g) {
+  constexpr int R1 = 2 * H, YS = R1 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], N = p.nn[0], nb = p.num_bins, Kx = p.Kx, kx = p.kx;
+  const int R2 = N / R1, LY = yg_line_stride(R1, R2);
+  float2* lines = reinterpret_cast<float2*>(smem);         // stage-0 lines, then kept values
+  float2* tw = lines + (size_t)p.kcx * LY;                  // W_N^m
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = kept_to_index(jy, p.ky, p.Ky, p.nn[1]);
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const float inv_kc = 1.0f / (float)kc;
+  const int nitems = kc * (Kx + 1);                         // (pair m, line k), k fastest
+  float2 A[kXgItems][2];
+#pragma unroll
+  for (int u = 0; u < kXgItems; ++u) A[u][0] = A[u][1] = make_float2(0.0f, 0.0f);
+  __syncthreads();                                          // twiddles
+  // A = sum_e f_e F_e at the kept pairs (as k_xfft_g)
+  for (int e = 0; e < p.E; ++e) {
+    const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = div_small(item, kc, inv_kc), k = item - j * kc;
+      float2 xe[H], xo[H];
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int x = j + R2 * r;
+        xe[r] = x < nx ? src[(size_t)x * kz + k] : make_float2(0.0f, 0.0f);
+      }
+      xo[0] = xe[0];
+#pragma unroll
+      for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);
+      Radix<H>::dft(xe);
+      Radix<H>::dft(xo);
+      float2* dst = lines + k * LY + YS * j;
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        dst[2 * t] = xe[t];
+        dst[2 * t + 1] = xo[t];
+      }
+    }
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXgItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < nitems) {
+        const int m = div_small(item, kc, inv_kc), k = item - m * kc;
+        const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
+        const float2* yl = lines + k * LY;
+        float2 Z = make_float2(0.0f, 0.0f), Zm = make_float2(0.0f, 0.0f);
+        int idx = 0;
+        for (int j = 0; j < R2; ++j) {
+          const float2 w = tw[idx];
+          idx += m;
+          if (idx >= N) idx -= N;
+          const float2 y1 = yl[YS * j + s], y2 = yl[YS * j + sm];
+          Z.x += y1.x * w.x - y1.y * w.y;
+          Z.y += y1.x * w.y + y1.y * w.x;
+          Zm.x += y2.x * w.x + y2.y * w.y;
+          Zm.y += y2.y * w.x - y2.x * w.y;
+        }
+        const int mz = k0 + k;
+        const int n0 = shell_key(m, my, mz, p);
+        if (n0 < p.lut_len) {
+          const float fv = ff[n0];
+          A[u][0].x += fv * Z.x;
+          A[u][0].y += fv * Z.y;
+        }
+        if (m > 0) {
+          const int n1 = shell_key(-m, my, mz, p);
+          if (n1 < p.lut_len) {
+            const float fv = ff[n1];
+            A[u][1].x += fv * Zm.x;
+            A[u][1].y += fv * Zm.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Abar = 2 mod w gH_b A at in-range points, 0 elsewhere (k_adj_x)
+  int keys[kXgItems][2];
+#pragma unroll
+  for (int u = 0; u < kXgItems; ++u) {
+    const int item = threadIdx.x + u * blockDim.x;
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      float2 ab = make_float2(0.0f, 0.0f);
+      int nsq = 1 << 30;
+      if (item < nitems) {
+        const int m = div_small(item, kc, inv_kc), k = item - m * kc;
+        if (o == 0 || m > 0) {
+          const int ix = o == 0 ? m : N - m, mx = o == 0 ? m : -m, mz = k0 + k;
+          nsq = shell_key(mx, my, mz, p);
+          if (nsq < p.lut_len) {
+            const int bin = b.lut_bin[nsq];
+            if (bin >= 0 && bin < nb) {
+              const float mod = b.modx[ix] * b.mody[iy] * b.modz[mz];
+              const bool origin = (ix == 0) && (iy == 0) && (mz == 0);
+              const float sc = 2.0f * mod * (origin ? 1.0f : 2.0f) * (float)g.gH[(size_t)f * nb + bin];
+              ab = make_float2(sc * A[u][o].x, sc * A[u][o].y);
+            }
+          }
+        }
+      }
+      A[u][o] = ab;
+      keys[u][o] = nsq;
+    }
+  }
+  // per element: kept v(m) = conj(f_e Abar(m)) of every z line into LDS
+  // (line k, kept index jx as in ybuf's x order), then Ybar_e(x), x < n_x
+  float2* vbuf = lines;                                     // [kc][kx]
+  for (int e = 0; e < p.E; ++e) {
+    for (int i = threadIdx.x; i < kc * kx; i += blockDim.x) vbuf[i] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+#pragma unroll
+    for (int u = 0; u < kXgItems; ++u) {
+      const int item = threadIdx.x + u * blockDim.x;
+      if (item < nitems) {
+        const int m = div_small(item, kc, inv_kc), k = item - m * kc;
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+          if (keys[u][o] >= p.lut_len) continue;
+          const float fv = ff[keys[u][o]];
+          const int jx = o == 0 ? m : kx - m;
+          vbuf[k * kx + jx] = make_float2(fv * A[u][o].x, -fv * A[u][o].y);   // conj
+        }
+      }
+    }
+    __syncthreads();
+    float2* dst = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = div_small(item, kc, inv_kc), k = item - j * kc;
+      const float2* vl = vbuf + k * kx;                     // v(m) = vl[m], v(N - m) = vl[kx - m]
+      const int stepR = (R1 * j) % N;
+      float2 Y[R1];
+      int base = 0;
+#pragma unroll
+      for (int s = 0; s < R1; ++s) {
+        float2 acc = make_float2(0.0f, 0.0f);
+        int idx = base;
+        for (int m = s; m <= Kx; m += R1) {
+          const float2 w = tw[idx], v = vl[m];
+          acc.x += v.x * w.x - v.y * w.y;
+          acc.y += v.x * w.y + v.y * w.x;
+          idx += stepR;
+          if (idx >= N) idx -= N;
+        }
+        int m = s == 0 ? R1 : R1 - s;
+        int idh = stepR - base;
+        if (idh < 0) idh += N;
+        for (; m <= Kx; m += R1) {
+          const float2 w = tw[idh], v = vl[kx - m];
+          acc.x += v.x * w.x + v.y * w.y;
+          acc.y += v.y * w.x - v.x * w.y;
+          idh += stepR;
+          if (idh >= N) idh -= N;
+        }
+        Y[s] = acc;
+        base += j;
+        if (base >= N) base -= N;
+      }
+      float2 ev[H], od[H];
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        ev[t] = Y[2 * t];
+        od[t] = Y[2 * t + 1];
+      }
+      Radix<H>::dft(ev);
+      Radix<H>::dft(od);
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int x = j + R2 * r;
+        if (x >= nx) continue;
+        const float2 o = r == 0 ? cadd(ev[0], od[0]) : cadd(ev[r], cmul(od[r], tw[r * R2]));
+        dst[(size_t)x * kz + k] = make_float2(o.x, -o.y);
+      }
+    }
+    __syncthreads();                                        // vbuf rewritten next element
+  }
+}
+
 // Adjoint of spread_cols: for each staged atom, the in-tile part of
 // sum_stencil drho~ grad(th_x th_y th_z), with drho~ in columns of stride cs
 // whose z halo repeats z < P - 1 after nz.  The stencil is unrolled and

@@ -337,7 +337,7 @@ struct saxs_plan {
   bool seg16 = false;          // ... with 16-bit counters (segments <= kSeg16MaxLen atoms)
   int seg_kr = 0;              // keys per k_prep_seg pass (a frame's keys in <= 4 passes)
   bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
-  int x_gen_h = 0;             // forward x pass by k_xfft_g<x_gen_h> (0: Stockham)
+  int x_gen_h = 0;             // x pass by k_xfft_g<x_gen_h>, its adjoint by k_adj_x_g (0: Stockham)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
 };
 
@@ -389,9 +389,9 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
       p.kcx = (p.kz + p.nchx - 1) / p.nchx;
     }
   }
-  // generic pruned x pass (forward only; the adjoint keeps the Stockham
-  // transform over the same z chunks): n_x <= N/2, few kept m_x, an even
-  // R1 = 2H dividing N, every kept pair of a chunk in kXgItems per thread
+  // generic pruned x pass and its adjoint (over the same z chunks): n_x <=
+  // N/2, few kept m_x, an even R1 = 2H dividing N, every kept pair of a
+  // chunk in kXgItems per thread
   pl->x_gen_h = 0;
   if (pl->pruned_z && !pl->x_pruned && 2 * p.n[0] <= Nx && p.kx < Nx && p.Kx < Nx / 8 &&
       p.kcx * (p.Kx + 1) <= saxs::kXgItems * saxs::kWG) {
@@ -1107,6 +1107,18 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } while (0)
       if (Nx == 128) SAXS_AXP(128); else if (Nx == 256) SAXS_AXP(256); else SAXS_AXP(512);
 #undef SAXS_AXP
+    } else if (pl->x_gen_h) {
+      const size_t lq = saxs::adj_x_g_lds_bytes(p, 2 * pl->x_gen_h);
+#define SAXS_AXG(HH) hipLaunchKernelGGL(saxs::k_adj_x_g<HH>, gx, dim3(saxs::kWG), lq, st, p, b, g)
+      switch (pl->x_gen_h) {
+        case 10: SAXS_AXG(10); break;
+        case 9: SAXS_AXG(9); break;
+        case 8: SAXS_AXG(8); break;
+        case 7: SAXS_AXG(7); break;
+        case 6: SAXS_AXG(6); break;
+        default: SAXS_AXG(5); break;
+      }
+#undef SAXS_AXG
     } else {
 #define SAXS_AX(RS) hipLaunchKernelGGL(saxs::k_adj_x<RS>, gx, dim3(saxs::kWG), saxs::adjx_lds_bytes(p), st, p, b, g)
       if (px == 1) SAXS_AX(1); else if (px == 0) SAXS_AX(0); else SAXS_AX(2);
