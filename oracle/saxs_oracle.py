@@ -22,6 +22,7 @@ where they decide integers (grid cell of an atom, FFT size, bin parameters):
                                       opsfact.cu:29-35
           modulus (S5)                saxsDeviceKernels.cu:142-162
           histogram (S6)              saxsDeviceKernels.cu:67-125
+  NVT     spectrum sum, one binning   saxsKernel.cu:17-39,184-189, RunSaxs.cu:124-130,208-211
   output  rows (b*dq, H_b/C_b)        saxsKernel.cu:210-225, RunSaxs.cu:224-227
 
 Where the reference's float32 rounding at exact bin edges is undefined
@@ -481,13 +482,9 @@ def half_spectrum_m(nn):
     return out
 
 
-def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=True):
-    """Per-element F_e over the half spectrum plus the in-range mask and q.
-
-    literal=True embeds exactly like the reference (pad everywhere outside
-    the cell, S3); literal=False transforms (rho - pad) inside the cell with
-    zero padding, which differs only at m = 0."""
-    oc = inverse_box_matrix(box)
+def spectrum_bins(plan: OraclePlan):
+    """Bin of every half-spectrum point (-1 outside q <= qcut or from
+    num_bins on), the in-range mask and |q|, for the plan's box."""
     mx, my, mz = half_spectrum_m(plan.nn)
     bv = plan.basis.astype(np.float64)
     qv = (mx[:, None, None, None] * bv[0] + my[None, :, None, None] * bv[1]
@@ -497,8 +494,17 @@ def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=
     cand = q2 <= float(plan.hp.qcut) ** 2 * (1.0 + 1e-9)
     ii, jj, kk = np.nonzero(cand)
     bins[cand] = plan.bins_for(np.stack([mx[ii], my[jj], mz[kk]], axis=1))
-    inr = bins >= 0
-    q = np.sqrt(q2)
+    return bins, bins >= 0, np.sqrt(q2)
+
+
+def frame_spectrum_terms(plan: OraclePlan, pos, elem_ids, symbols, box, literal=True):
+    """Per-element F_e over the half spectrum plus the in-range mask and q.
+
+    literal=True embeds exactly like the reference (pad everywhere outside
+    the cell, S3); literal=False transforms (rho - pad) inside the cell with
+    zero padding, which differs only at m = 0."""
+    oc = inverse_box_matrix(box)
+    bins, inr, q = spectrum_bins(plan)
     Fs = []
     for e, sym in enumerate(symbols):
         sel = np.asarray(elem_ids) == e
@@ -582,6 +588,21 @@ def trajectory_histogram(plan, frames, elem_ids, symbols, literal=True):
         H += h
         C += c
     return H, C
+
+
+def trajectory_histogram_nvt(plan0, frames, elem_ids, symbols, plan_of_box, literal=True):
+    """NVT mode: every frame's intensity, with its own box's q cut and form
+    factors (plan_of_box(box)), is summed over the half spectrum
+    (gridAddKernel, saxsKernel.cu:184-189), and the sum is binned once with
+    the first box's q (plan0; RunSaxs.cu:124-130,208-211), every value times
+    1/F (calculate_histogram with frames_fact, saxsKernel.cu:17-39 and
+    saxsDeviceKernels.cu:9-66).  Returns (H, C) of that one binning."""
+    Iacc = None
+    for pos, box in frames:
+        I, _, _ = frame_intensity(plan_of_box(box), pos, elem_ids, symbols, box, literal)
+        Iacc = I if Iacc is None else Iacc + I
+    bins, inr, _ = spectrum_bins(plan0)
+    return histogram(plan0, Iacc / float(len(frames)), inr, bins)
 
 
 # ------------------------------------------------------------- adjoint ---

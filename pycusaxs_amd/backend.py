@@ -186,6 +186,10 @@ def run(obj_topology, topology: str, trajectory: str, grid: Sequence[int],
     sigma = float(np.float32(scale_factor))
     out_path = output or DEFAULT_OUTPUT
     sim = simulation or DEFAULT_SIMULATION
+    if sim not in ("npt", "nvt"):
+        # the reference accepts any string and then bins nothing
+        # (saxsKernel.cu:184-196), writing an empty saxs.dat
+        raise ValueError(f"simulation must be 'npt' or 'nvt', got {simulation!r}")
     ions = {"Na": int(sodium), "Cl": int(chlorine)}
     summary = summary_text(topology, trajectory, grid3, scaled3, int(begin), int(end), int(stride),
                            int(order), sigma, float(bin_size), float(qcut), water_model, ions,
@@ -235,11 +239,15 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     gpu_ms = 0.0
     n_done = 0
 
+    # NVT bins the summed spectrum once with the first frame's box
+    # (RunSaxs.cu:124-130,208-211): every frame is binned by box0's q
+    bin_box = box0 if sim == "nvt" else None
+
     def flush():
         nonlocal gpu_ms
         t0 = time.perf_counter()
         try:
-            stager.launch(eng, ids_t)
+            stager.launch(eng, ids_t, bin_box)
         except (ValueError, RuntimeError) as exc:
             # RunSaxs.cu:195-199 logs a failing frame and goes on
             print(f"Error processing frames: {exc}", file=sys.stderr)
@@ -256,7 +264,7 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     if stager.count:
         flush()
     torch.cuda.synchronize(dev)
-    rows = eng.rows()
+    rows = eng.nvt_rows(box0, eng.frames_done) if sim == "nvt" else eng.rows()
     with open(out_path, "w") as fh:
         fh.write(format_rows(rows))
     n_frames = len(range(begin, end + 1, stride))
@@ -318,7 +326,7 @@ class FrameStager:
         self.count += 1
         self.box = box
 
-    def launch(self, eng, ids_t):
+    def launch(self, eng, ids_t, bin_box=None):
         import torch
         k, s = self.count, self.slot
         self.count = 0
@@ -336,7 +344,7 @@ class FrameStager:
             xyz = self.sel[:k]
             if self.n_atoms:
                 torch.index_select(self.dev[s][:k], 1, self.idx_t, out=xyz)
-            eng.forward(xyz, ids_t, box=self.box)
+            eng.forward(xyz, ids_t, box=self.box, bin_box=bin_box)
         finally:
             done = torch.cuda.Event()
             done.record(compute)

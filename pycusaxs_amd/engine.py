@@ -17,9 +17,11 @@ import torch
 
 from . import _lib
 from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_points,
-                       form_factor_table, form_factor_vectors, hist_params, inverse_box_matrix,
-                       is_triclinic, point_bins, point_kmax, reciprocal_basis, shell_bins,
-                       triclinic_bins, triclinic_kmax)
+                       form_factor_table, form_factor_vectors, half_spectrum_counts, hist_params,
+                       inverse_box_matrix, is_triclinic, point_bins, point_kmax, reciprocal_basis,
+                       shell_bins, triclinic_bins, triclinic_kmax)
+
+_NO_BIN_LIMIT = 2 ** 31 - 1     # bins past num_bins still mark q <= qcut
 
 
 @dataclass
@@ -135,19 +137,69 @@ class SaxsEngine:
         return (np.ascontiguousarray(bins, dtype=np.int32),
                 np.ascontiguousarray(ff, dtype=np.float32), kmax)
 
-    def set_box(self, box):
-        """Rebuilds the shell tables for a new box (NPT).  dq, qcut and
-        num_bins stay those of the first frame, as in the reference."""
-        cell = inverse_box_matrix(box)
+    def _q_tables_binned(self, h: Histogram, cell, h0: Histogram, cell0):
+        """Tables for a frame in box `cell` binned by the spacings of box
+        `cell0` (NVT: the reference sums every frame's spectrum and bins the
+        sum once with the first box, saxsKernel.cu:184-189 and RunSaxs.cu:
+        124-130,208-211).  The kept range and the form factors are the
+        frame's own: its amplitude is zero where its own q exceeds qcut
+        (saxsDeviceKernels.cu:177-216), so those points carry f = 0."""
+        nn, syms = self.spec.nn, self.spec.symbols
+        s, s0 = h.spacing, h0.spacing
+        if is_triclinic(cell) or is_triclinic(cell0):
+            if not is_triclinic(cell):
+                raise ValueError("NVT: a frame with a rectangular box after a triclinic first "
+                                 "box is not supported")
+            basis = reciprocal_basis(cell, self.spec.n, nn)
+            kmax = triclinic_kmax(basis, h.qcut, nn)
+            bins = triclinic_bins(reciprocal_basis(cell0, self.spec.n, nn), h.dq, h.qcut,
+                                  h.num_bins, kmax)
+            keep = triclinic_bins(basis, h.dq, h.qcut, _NO_BIN_LIMIT, kmax) >= 0
+            ff = form_factor_vectors(syms, basis, kmax) * keep
+        elif s[0] == s[1] == s[2] and s0[0] == s0[1] == s0[2]:
+            n_shells = len(shell_bins(s[0], h.dq, h.qcut, h.num_bins))
+            b0 = shell_bins(s0[0], h.dq, h.qcut, h.num_bins)[:n_shells]
+            bins = np.full(n_shells, -1, dtype=np.int32)
+            bins[:len(b0)] = b0
+            ff = form_factor_table(syms, s[0], n_shells)
+            kmax = None
+        else:
+            kmax = point_kmax(s, h.qcut)
+            kmax = tuple(min(k, n // 2) for k, n in zip(kmax, nn))
+            bins = point_bins(s0, h.dq, h.qcut, h.num_bins, kmax)
+            keep = point_bins(s, h.dq, h.qcut, _NO_BIN_LIMIT, kmax) >= 0
+            ff = form_factor_points(syms, s, kmax) * keep
+        return (np.ascontiguousarray(bins, dtype=np.int32),
+                np.ascontiguousarray(ff, dtype=np.float32), kmax)
+
+    def _hist_for(self, cell) -> Histogram:
+        """This run's dq, qcut and num_bins (those of the first frame, as in
+        the reference) with the reciprocal spacings of `cell`."""
         oc = np.ascontiguousarray(np.diag(cell), dtype=np.float32)
-        key = tuple(float(v) for v in cell.ravel())
-        if key == self._box_key:
-            return oc
         h = hist_params(self.spec.n, self.spec.nn, oc, self.spec.sigma, self.spec.dq_req,
                         self.spec.qcut_req, self.spec.dq_policy)
-        h = Histogram(self.hist.dq, self.hist.qcut, self.hist.num_bins, h.spacing,
-                      self.hist.dq_reset, self.hist.qcut_reset)
-        bins, ff, kmax = self._q_tables(h, cell)
+        return Histogram(self.hist.dq, self.hist.qcut, self.hist.num_bins, h.spacing,
+                         self.hist.dq_reset, self.hist.qcut_reset)
+
+    def set_box(self, box, bin_box=None):
+        """Rebuilds the q tables for a new box (NPT).  dq, qcut and num_bins
+        stay those of the first frame, as in the reference.  With `bin_box`
+        the frame's points are binned by that box's q instead (NVT)."""
+        cell = inverse_box_matrix(box)
+        oc = np.ascontiguousarray(np.diag(cell), dtype=np.float32)
+        cell0 = None if bin_box is None else inverse_box_matrix(bin_box)
+        if cell0 is not None and np.array_equal(cell0, cell):
+            cell0 = None
+        key = tuple(float(v) for v in cell.ravel())
+        if cell0 is not None:
+            key += tuple(float(v) for v in cell0.ravel())
+        if key == self._box_key:
+            return oc
+        h = self._hist_for(cell)
+        if cell0 is None:
+            bins, ff, kmax = self._q_tables(h, cell)
+        else:
+            bins, ff, kmax = self._q_tables_binned(h, cell, self._hist_for(cell0), cell0)
         torch.cuda.current_stream().synchronize()
         if kmax is None:
             _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
@@ -199,11 +251,12 @@ class SaxsEngine:
             raise ValueError("element ids must be [atoms] matching the positions")
         return xyz, elem
 
-    def forward(self, xyz: torch.Tensor, elem: torch.Tensor, box=None):
-        """Accumulates frames into self.H / self.C (asynchronous)."""
+    def forward(self, xyz: torch.Tensor, elem: torch.Tensor, box=None, bin_box=None):
+        """Accumulates frames into self.H / self.C (asynchronous).  With
+        `bin_box` (NVT) the frames' points are binned by that box's q."""
         self._last = None
         xyz, elem = self._prep(xyz, elem)
-        oc = self.set_box(box) if box is not None else self.oc
+        oc = self.set_box(box, bin_box) if box is not None else self.oc
         F = xyz.shape[0]
         for f0 in range(0, F, self.max_frames):
             chunk = xyz[f0:f0 + self.max_frames]
@@ -303,6 +356,25 @@ class SaxsEngine:
         (saxsKernel.cu:210-225); q is the float32 product b*dq."""
         H = self.H.cpu().numpy()
         Cn = self.C.cpu().numpy()
+        dq = np.float32(self.hist.dq)
+        return [(float(np.float32(np.float32(b) * dq)), float(H[b] / Cn[b]))
+                for b in range(1, len(H)) if Cn[b] != 0.0]
+
+    def counts(self, box) -> np.ndarray:
+        """C_b of one binning of the half spectrum with the q tables of `box`
+        (host arithmetic on the same tables the kernels read)."""
+        cell = inverse_box_matrix(box)
+        bins, _, kmax = self._q_tables(self._hist_for(cell), cell)
+        return half_spectrum_counts(self.spec.nn, bins, kmax, self.hist.num_bins,
+                                    signed=is_triclinic(cell))
+
+    def nvt_rows(self, box0, frames: int):
+        """NVT output rows (saxsKernel.cu:17-39,184-189 and :210-225): the
+        spectrum summed over `frames` frames, binned by box0's q (every frame
+        was run with bin_box=box0), times 1/frames, over the counts of one
+        binning with box0."""
+        H = self.H.cpu().numpy() / float(max(frames, 1))
+        Cn = self.counts(box0)
         dq = np.float32(self.hist.dq)
         return [(float(np.float32(np.float32(b) * dq)), float(H[b] / Cn[b]))
                 for b in range(1, len(H)) if Cn[b] != 0.0]

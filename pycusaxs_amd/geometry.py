@@ -11,6 +11,7 @@ the engine's C ABI as plain arrays.
   shell_bins           exact bin of every shell n = |m|^2
   point_bins           exact bin of every point of an orthorhombic grid
   triclinic_bins       exact bin of every signed point of a triclinic grid
+  half_spectrum_counts  the counts calculate_histogram adds (saxsDeviceKernels.cu:67-125)
   bspline_moduli       BSpmod.cpp:45-211
   form_factor_table    opsfact.cu:29-35 on the shell q values
   explicit_solvent_pads  AtomCounter.cpp:56-103 (padding::given, Options.h:7-11)
@@ -310,6 +311,45 @@ def form_factor_points(symbols: Sequence[str], spacing: Sequence[float],
         out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
                   + a4 * np.exp(-b4 * s2) + c)
     return out
+
+
+def half_spectrum_m(nn: Sequence[int]):
+    """Signed frequencies of the R2C half spectrum per axis: index i maps to
+    i below nf = nn/2 (even) or nn/2 + 1 (odd), else i - nn; z runs over
+    0..nn_z/2 (saxsDeviceKernels.cu:75-83)."""
+    out = []
+    for a, N in enumerate(nn):
+        N = int(N)
+        i = np.arange(N if a < 2 else N // 2 + 1, dtype=np.int64)
+        nf = N // 2 if N % 2 == 0 else N // 2 + 1
+        out.append(np.where(i < nf, i, i - N))
+    return out
+
+
+def half_spectrum_counts(nn: Sequence[int], bins: np.ndarray, kmax, num_bins: int,
+                         signed: bool = False) -> np.ndarray:
+    """C_b of one binning of the half spectrum with the tables (bins, kmax)
+    of the engine's q tables: every point with a bin adds 2, the origin 1
+    (saxsDeviceKernels.cu:100-121).  kmax None means shell keys n = |m|^2;
+    otherwise point keys (|m_x|, |m_y|, |m_z|), or signed m when `signed`."""
+    bins = np.asarray(bins, dtype=np.int64)
+    if kmax is None:
+        K = (math.isqrt(len(bins) - 1),) * 3
+    else:
+        K = tuple(int(k) for k in kmax)
+    mx, my, mz = (m[np.abs(m) <= k] for m, k in zip(half_spectrum_m(nn), K))
+    X, Y, Z = mx[:, None, None], my[None, :, None], mz[None, None, :]
+    if kmax is None:
+        key = X * X + Y * Y + Z * Z
+        b = np.where(key < len(bins), bins[np.minimum(key, len(bins) - 1)], -1)
+    elif signed:
+        b = bins[((X + K[0]) * (2 * K[1] + 1) + Y + K[1]) * (2 * K[2] + 1) + Z + K[2]]
+    else:
+        b = bins[(np.abs(X) * (K[1] + 1) + np.abs(Y)) * (K[2] + 1) + np.abs(Z)]
+    w = np.full(b.shape, 2.0)
+    w[(X == 0) & (Y == 0) & (Z == 0)] = 1.0
+    ok = b >= 0
+    return np.bincount(b[ok], weights=w[ok], minlength=num_bins)[:num_bins]
 
 
 def bspline_moduli(N: int, order: int) -> np.ndarray:

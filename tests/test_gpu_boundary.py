@@ -63,6 +63,53 @@ def test_run_with_scaled_grid_and_nvt(gpu, tmp_path):
         assert abs(ig - iw) <= 2e-5 * abs(iw)
 
 
+TRIC_A = np.array([[37.0, 0.0, 0.0], [3.0, 36.5, 0.0], [-2.0, 1.5, 37.5]], dtype=np.float32)
+TRIC_B = np.array([[37.3, 0.0, 0.0], [2.5, 36.8, 0.0], [-1.5, 2.0, 37.2]], dtype=np.float32)
+NVT_BOXES = {
+    "cubic": [np.diag([37.0] * 3), np.diag([37.6] * 3), np.diag([36.5] * 3), np.diag([37.2] * 3)],
+    "orthorhombic": [np.diag([37.0] * 3), np.diag([37.5, 36.8, 37.3]), np.diag([37.0] * 3),
+                     np.diag([36.9, 37.4, 37.0])],
+    "triclinic": [np.diag([37.0] * 3), TRIC_A, np.diag([37.3] * 3), TRIC_B],
+}
+
+
+@pytest.mark.parametrize("kind", sorted(NVT_BOXES))
+def test_run_nvt_changing_box(gpu, tmp_path, kind):
+    """NVT with a box that changes between frames: every frame's spectrum
+    (its own q cut and form factors) is summed and binned once with the
+    first frame's box, times 1/F, against the oracle's restatement."""
+    boxes = [np.asarray(b, dtype=np.float32) for b in NVT_BOXES[kind]]
+    base, syms = O.synthetic_system(3000, 37.0, seed=23)
+    frames = [O.frame_positions(base, 23, f, 0.5) for f in range(4)]
+    topo = FakeTopology(frames, syms, boxes[0])
+    topo.boxes = boxes
+    out = tmp_path / "nvt.dat"
+    pycusaxs_cuda.run(topo, "sys.tpr", "traj.xtc", [48], end=3, output=str(out),
+                      scale_factor=2.0, bin_size=0.02, qcut=0.8, simulation="nvt")
+    symbols, ids = O.element_ids(syms)
+    plan0 = O.make_plan((48,) * 3, boxes[0], 4, 2.0, None, 0.02, 0.8)
+
+    def plan_of(box):
+        p = O.make_plan((48,) * 3, box, 4, 2.0, None, 0.02, 0.8)
+        p.hp = O.HistParams(plan0.hp.dq, plan0.hp.qcut, plan0.hp.num_bins, p.hp.spacing)
+        return p
+
+    H, C = O.trajectory_histogram_nvt(plan0, list(zip(frames, boxes)), ids, symbols, plan_of)
+    want = O.saxs_rows(plan0, H, C)
+    got = parse(out)
+    assert len(got) == len(want) and len(got) >= 8
+    for (qg, ig), (qw, iw) in zip(got, want):
+        assert ("%10.5f" % qg) == ("%10.5f" % qw)
+        assert abs(ig - iw) <= 1e-5 * abs(iw) + 1e-5 * 10 ** np.floor(np.log10(abs(iw)))
+
+
+def test_run_rejects_unknown_ensemble(tmp_path):
+    topo = FakeTopology([np.zeros((3, 3), np.float32)], ["O", "H", "H"], np.diag([30.0] * 3))
+    with pytest.raises(ValueError, match="simulation"):
+        pycusaxs_cuda.run(topo, "a", "b", [16], output=str(tmp_path / "x.dat"),
+                          scale_factor=2.0, simulation="NVT")
+
+
 def test_run_given_padding(gpu, tmp_path):
     """padding="given" (keyword-only extension): explicit SPCE water plus the
     sodium/chlorine counts as the bulk-solvent pad, against the oracle with
