@@ -148,35 +148,66 @@ def triclinic_kmax(basis, qcut: float, nn: Sequence[int]) -> Tuple[int, int, int
     return tuple(out)
 
 
+def _bins_filtered(q2, err, dq: float, qcut: float, num_bins: int, exact) -> np.ndarray:
+    """Bins of points whose squared magnitudes q2 are known in float64 to
+    within err: max{b : b^2 dq^2 <= q2}, -1 outside q2 <= qcut^2 or from
+    num_bins on.  A point whose q2 lies within 1e-9 q2 + err of qcut^2 or of a
+    bin edge b^2 dq^2 is decided by exact(i) in integer arithmetic, so the
+    result equals the exact rule everywhere."""
+    dq, qcut = float(dq), float(qcut)
+    dq2, qc2 = dq * dq, qcut * qcut
+    tol = 1e-9 * q2 + err
+    t = q2 / dq2
+    b = np.floor(np.sqrt(t))
+    amb = ((np.abs(q2 - qc2) <= tol) | ((t - b * b) * dq2 <= tol)
+           | (((b + 1.0) ** 2 - t) * dq2 <= tol))
+    bi = b.astype(np.int64)
+    out = np.where((q2 <= qc2) & (bi < num_bins), bi, -1)
+    for i in np.flatnonzero(amb):
+        out[i] = exact(int(i))
+    return out.astype(np.int32)
+
+
 def triclinic_bins(basis, dq: float, qcut: float, num_bins: int,
                    kmax: Sequence[int]) -> np.ndarray:
     """bin of every signed point m of the box |m_a| <= kmax[a] (layout
     ((m_x+K_x)(2K_y+1) + m_y+K_y)(2K_z+1) + m_z+K_z): q = sum_a m_a b_a with
     the float32 rows as exact integers over one denominator,
-    bin = max{b : b^2 dq^2 <= |q|^2}, -1 outside |q| <= qcut or from num_bins."""
+    bin = max{b : b^2 dq^2 <= |q|^2}, -1 outside |q| <= qcut or from num_bins.
+    Evaluated in float64 with an error bound; points near an edge are
+    decided exactly in integers."""
     B, D = _basis_integers(basis)
     dn, dd = float(dq).as_integer_ratio()
     qn, qd = float(qcut).as_integer_ratio()
     kx, ky, kz = (int(v) for v in kmax)
-    out = np.empty((2 * kx + 1) * (2 * ky + 1) * (2 * kz + 1), dtype=np.int32)
+    shape = (2 * kx + 1, 2 * ky + 1, 2 * kz + 1)
+    size = shape[0] * shape[1] * shape[2]
+    if dn <= 0:
+        return np.full(size, -1, dtype=np.int32)
     lim = qn * qn * D * D                          # |q D|^2 qd^2 <= qn^2 D^2
     den = dn * dn * D * D
-    i = 0
-    for mx in range(-kx, kx + 1):
-        for my in range(-ky, ky + 1):
-            base = [mx * B[0][j] + my * B[1][j] for j in range(3)]
-            for mz in range(-kz, kz + 1):
-                s2 = 0
-                for j in range(3):
-                    v = base[j] + mz * B[2][j]
-                    s2 += v * v                    # |q|^2 D^2
-                if dn <= 0 or s2 * qd * qd > lim:
-                    out[i] = -1
-                else:
-                    bb = math.isqrt((s2 * dd * dd) // den)
-                    out[i] = bb if bb < num_bins else -1
-                i += 1
-    return out
+
+    def exact(i):
+        a, b, c = np.unravel_index(i, shape)
+        m = (int(a) - kx, int(b) - ky, int(c) - kz)
+        s2 = 0
+        for j in range(3):
+            v = m[0] * B[0][j] + m[1] * B[1][j] + m[2] * B[2][j]
+            s2 += v * v                            # |q|^2 D^2
+        if s2 * qd * qd > lim:
+            return -1
+        bb = math.isqrt((s2 * dd * dd) // den)
+        return bb if bb < num_bins else -1
+
+    bv = np.asarray(basis, dtype=np.float64)
+    m = [np.arange(-k, k + 1, dtype=np.float64) for k in (kx, ky, kz)]
+    terms = [m[a][(slice(None) if a == 0 else None), (slice(None) if a == 1 else None),
+                  (slice(None) if a == 2 else None), None] * bv[a] for a in range(3)]
+    qv = terms[0] + terms[1] + terms[2]
+    S = np.abs(terms[0]) + np.abs(terms[1]) + np.abs(terms[2])
+    q2 = (qv ** 2).sum(axis=3).reshape(-1)
+    err = 1e-12 * (S ** 2).sum(axis=3).reshape(-1)
+    return _bins_filtered(q2, err, dq, qcut, num_bins, exact)
 
 
 def form_factor_vectors(symbols: Sequence[str], basis, kmax: Sequence[int]) -> np.ndarray:
@@ -271,8 +302,9 @@ def point_bins(spacing: Sequence[float], dq: float, qcut: float, num_bins: int,
     """bin of every point (|m_x|, |m_y|, |m_z|) of the box |m_a| <= kmax[a], laid
     out as (|m_x| (kmax_y+1) + |m_y|) (kmax_z+1) + |m_z|, for an orthorhombic
     reciprocal grid: q^2 = sum_a (m_a s_a)^2, bin = max{b : b^2 dq^2 <= q^2},
-    -1 outside q <= qcut or from num_bins on.  Exact in integers: the float
-    values are expanded to integer ratios over one common denominator."""
+    -1 outside q <= qcut or from num_bins on.  Evaluated in float64 (each
+    m_a s_a is exact there); points near an edge are decided exactly with the
+    float values expanded to integer ratios over one common denominator."""
     sr = [float(v).as_integer_ratio() for v in spacing]
     dn, dd = float(dq).as_integer_ratio()
     qn, qd = float(qcut).as_integer_ratio()
@@ -281,20 +313,22 @@ def point_bins(spacing: Sequence[float], dq: float, qcut: float, num_bins: int,
         den = den * sd * sd // math.gcd(den, sd * sd)
     w = [sn * sn * (den // (sd * sd)) for sn, sd in sr]     # q^2 * den = sum m_a^2 w_a
     kx, ky, kz = (int(v) for v in kmax)
-    out = np.empty((kx + 1) * (ky + 1) * (kz + 1), dtype=np.int32)
-    i = 0
-    for ax in range(kx + 1):
-        for ay in range(ky + 1):
-            base = ax * ax * w[0] + ay * ay * w[1]
-            for az in range(kz + 1):
-                q2 = base + az * az * w[2]                   # q^2 = q2 / den
-                if dn <= 0 or q2 * qd * qd > qn * qn * den:
-                    out[i] = -1
-                else:
-                    b = math.isqrt((q2 * dd * dd) // (dn * dn * den))
-                    out[i] = b if b < num_bins else -1
-                i += 1
-    return out
+    shape = (kx + 1, ky + 1, kz + 1)
+    if dn <= 0:
+        return np.full(shape[0] * shape[1] * shape[2], -1, dtype=np.int32)
+
+    def exact(i):
+        ax, ay, az = (int(v) for v in np.unravel_index(i, shape))
+        q2 = ax * ax * w[0] + ay * ay * w[1] + az * az * w[2]    # q^2 = q2 / den
+        if q2 * qd * qd > qn * qn * den:
+            return -1
+        b = math.isqrt((q2 * dd * dd) // (dn * dn * den))
+        return b if b < num_bins else -1
+
+    ax = [np.arange(k, dtype=np.float64) * float(v) for k, v in zip(shape, spacing)]
+    q2 = (ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2
+          + ax[2][None, None, :] ** 2).reshape(-1)
+    return _bins_filtered(q2, 1e-12 * q2, dq, qcut, num_bins, exact)
 
 
 def form_factor_points(symbols: Sequence[str], spacing: Sequence[float],

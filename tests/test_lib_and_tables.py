@@ -194,10 +194,13 @@ def test_unknown_water_model_fails_like_reference():
 
 @pytest.mark.parametrize("Ls,n,dq,qcut,policy", [((40.0, 50.0, 60.0), 32, 0.0, 0.9, "reference"),
                                                  ((33.0, 40.0, 37.0), 30, 0.0, 1.2, "reference"),
-                                                 ((60.0, 70.0, 80.0), 64, 0.01, 0.6, "exact")])
+                                                 ((60.0, 70.0, 80.0), 64, 0.01, 0.6, "exact"),
+                                                 ((40.0, 40.0, 20.0), 32, 0.0, 0.9, "reference")])
 def test_point_bins_match_oracle(Ls, n, dq, qcut, policy):
-    """Point-keyed tables of orthorhombic cells (engine host code, integer
-    arithmetic) against the oracle's exact rational rule, point by point."""
+    """Point-keyed tables of orthorhombic cells (engine host code, float64
+    with an exact integer decision near edges) against the oracle's exact
+    rational rule, point by point.  The 40:40:20 cell puts many points exactly
+    on bin edges (dq is twice the two equal spacings)."""
     oc = G.inverse_box_diag(np.diag(Ls))
     nn = (2 * n,) * 3
     h = G.hist_params((n,) * 3, nn, oc, 2.0, dq, qcut, policy)
