@@ -124,7 +124,7 @@ class SaxsEngine:
             basis = reciprocal_basis(cell, self.spec.n, self.spec.nn)
             kmax = triclinic_kmax(basis, hist.qcut, self.spec.nn)
             bins = triclinic_bins(basis, hist.dq, hist.qcut, hist.num_bins, kmax)
-            ff = form_factor_vectors(self.spec.symbols, basis, kmax)
+            ff = form_factor_vectors(self.spec.symbols, basis, kmax, bins >= 0)
         elif s[0] == s[1] == s[2]:
             bins = shell_bins(s[0], hist.dq, hist.qcut, hist.num_bins)
             ff = form_factor_table(self.spec.symbols, s[0], len(bins))
@@ -155,7 +155,7 @@ class SaxsEngine:
             bins = triclinic_bins(reciprocal_basis(cell0, self.spec.n, nn), h.dq, h.qcut,
                                   h.num_bins, kmax)
             keep = triclinic_bins(basis, h.dq, h.qcut, _NO_BIN_LIMIT, kmax) >= 0
-            ff = form_factor_vectors(syms, basis, kmax) * keep
+            ff = form_factor_vectors(syms, basis, kmax, keep)
         elif s[0] == s[1] == s[2] and s0[0] == s0[1] == s0[2]:
             n_shells = len(shell_bins(s[0], h.dq, h.qcut, h.num_bins))
             b0 = shell_bins(s0[0], h.dq, h.qcut, h.num_bins)[:n_shells]

@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Dict, Mapping, Sequence, Tuple
+from typing import Dict, Mapping, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -199,30 +199,44 @@ def triclinic_bins(basis, dq: float, qcut: float, num_bins: int,
         bb = math.isqrt((s2 * dd * dd) // den)
         return bb if bb < num_bins else -1
 
-    bv = np.asarray(basis, dtype=np.float64)
-    m = [np.arange(-k, k + 1, dtype=np.float64) for k in (kx, ky, kz)]
-    terms = [m[a][(slice(None) if a == 0 else None), (slice(None) if a == 1 else None),
-                  (slice(None) if a == 2 else None), None] * bv[a] for a in range(3)]
-    qv = terms[0] + terms[1] + terms[2]
-    S = np.abs(terms[0]) + np.abs(terms[1]) + np.abs(terms[2])
-    q2 = (qv ** 2).sum(axis=3).reshape(-1)
-    err = 1e-12 * (S ** 2).sum(axis=3).reshape(-1)
+    q2, err = _triclinic_q2(basis, kmax)
     return _bins_filtered(q2, err, dq, qcut, num_bins, exact)
 
 
-def form_factor_vectors(symbols: Sequence[str], basis, kmax: Sequence[int]) -> np.ndarray:
-    """f_e(|q|), q = sum_a m_a b_a, over the signed box in triclinic_bins order."""
+def _quadratic_form(G, kmax, signed: bool = True):
+    """sum_ab G_ab m_a m_b over the signed box |m_a| <= kmax[a] (or over
+    |m_a| when not signed), flattened in triclinic_bins order."""
+    m = [np.arange(-int(k), int(k) + 1, dtype=np.float64) for k in kmax]
+    if not signed:
+        m = [np.abs(v) for v in m]
+    X, Y, Z = m[0][:, None, None], m[1][None, :, None], m[2][None, None, :]
+    out = (G[0, 0] * X * X + G[1, 1] * Y * Y) + G[2, 2] * Z * Z
+    out = out + (2.0 * G[0, 1] * X * Y + 2.0 * G[0, 2] * X * Z) + 2.0 * G[1, 2] * Y * Z
+    return out.reshape(-1)
+
+
+def _triclinic_q2(basis, kmax):
+    """|q|^2 = m^T (b b^T) m over the signed box, and a bound on its float64
+    error: 1e-12 sum_ab (|b| |b|^T)_ab |m_a| |m_b|, a thousand times the
+    rounding of the Gram matrix and of the form (a few 1e-16 of that sum)."""
     b = np.asarray(basis, dtype=np.float64)
-    ax = [np.arange(-int(k), int(k) + 1, dtype=np.float64) for k in kmax]
-    qv = (ax[0][:, None, None, None] * b[0] + ax[1][None, :, None, None] * b[1]
-          + ax[2][None, None, :, None] * b[2])
-    q = np.sqrt((qv ** 2).sum(axis=3)).reshape(-1)
-    s2 = (q / (4.0 * np.pi)) ** 2
-    out = np.empty((len(symbols), len(q)))
+    ab = np.abs(b)
+    return _quadratic_form(b @ b.T, kmax), 1e-12 * _quadratic_form(ab @ ab.T, kmax, signed=False)
+
+
+def form_factor_vectors(symbols: Sequence[str], basis, kmax: Sequence[int],
+                        mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """f_e(|q|), q = sum_a m_a b_a, over the signed box in triclinic_bins order;
+    with `mask`, only at the masked points (0 elsewhere)."""
+    q2, _ = _triclinic_q2(basis, kmax)
+    q = np.sqrt(np.maximum(q2, 0.0))
+    sel = np.ones(len(q), dtype=bool) if mask is None else np.asarray(mask, dtype=bool)
+    s2 = (q[sel] / (4.0 * np.pi)) ** 2
+    out = np.zeros((len(symbols), len(q)))
     for e, sym in enumerate(symbols):
         a1, a2, a3, a4, b1, b2, b3, b4, c = coefficients(sym)
-        out[e] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
-                  + a4 * np.exp(-b4 * s2) + c)
+        out[e, sel] = (a1 * np.exp(-b1 * s2) + a2 * np.exp(-b2 * s2) + a3 * np.exp(-b3 * s2)
+                       + a4 * np.exp(-b4 * s2) + c)
     return out
 
 
