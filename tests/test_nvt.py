@@ -2,7 +2,10 @@
 frames, binned once with the first frame's box, times 1/F) pinned by two
 closed forms, and the host count table the engine's NVT rows divide by
 (geometry.half_spectrum_counts) against the oracle's counts, which
-test_oracle_pins.py pins by enumeration.
+test_oracle_pins.py pins by enumeration; and the engine's NVT q tables,
+applied on the CPU to the oracle's per-element spectra as the kernels apply
+them, against the oracle's NVT restatement (cubic, orthorhombic and
+triclinic boxes changing between frames).
 
 Closed forms: (1) for a fixed box NVT gives the NPT rows, since the binning
 is linear and the counts of F equal binnings are F times one binning's;
@@ -92,3 +95,69 @@ def test_half_spectrum_counts_match_oracle(box):
     bo, inr, _ = O.spectrum_bins(plan)
     _, Co = O.histogram(plan, np.zeros(bo.shape), inr, bo)
     assert np.array_equal(got, Co) and Co.sum() > 300
+
+
+NVT_BOXES = {
+    "cubic": [np.diag([30.0] * 3), np.diag([30.5] * 3), np.diag([29.6] * 3)],
+    "orthorhombic": [np.diag([30.0] * 3), np.diag([30.4, 29.8, 30.2]), np.diag([29.7, 30.0, 30.3])],
+    "triclinic": [np.diag([30.0] * 3), np.array([[30.0, 0, 0], [2.5, 29.5, 0], [-1.5, 1.0, 30.4]]),
+                  np.array([[30.3, 0, 0], [2.0, 29.8, 0], [-1.0, 1.5, 30.1]])],
+}
+
+
+@pytest.mark.parametrize("kind", sorted(NVT_BOXES))
+def test_binned_tables_reproduce_oracle_nvt(kind):
+    """The engine's NVT tables (SaxsEngine._q_tables_binned: frame 0's bins,
+    the frame's own form factors and cut) and count table, applied on the
+    CPU to the oracle's per-element spectra the way the kernels apply them
+    (A = sum_e f_e[key] F_e at keyed points, 2 I per point, I at the origin,
+    into the bin of the key), against the oracle's NVT restatement."""
+    import types
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec
+    boxes = [np.asarray(b, dtype=np.float32) for b in NVT_BOXES[kind]]
+    n3, qcut = (24, 24, 24), 0.9
+    base, syms = O.synthetic_system(400, 30.0, seed=33)
+    frames = [O.frame_positions(base, 33, f, 0.5) for f in range(len(boxes))]
+    symbols, ids = O.element_ids(syms)
+    plan0 = O.make_plan(n3, boxes[0], 4, 2.0, None, 0.0, qcut)
+    plan_of = _plan_of(n3, plan0, qcut)
+    H, C = O.trajectory_histogram_nvt(plan0, list(zip(frames, boxes)), ids, symbols, plan_of)
+
+    nn = plan0.nn
+    fe = types.SimpleNamespace(spec=SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, qcut))
+    cell0 = G.inverse_box_matrix(boxes[0])
+    fe.hist = G.hist_params(n3, nn, np.diag(cell0).astype(np.float32), 2.0, 0.0, qcut)
+    fe._q_tables = lambda h, c: SaxsEngine._q_tables(fe, h, c)
+    fe._hist_for = lambda c: SaxsEngine._hist_for(fe, c)
+    He = np.zeros(fe.hist.num_bins)
+    X, Y, Z = np.meshgrid(*G.half_spectrum_m(nn), indexing="ij")
+    w = np.where((X == 0) & (Y == 0) & (Z == 0), 1.0, 2.0)
+    for pos, box in zip(frames, boxes):
+        cell = G.inverse_box_matrix(box)
+        h = fe._hist_for(cell)
+        if np.array_equal(cell, cell0):
+            bins, ff, kmax = fe._q_tables(h, cell)
+        else:
+            bins, ff, kmax = SaxsEngine._q_tables_binned(fe, h, cell, fe._hist_for(cell0), cell0)
+        K = (int(np.sqrt(len(bins) - 1)),) * 3 if kmax is None else kmax
+        valid = (np.abs(X) <= K[0]) & (np.abs(Y) <= K[1]) & (np.abs(Z) <= K[2])
+        if kmax is None:
+            key = X * X + Y * Y + Z * Z
+            valid &= key < len(bins)
+        elif G.is_triclinic(cell):
+            key = ((X + K[0]) * (2 * K[1] + 1) + Y + K[1]) * (2 * K[2] + 1) + Z + K[2]
+        else:
+            key = (np.abs(X) * (K[1] + 1) + np.abs(Y)) * (K[2] + 1) + np.abs(Z)
+        key = np.where(valid, key, 0)
+        p = plan_of(box)
+        Fs, _, _, _ = O.frame_spectrum_terms(p, pos, ids, symbols, box)
+        A = sum(np.where(valid, ff[e][key], 0.0) * Fs[e] for e in range(len(symbols)))
+        I = np.abs(A) ** 2 * p.moduli[0][:, None, None] * p.moduli[1][None, :, None] \
+            * p.moduli[2][None, None, :A.shape[2]]
+        b = np.where(valid, bins[key], -1)
+        np.add.at(He, b[b >= 0], (w * I)[b >= 0])
+    bins0, _, kmax0 = fe._q_tables(fe.hist, cell0)
+    Ce = G.half_spectrum_counts(nn, bins0, kmax0, fe.hist.num_bins, signed=G.is_triclinic(cell0))
+    assert np.array_equal(Ce, C)
+    ok = C > 0
+    np.testing.assert_allclose(He[ok] / len(boxes), H[ok], rtol=1e-6)
