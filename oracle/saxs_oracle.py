@@ -38,8 +38,13 @@ closed forms and the compiled reference helpers (oracle/_ref); the FFT
 against a brute-force DFT; spreading against single-atom stencils and
 partition of unity; end-to-end I(m) against the direct structure factor for
 atoms on grid nodes; bin counts against the known answers 34, 82, 178, 258,
-402 (64^3 cell, 128^3 FFT, 37 A box).  The adjoint is pinned by central
-finite differences of the forward oracle.
+402 (64^3 cell, 128^3 FFT, 37 A box); the border-slab pad against its closed
+form (the z-plane B-spline weights of atoms straddling the slab edges, and
+a full lattice whose density is exactly 1); the dq/qcut reset, row count
+and q column against the five profiles stored in the reference's own
+database (pycusaxs/data/reference_solvents.db, two of them with a lowered
+cutoff).  The adjoint is pinned by central finite differences of the
+forward oracle.
 """
 from __future__ import annotations
 

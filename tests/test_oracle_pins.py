@@ -453,3 +453,148 @@ def test_triclinic_counts_match_enumeration():
     symbols, ids = O.element_ids(syms)
     _, Cg = O.frame_histogram(plan, (pos @ B).astype(np.float32), ids, symbols, B)
     assert list(Cg.astype(int)) == C and sum(C) > 300
+
+
+# ---- border-slab pad (S2) by closed form -----------------------------------
+#
+# saxsDeviceKernels.cu:446-465: a voxel is in the border when NOT
+# (z > d && z < n_z - d), i.e. z <= d or z >= n_z - d, over every x and y
+# (the x / y terms are commented out), and the pad is the voxel sum divided
+# by the voxel count n_x n_y (2d + 1).  A lone atom deposits sum_{x,y}
+# th_x th_y = 1 over each z plane of its stencil, so the pad is the sum of
+# the z weights M_4(w + 3 - o) that land on border planes, over the count.
+
+
+def _border_expect(nodes_z, w_z, n, d, p=4):
+    """Closed-form pad for atoms with cell index nodes_z and fraction w_z."""
+    nx, ny, nz = n
+    border = set(range(0, d + 1)) | set(range(nz - d, nz))
+    total = 0.0
+    for mz, w in zip(nodes_z, w_z):
+        for o in range(p):
+            z = mz - p + o
+            if z < 0:
+                z += nz
+            if z in border:
+                total += cardinal_bspline(w + p - 1 - o, p)
+    return total / (nx * ny * (2 * d + 1))
+
+
+@pytest.mark.parametrize("nz", [32, 64, 128])
+def test_border_pad_closed_form_straddling_atoms(nz):
+    """Atoms whose z stencils straddle z = d / d + 1 and n_z - d - 1 / n_z - d,
+    and one that wraps through z = 0: the oracle's spread + border mean must
+    equal the closed form.  The wrong slab (z < d, or z <= d without the
+    upper slab) or the all-faces border changes the value."""
+    n = (16, 16, nz)
+    L = (16.0, 16.0, float(nz))                 # 1 A cells: u = x / L exactly
+    oc = np.array([1.0 / L[0], 1.0 / L[1], 1.0 / L[2]], dtype=np.float32)
+    d = O.border_planes(nz)
+    # stencil of cell index m covers m-4 .. m-1; pick m so that it straddles
+    # the lower edge (d, d+1), the upper edge (nz-d-1, nz-d) where d >= 2
+    # lets a stencil reach it, and wraps through z = 0 (m = 2, 1)
+    ms = [d + 3, d + 4, min(nz - 1, nz - d + 1), nz - 1, 2, 1]
+    fr = [0.25, 0.625, 0.5, 0.125, 0.375, 0.75]  # exact binary fractions
+    xy = [(5.5, 7.25), (9.0, 3.5), (11.75, 12.5), (2.5, 8.0), (6.0, 6.0), (0.5, 15.5)]
+    pos = np.array([[x, y, m + f] for (x, y), m, f in zip(xy, ms, fr)], dtype=np.float32)
+    m, w = O.grid_coords(pos, oc, n)
+    assert list(m[:, 2]) == ms and np.allclose(w[:, 2], fr, atol=0)
+    g = O.spread(pos, oc, n, 4)
+    pad = O.border_pad(g, d)
+    expect = _border_expect(ms, fr, n, d)
+    assert expect > 0
+    assert abs(pad - expect) <= 1e-15 * max(1.0, expect), (pad, expect)
+    # each plausible mistake gives a different value on these atoms
+    wrong_lt = np.concatenate([g[:, :, :d], g[:, :, nz - d:]], axis=2).mean()
+    faces = np.ones(n, dtype=bool)
+    faces[d + 1:n[0] - d, d + 1:n[1] - d, d + 1:nz - d] = False
+    wrong_faces = g[faces].mean()
+    assert abs(wrong_lt - expect) > 1e-6 * expect
+    assert abs(wrong_faces - expect) > 1e-6 * expect
+
+
+def test_border_pad_full_lattice_is_one_and_spectrum_vanishes():
+    """One atom on every node: the density is 1 everywhere (partition of
+    unity), so the pad is exactly 1, the literal embedding is a constant
+    nn^3 grid and every q != 0 of the spectrum vanishes."""
+    n3 = (16, 16, 16)
+    L = 32.0
+    box = np.diag([L] * 3)
+    plan = O.make_plan(n3, box, 4, 2.0, None, 0.0, 100.0)
+    g = np.stack(np.meshgrid(*[np.arange(16)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)
+    pos = (g * (L / 16)).astype(np.float32)
+    oc = np.array([1.0 / L] * 3, dtype=np.float32)
+    rho = O.spread(pos, oc, n3, 4)
+    assert np.allclose(rho, 1.0, atol=1e-13, rtol=0)
+    assert abs(O.border_pad(rho, plan.border) - 1.0) < 1e-13
+    syms = ["C"] * len(pos)
+    symbols, ids = O.element_ids(syms)
+    I, inr, _ = O.frame_intensity(plan, pos, ids, symbols, box, literal=True)
+    I0 = I[0, 0, 0]
+    off = inr.copy()
+    off[0, 0, 0] = False
+    assert I0 > 1e10 and I[off].max() <= 1e-20 * I0
+
+
+# ---- q axis of the reference's own stored profiles ---------------------------
+#
+# pycusaxs/data/reference_solvents.db holds five profiles written by the
+# reference (fixture tests/golden/reference_solvents_db.json): for each, the
+# cell grid, the supercell scale sigma, the requested bin size and cutoff,
+# the number of rows and the first ten rows.  The rows are b = 1 .. num_bins-1
+# with a non-zero count, q = b * dq printed %10.5f (saxsKernel.cu:210-225),
+# num_bins = int(qcut / dq) + 1 (:247) after the reset of :489-526.  Rows 4
+# and 5 (64^3, sigma 12, 90 A) exercise the qcut reset: 3.7 -> dq (nf - 1).
+
+
+def _db_rows():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_solvents_db.json")
+    with open(path) as fh:
+        return json.load(fh)["rows"]
+
+
+@pytest.mark.parametrize("row", range(5))
+def test_hist_params_reproduce_reference_database_q_axis(row):
+    from pycusaxs_amd import geometry as G
+    r = _db_rows()[row]
+    n = r["grid_size"]
+    L = r["box"]
+    sigma = r["supercell_scale"]
+    nn = [O.find_closest_product(v, sigma) for v in n]
+    assert nn == [G.fft_size(v, sigma) for v in n]
+    oc = [np.float32(1.0) / np.float32(v) for v in L]
+    hp = O.hist_params(n, nn, oc, sigma, r["bin_size"], r["qcut"])
+    hg = G.hist_params(n, nn, oc, sigma, r["bin_size"], r["qcut"])
+    assert (float(hp.dq), float(hp.qcut), hp.num_bins) == (hg.dq, hg.qcut, hg.num_bins)
+    # every bin 1 .. num_bins-1 holds the shell |m| = b on an axis (dq was
+    # reset to the grid spacing), so the stored row count is num_bins - 1
+    assert float(hp.dq) > r["bin_size"]
+    assert hp.num_bins - 1 == r["n_points"], (hp.num_bins, r["n_points"])
+    qs = ["%10.5f" % float(np.float32(np.float32(b) * hp.dq)) for b in range(1, 11)]
+    assert [float(s) for s in qs] == [pt[0] for pt in r["first_points"]]
+    # the oracle's exact bin rule puts a point in each of those bins
+    counts = np.zeros(hp.num_bins, dtype=np.int64)
+    K = int(float(hp.qcut) / float(hp.spacing[0])) + 1
+    for nsq in range(K * K + 1):
+        b = O.bin_of_n(nsq, hp.spacing[0], hp.dq, hp.qcut, hp.num_bins)
+        if b >= 0:
+            counts[b] += 1
+    assert np.count_nonzero(counts[1:]) == r["n_points"]
+
+
+def test_reference_database_qcut_reset_is_exercised():
+    """Rows 4 and 5 are the ones whose cutoff the grid lowers: a reset with
+    nf instead of nf - 1, or none at all, changes their row count."""
+    r = _db_rows()[3]
+    n, L, sigma = r["grid_size"], r["box"], r["supercell_scale"]
+    nn = [O.find_closest_product(v, sigma) for v in n]
+    oc = [np.float32(1.0) / np.float32(v) for v in L]
+    hp = O.hist_params(n, nn, oc, sigma, r["bin_size"], r["qcut"])
+    assert float(hp.qcut) < r["qcut"]
+    dq = hp.dq
+    nf = nn[0] // 2
+    wrong_nf = int(np.float32(np.float32(dq * np.float32(nf)) / dq)) + 1
+    wrong_none = int(np.float32(np.float32(r["qcut"]) / dq)) + 1
+    assert wrong_nf - 1 != r["n_points"] and wrong_none - 1 != r["n_points"]
