@@ -27,17 +27,17 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N_ATOMS = 100_000
-BOX = 100.0
-GRID = 128
-SIGMA = 2.0
-DQ = 0.005
-QCUT = 0.6
-ORDER = 4
-FRAMES = 100
-SEED = 2
-DISP = 0.5
-MIX = (("H", 0.50), ("C", 0.30), ("O", 0.15), ("N", 0.04), ("S", 0.01))
+from pycusaxs_amd import workloads as W  # noqa: E402  (numpy-only module)
+
+N_ATOMS = W.TRAJ["n_atoms"]
+BOX = W.TRAJ["box"]
+GRID = W.TRAJ["grid"]
+SIGMA = W.TRAJ["sigma"]
+DQ = W.TRAJ["dq"]
+QCUT = W.TRAJ["qcut"]
+ORDER = W.TRAJ["order"]
+FRAMES = W.TRAJ["frames"]
+SEED = W.TRAJ["seed"]
 HBM_PEAK_GBS = 8000.0
 
 
@@ -50,80 +50,25 @@ MEASURED_TRAFFIC = {"prep": 211.2e6, "scatter": 572.5e6, "spread_z": 1675.8e6, "
                     "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1710.2e6, "adj_z": 2718.9e6,
                     "grad_sum": 755.9e6}
 
+# Wave64 VALU instructions per launch of each kernel for the default workload,
+# from rocprofv3 SQ_INSTS_VALU on MI355X (commands and the per-kernel table:
+# profiles/sq_counters_trajectory_100k_128.md).  A CU issues at most one wave64
+# VALU instruction per clock, so the chip's issue peak is 256 x 2.4e9 per second.
+MEASURED_VALU = {"prep": 37.3e6, "scatter": 12.2e6, "spread_z": 919.4e6, "yfft": 145.8e6,
+                 "xfft_epi": 41.0e6, "adj_x": 77.2e6, "adj_y": 158.7e6, "adj_z": 913.7e6,
+                 "grad_sum": 88.8e6}
+VALU_ISSUE_PEAK = 256 * 2.4e9          # wave64 VALU instructions per second
+
 
 def make_inputs(n_atoms=N_ATOMS, frames=FRAMES):
-    rng = np.random.default_rng(SEED)
-    base = rng.uniform(0.0, BOX, size=(n_atoms, 3)).astype(np.float32)
-    counts = {s: int(round(n_atoms * f)) for s, f in MIX}
-    counts["H"] += n_atoms - sum(counts.values())
-    syms = []
-    for s, c in counts.items():
-        syms += [s] * c
-    syms = list(np.asarray(syms)[rng.permutation(n_atoms)])
-    xyz = np.empty((frames, n_atoms, 3), dtype=np.float32)
-    for f in range(frames):
-        r = np.random.default_rng([SEED, f])
-        xyz[f] = base + r.normal(0.0, DISP, size=base.shape).astype(np.float32)
-    symbols = sorted(set(syms))
-    lut = {s: i for i, s in enumerate(symbols)}
-    ids = np.array([lut[s] for s in syms], dtype=np.int32)
-    return xyz, symbols, ids, syms
-
-
-def smooth_at_least(x: float) -> int:
-    """Smallest 2,3,5,7-smooth integer >= x."""
-    v = max(1, int(np.ceil(x)))
-    while True:
-        m = v
-        for p_ in (2, 3, 5, 7):
-            while m % p_ == 0:
-                m //= p_
-        if m == 1:
-            return v
-        v += 1
-
-
-def mix_symbols(rng, n_atoms, mix):
-    counts = {s: int(round(n_atoms * f)) for s, f in mix}
-    counts[mix[0][0]] += n_atoms - sum(counts.values())
-    syms = []
-    for s_, c in counts.items():
-        syms += [s_] * c
-    syms = np.asarray(syms)[rng.permutation(n_atoms)]
-    symbols = sorted(set(syms.tolist()))
-    lut = {s_: i for i, s_ in enumerate(symbols)}
-    return symbols, np.array([lut[s_] for s_ in syms], dtype=np.int32)
-
-
-def batched_uneven_workload(count=32, frames=8, seed=3):
-    """32 systems, atom counts log-uniform in 10k-300k at 0.1 atoms/A^3,
-    cubic grids = smallest 7-smooth >= L/0.75 A clipped to [64, 192],
-    sigma 2, 8 frames each (0.5 A jitter), H/C/O/N/S 50/30/15/4/1 %."""
-    out = []
-    rng = np.random.default_rng(seed)
-    for s_ in range(count):
-        n_atoms = int(round(np.exp(rng.uniform(np.log(1e4), np.log(3e5)))))
-        L = (n_atoms / 0.1) ** (1.0 / 3.0)
-        n = min(192, max(64, smooth_at_least(L / 0.75)))
-        r = np.random.default_rng([seed, s_])
-        base = r.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
-        symbols, ids = mix_symbols(r, n_atoms, MIX)
-        xyz = np.stack([base + np.random.default_rng([seed, s_, f]).normal(0.0, DISP, size=base.shape)
-                        .astype(np.float32) for f in range(frames)])
-        out.append(dict(n_atoms=n_atoms, L=float(np.float32(L)), n=n, symbols=symbols, ids=ids,
-                        xyz=xyz))
-    return out
-
-
-MIX9 = (("H", 0.48), ("C", 0.30), ("O", 0.15), ("N", 0.04), ("S", 0.01), ("P", 0.005),
-        ("Na", 0.005), ("Cl", 0.005), ("K", 0.005))
+    return W.trajectory(n_atoms, frames)
 
 
 def leg_batched_uneven(torch, steps):
     from pycusaxs_amd.batch import BatchedSystems, SystemInput
     from pycusaxs_amd.engine import SystemSpec, resolve_grid
     from pycusaxs_amd.tuning import Autotuner
-    work = batched_uneven_workload()
+    work = W.uneven_systems()
     tuner = Autotuner(frames=2, repeats=1)
     systems = []
     for w in work:
@@ -156,14 +101,11 @@ def leg_batched_uneven(torch, steps):
 
 def leg_large(torch, steps):
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
-    n_atoms, L, n, F = 1_000_000, 215.0, 256, 16
-    rng = np.random.default_rng(5)
-    base = rng.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
-    symbols, ids = mix_symbols(rng, n_atoms, MIX9)
+    n_atoms, L, n, F = W.LARGE["n_atoms"], W.LARGE["box"], W.LARGE["grid"], W.LARGE["frames"]
+    base, symbols, ids = W.large_base()
     xyz = torch.empty((F, n_atoms, 3), dtype=torch.float32, device="cuda")
     for f in range(F):
-        fr = base + np.random.default_rng([5, f]).normal(0.0, DISP, size=base.shape).astype(np.float32)
-        xyz[f].copy_(torch.from_numpy(fr))
+        xyz[f].copy_(torch.from_numpy(W.large_frame(base, f)))
     idt = torch.from_numpy(ids).cuda()
     n3, nn = resolve_grid((n,) * 3, SIGMA)
     eng = SaxsEngine(SystemSpec(n3, nn, ORDER, tuple(symbols), SIGMA, 0.0, 0.5),
@@ -194,24 +136,10 @@ def leg_large(torch, steps):
     return out
 
 
-def restraint_workload():
-    """50,000 atoms in an 80 A box, 128^3 cell, sigma 2, order 6; target
-    profile from the structure displaced by N(0, 1 A^2); chi^2 over the
-    requested 100 bins from 0.005 to 0.5 A^-1 with 1 % errors."""
-    rng = np.random.default_rng(4)
-    n_atoms, L = 50_000, 80.0
-    base = rng.uniform(0.0, L, size=(n_atoms, 3)).astype(np.float32)
-    symbols, ids = mix_symbols(rng, n_atoms, MIX)
-    target = (base + np.random.default_rng([4, 1]).normal(0.0, 1.0, size=base.shape)
-              ).astype(np.float32)
-    return dict(base=base, target=target, symbols=symbols, ids=ids, L=L, n=128, order=6,
-                dq=0.005, qcut=0.5)
-
-
 def leg_restraint(torch, steps, keep):
     from pycusaxs_amd.autograd import chi2, saxs_profile
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
-    w = restraint_workload()
+    w = W.restraint()
     n3, nn = resolve_grid((w["n"],) * 3, SIGMA)
     box = np.diag([w["L"]] * 3)
     eng = SaxsEngine(SystemSpec(n3, nn, w["order"], tuple(w["symbols"]), SIGMA, w["dq"],
@@ -356,6 +284,25 @@ def fixed_grad_H(torch, eng, xyz, ids, dev):
             / Hf.abs().clamp_min(1.0).cpu()).to(dev)
 
 
+def host_cpu():
+    """CPU model of this host, its logical CPU count and the CPUs this process
+    may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
 def cpu_baseline(xyz, symbols, ids, frames, threads):
     from oracle import saxs_oracle as O
     used = O.set_fft_workers(threads)
@@ -380,7 +327,10 @@ def main():
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--plain-frames", type=int, default=10)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = every CPU this process may run on (its affinity set)")
+    ap.add_argument("--cpu-frames-1t", type=int, default=2,
+                    help="frames of the single-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and 1M-atom legs")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-frame plain leg")
@@ -590,24 +540,54 @@ def main():
     dom = max((k for k in per_kernel if k in kb), key=lambda k: times[k][0])
     d = per_kernel[dom]
     default_workload = args.frames == FRAMES and batch == FRAMES
-    line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"],
+    if default_workload:
+        # counter-based fractions: VALU issue (instructions per launch from
+        # rocprofv3 over the live launch time at the peak issue rate) and
+        # measured HBM bytes over the live launch time
+        for k, v in per_kernel.items():
+            avg_s = v["avg_ms"] * 1e-3
+            if k in MEASURED_VALU:
+                v["valu_issue_frac"] = round(MEASURED_VALU[k] / avg_s / VALU_ISSUE_PEAK, 4)
+            if k in MEASURED_TRAFFIC:
+                v["hbm_traffic_frac"] = round(MEASURED_TRAFFIC[k] / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+    valu_frac = per_kernel[dom].get("valu_issue_frac")
+    hbm_frac = per_kernel[dom].get("hbm_traffic_frac")
+    bound = "hbm"
+    if valu_frac is not None and hbm_frac is not None and valu_frac > hbm_frac:
+        bound = "valu"
+    # achieved / frac: the byte model (algorithmic bytes over the live launch
+    # time); valu_issue_frac and hbm_traffic_frac: what the counters say of
+    # the same launches; bound: whichever counter-based fraction is larger
+    line["roofline"] = {"bound": bound, "kernel": dom, "achieved": d["GBps"],
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                        "traffic": MEASURED_TRAFFIC.get(dom) if default_workload else None}
+                        "traffic": MEASURED_TRAFFIC.get(dom) if default_workload else None,
+                        "valu_issue_frac": valu_frac, "hbm_traffic_frac": hbm_frac,
+                        "frac_model": "pruned byte model (density grids counted although they "
+                                      "stay in LDS)"}
     line["kernels"] = per_kernel
     print("# kernels: " + json.dumps(per_kernel), file=sys.stderr, flush=True)
 
     # parity of frame 0 against the oracle + CPU baseline
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        model, ncpu, usable = host_cpu()
+        threads = args.cpu_threads or usable
         Hf, Cf = eng.forward_frames(xyz[:1], ids)
         torch.cuda.synchronize()
         cps, used, dt, Ho, Co = cpu_baseline(xyz_h, symbols, ids_h, args.cpu_frames, threads)
+        cps1, _, dt1, _, _ = cpu_baseline(xyz_h, symbols, ids_h, args.cpu_frames_1t, 1)
         line["cpu_baseline"] = {"value": round(cps, 5), "unit": "frames/s", "cores": used,
                                 "kind": "port",
-                                "sample": f"FP64 oracle (literal reference pipeline, numpy/scipy "
-                                          f"FFT) on frames 0..{args.cpu_frames - 1} of the "
-                                          f"workload, {dt:.1f} s, host {platform.processor() or platform.machine()}"}
+                                "single_thread_value": round(cps1, 5),
+                                "cpu_model": model, "os_cpu_count": ncpu,
+                                "cpus_usable": usable,
+                                "sample": f"FP64 oracle (literal reference pipeline, numpy "
+                                          f"spreading + scipy FFT) on frames "
+                                          f"0..{args.cpu_frames - 1} of the workload with "
+                                          f"{used} FFT threads ({dt:.1f} s), and frames "
+                                          f"0..{args.cpu_frames_1t - 1} with 1 thread "
+                                          f"({dt1:.1f} s); host {model}, os.cpu_count() "
+                                          f"{ncpu}, {usable} usable"}
         h0, c0 = Hf[0].cpu().numpy(), Cf[0].cpu().numpy()
         o_h, o_c = Ho, Co
         sel = (np.arange(len(c0)) >= 1) & (o_c > 0)

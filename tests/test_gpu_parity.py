@@ -471,3 +471,119 @@ def test_segmented_sort_in_key_passes_matches_single_pass(gpu):
         eng.close()
     assert np.array_equal(out[0][1], out[1][1])
     assert np.allclose(out[0][0], out[1][0], rtol=1e-12, atol=0)
+
+
+def _lambda2(m, nn, p=4):
+    """lambda(m)^2 = (G_p(x) / G_2p(x))^2, x = pi m / nn, by the finite series
+    G_r(x) = 1 + sum_{k=1}^{50} (x/(x + pi k))^r + (x/(x - pi k))^r."""
+    if m == 0:
+        return 1.0
+    x = np.pi * m / nn
+    k = np.arange(1, 51)
+    g = lambda r: 1.0 + np.sum((x / (x + np.pi * k)) ** r + (x / (x - np.pi * k)) ** r)
+    return (g(p) / g(2 * p)) ** 2
+
+
+@pytest.mark.parametrize("n,sigma,tile", [(256, 2.0, (4, 4)),    # 512, n_z = N/2, pruned
+                                          (128, 4.0, (4, 4)),    # 512, n_z = N/4, pruned
+                                          (256, 2.0, (0, 0))])   # 512, default plan
+def test_512_grid_on_node_atoms_direct_structure_factor(gpu, n, sigma, tile):
+    """FFT grid 512 pinned against mathematics rather than against the
+    engine's own transforms: atoms on grid nodes whose stencils reach no
+    border slab (pad 0) give, at every half-spectrum point m,
+    I(m) = |sum_j f_j(q) exp(-2 pi i m.g_j / nn)|^2 prod_a lambda(m_a)^2
+    (theta(0) = (1/6, 4/6, 1/6, 0) per axis is b(m) times a phase, and the
+    modulus table divides |b|^2 out), so H_b / C_b is the mean of that
+    direct sum over the half-spectrum points of bin b."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    h = 0.75                                   # cell size, exact in float32
+    L = n * h
+    box = np.diag([L] * 3)
+    n3, nn = resolve_grid((n,) * 3, sigma)
+    assert nn == (512, 512, 512)
+    qcut = 0.4
+    rng = np.random.default_rng(77)
+    d = O.border_planes(n)
+    k_at = 16
+    nodes = np.stack([rng.integers(4, n, k_at), rng.integers(4, n, k_at),
+                      rng.integers(d + 5, n - d, k_at)], axis=1)
+    pos = (nodes * h).astype(np.float32)
+    syms = [("C", "O", "N", "S")[i % 4] for i in range(k_at)]
+    symbols, ids = O.element_ids(syms)
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), sigma, 0.0, qcut)
+    eng = SaxsEngine(spec, box, max_atoms=k_at, max_frames=1, tile=tile)
+    eng.forward(torch.from_numpy(pos[None]), torch.from_numpy(ids))
+    torch.cuda.synchronize()
+    H, C = eng.H.cpu().numpy(), eng.C.cpu().numpy()
+    kept = eng.kept()
+    eng.close()
+    hp = O.hist_params(n3, nn, [np.float32(1.0) / np.float32(L)] * 3, sigma, 0.0, qcut)
+    s = float(hp.spacing[0])
+    K = int(qcut / s) + 1
+    assert kept[2] <= 512 // 8
+    mx = np.arange(-K, K + 1)
+    mz = np.arange(0, K + 1)
+    M = np.stack(np.meshgrid(mx, mx, mz, indexing="ij"), axis=-1).reshape(-1, 3)
+    nsq = (M ** 2).sum(axis=1)
+    M, nsq = M[nsq * s * s <= float(hp.qcut) ** 2 * (1 + 1e-9)], nsq[nsq * s * s <= float(hp.qcut) ** 2 * (1 + 1e-9)]
+    bins = np.array([O.bin_of_n(int(v), hp.spacing[0], hp.dq, hp.qcut, hp.num_bins) for v in nsq])
+    keep = bins >= 0
+    M, nsq, bins = M[keep], nsq[keep], bins[keep]
+    q = s * np.sqrt(nsq)
+    phase = np.exp(-2j * np.pi * (M @ nodes.T) / 512.0)            # [points, atoms]
+    ff = np.stack([O.form_factor(sy, q) for sy in syms], axis=1)   # [points, atoms]
+    A = (ff * phase).sum(axis=1)
+    lam = {m: _lambda2(m, 512) for m in range(-K, K + 1)}
+    lam2 = np.array([lam[a] * lam[b] * lam[c] for a, b, c in M])
+    I = np.abs(A) ** 2 * lam2
+    Hx = np.bincount(bins, weights=I, minlength=hp.num_bins)
+    Cx = np.bincount(bins, minlength=hp.num_bins).astype(float)
+    sel = (np.arange(hp.num_bins) >= 1) & (Cx > 0)
+    assert np.array_equal(C[sel], 2 * Cx[sel])
+    rel = np.abs(H[sel] / C[sel] - Hx[sel] / Cx[sel]) / np.abs(Hx[sel] / Cx[sel])
+    print(f"[512 on-node n={n} tile={tile}] max rel err {rel.max():.3e} over {sel.sum()} bins",
+          flush=True)
+    assert rel.max() <= REL_TOL
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_512_point_axis_forward_and_adjoint_against_oracle(gpu, axis):
+    """One axis with a 512-point FFT (256 cells, sigma 2) and two short
+    ones (16 cells -> 32): the 512-point pruned forward and adjoint kernels of
+    that axis against the FP64 oracle and its gradient (the oracle's
+    transform is then only 32 x 32 x 512)."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    n3 = [16, 16, 16]
+    n3[axis] = 256
+    Ls = [16.0, 16.0, 16.0]
+    Ls[axis] = 256.0
+    box = np.diag(Ls)
+    n3, nn = resolve_grid(tuple(n3), 2.0)
+    assert nn[axis] == 512
+    rng = np.random.default_rng(30 + axis)
+    n_atoms = 600
+    pos = (rng.uniform(0.0, 1.0, size=(n_atoms, 3)) * np.array(Ls)).astype(np.float32)
+    syms = [("H", "C", "O")[i % 3] for i in range(n_atoms)]
+    symbols, ids = O.element_ids(syms)
+    qcut = 0.5
+    spec = SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, qcut)
+    eng = SaxsEngine(spec, box, max_atoms=n_atoms, max_frames=1, tile=(4, 4))
+    xyz = torch.from_numpy(pos[None])
+    idt = torch.from_numpy(ids)
+    Hf, Cf = eng.forward_frames(xyz, idt)
+    Hn = Hf.cpu().numpy()
+    gH = np.random.default_rng(5).normal(size=Hn.shape) / np.maximum(np.abs(Hn), 1.0)
+    g = eng.backward(xyz, idt, torch.from_numpy(gH)).cpu().numpy()[0]
+    kept = eng.kept()
+    eng.close()
+    assert kept[axis] < (512 if axis < 2 else 257)        # the 512-point axis is pruned
+    plan = O.make_plan(n3, box, 4, 2.0, None, 0.0, qcut)
+    Ho, Co = O.frame_histogram(plan, pos, ids, symbols, box)
+    compare(Hn[0], Cf[0].cpu().numpy(), Ho, Co, label=f"512 on axis {axis}")
+    go = O.grad_linear_H(plan, pos, ids, symbols, box, gH[0])
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    mx = np.abs(g - go).max() / np.abs(go).max()
+    print(f"[512 on axis {axis}] kept {kept}, adjoint rel-L2 {rel:.2e} max {mx:.2e}", flush=True)
+    assert rel <= 2e-4 and mx <= 1e-3
