@@ -9,6 +9,10 @@
  *   SAXS_ENOMEM  (-2) allocation failed  -> Python RuntimeError
  *   SAXS_EHIP    (-3) HIP runtime error  -> Python RuntimeError
  *   SAXS_EUNSUP  (-4) unsupported shape  -> Python ValueError
+ *   SAXS_ECAPACITY (-5) new q tables need more workspace than the plan
+ *                holds (a box change that keeps more frequencies); the
+ *                plan is unchanged and a plan for the new box is needed
+ *                -> Python ValueError subclass CapacityError
  * saxs_last_error() returns the calling thread's last message.
  *
  * One plan is used by one stream at a time; calls on one plan must not
@@ -37,6 +41,7 @@ extern "C" {
 #define SAXS_ENOMEM (-2)
 #define SAXS_EHIP (-3)
 #define SAXS_EUNSUP (-4)
+#define SAXS_ECAPACITY (-5)
 
 typedef struct saxs_plan saxs_plan;
 
@@ -122,8 +127,10 @@ typedef struct saxs_desc {
 int saxs_plan_create(const saxs_desc* desc, saxs_plan** out);
 
 /* Replaces the shell tables (bin_lut, ff_lut) after a box change; lut_len
- * may differ from the plan's (kept-frequency ranges are recomputed; it must
- * not exceed what the workspace was sized for).  Synchronous. */
+ * may differ from the plan's (kept-frequency ranges are recomputed).  The
+ * workspace holds tables up to the point box (K+2)^3 and one more kept
+ * frequency per axis than the creating box; beyond that the call returns
+ * SAXS_ECAPACITY and leaves the plan as it was.  Synchronous. */
 int saxs_plan_set_tables(saxs_plan* plan, int32_t lut_len, const int32_t* bin_lut,
                          const float* ff_lut);
 
@@ -141,7 +148,7 @@ int saxs_plan_set_point_tables(saxs_plan* plan, const float* cell, const int32_t
  * the mean density over the border z-slabs (padding::avg, the reference's
  * only live mode: Options.cpp:13, saxsDeviceKernels.cu:446-465,
  * saxsKernel.cu:148-161).  Otherwise pad[num_elements] (host, atoms per
- * cell voxel, each in [0, 2^40]) is used as a constant for every frame
+ * cell voxel, each in [0, 2^31 / n_z]) is used as a constant for every frame
  * (padding::given, declared in Options.h:7-11 with its values from
  * AtomCounter::calculateAtomCounts, AtomCounter.cpp:77-103); the adjoint
  * then has no pad-mean term.  Synchronous; applies to later calls. */

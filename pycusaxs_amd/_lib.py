@@ -10,8 +10,8 @@ from pathlib import Path
 
 from ._build import LIB_PATH
 
-SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP = 0, -1, -2, -3, -4
-ABI_VERSION = 4
+SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP, SAXS_ECAPACITY = 0, -1, -2, -3, -4, -5
+ABI_VERSION = 5
 
 # every symbol include/saxs_engine.h declares
 EXPORTED = (
@@ -61,6 +61,11 @@ _lib = None
 
 class EngineError(RuntimeError):
     pass
+
+
+class CapacityError(ValueError):
+    """The plan's workspace cannot hold the q tables of a new box (more kept
+    frequencies than it was sized for); the plan is unchanged."""
 
 
 def load(path: Path = LIB_PATH):
@@ -113,6 +118,8 @@ def check(rc: int, what: str):
     if rc == SAXS_OK:
         return
     msg = load().saxs_last_error().decode(errors="replace")
+    if rc == SAXS_ECAPACITY:
+        raise CapacityError(f"{what}: {msg}")
     if rc in (SAXS_EINVAL, SAXS_EUNSUP):
         raise ValueError(f"{what}: {msg}")
     raise EngineError(f"{what}: {msg}")

@@ -244,13 +244,18 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     bin_box = box0 if sim == "nvt" else None
 
     def flush():
+        # RunSaxs.cu:195-199 logs a failing frame and goes on, so saxs.dat
+        # silently lacks it; here a batch that cannot be processed ends the
+        # run with the error (a box that needs a larger workspace does not
+        # fail: the engine makes a plan for it)
         nonlocal gpu_ms
         t0 = time.perf_counter()
         try:
             stager.launch(eng, ids_t, bin_box)
         except (ValueError, RuntimeError) as exc:
-            # RunSaxs.cu:195-199 logs a failing frame and goes on
             print(f"Error processing frames: {exc}", file=sys.stderr)
+            eng.close()
+            raise
         gpu_ms += (time.perf_counter() - t0) * 1e3
 
     frames_iter = topo.iter_frames_stream(begin, end + 1, stride)

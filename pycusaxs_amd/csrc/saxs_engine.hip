@@ -436,8 +436,10 @@ void set_cell(saxs::Params& p, const float* cell) {
 int set_tables(saxs_plan* pl, const float* cell, const int32_t* kmax, int32_t lut_len,
                const int32_t* bin_lut, const float* ff_lut) {
   if (!pl || !bin_lut || !ff_lut) return fail(SAXS_EINVAL, "NULL argument");
-  if (lut_len < 1 || lut_len > pl->lut_cap)
-    return fail(SAXS_EINVAL, "lut_len %d outside [1, %d] (recreate the plan)", lut_len, pl->lut_cap);
+  if (lut_len < 1) return fail(SAXS_EINVAL, "lut_len %d < 1", lut_len);
+  if (lut_len > pl->lut_cap)
+    return fail(SAXS_ECAPACITY, "lut_len %d exceeds the plan's %d (create a plan for this box)",
+                lut_len, pl->lut_cap);
   saxs::Params saved = pl->p;
   const bool tric = is_triclinic(cell);
   pl->p.point_keys = kmax ? (tric ? 2 : 1) : 0;
@@ -445,7 +447,9 @@ int set_tables(saxs_plan* pl, const float* cell, const int32_t* kmax, int32_t lu
   set_cell(pl->p, cell);
   int rc = derive_ranges(pl, lut_len);
   if (!rc && (pl->p.kz > pl->kz_cap || pl->p.ky > pl->ky_cap || pl->p.nchx > pl->nchx_cap))
-    rc = fail(SAXS_EINVAL, "new tables need a larger workspace (recreate the plan)");
+    rc = fail(SAXS_ECAPACITY, "new tables keep kz %d, ky %d, %d x chunks; the workspace holds "
+              "%d, %d, %d (create a plan for this box)", pl->p.kz, pl->p.ky, pl->p.nchx,
+              pl->kz_cap, pl->ky_cap, pl->nchx_cap);
   if (rc) {
     pl->p = saved;
     (void)derive_ranges(pl, saved.lut_len);          // restores the pass flags
@@ -761,7 +765,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 
 extern "C" {
 
-int32_t saxs_abi_version(void) { return 4; }
+int32_t saxs_abi_version(void) { return 5; }
 
 const char* saxs_last_error(void) { return g_err.c_str(); }
 
@@ -987,10 +991,13 @@ int saxs_plan_set_padding(saxs_plan* pl, const double* pad) {
   }
   const int E = pl->p.E;
   std::vector<unsigned long long> units(E);
+  // pad * 2^32 fixed-point units, summed over at most n_z voxels of a column,
+  // must stay below 2^63: pad <= 2^31 / n_z atoms per voxel
+  const double pad_max = 2147483648.0 / (double)pl->p.n[2];
   for (int e = 0; e < E; ++e) {
-    // 2^40 atoms per voxel keeps pad * 2^32 and the column sums below 2^63
-    if (!(pad[e] >= 0.0 && pad[e] <= 1099511627776.0))
-      return fail(SAXS_EINVAL, "pad[%d] = %g outside [0, 2^40] atoms per voxel", e, pad[e]);
+    if (!(pad[e] >= 0.0 && pad[e] <= pad_max))
+      return fail(SAXS_EINVAL, "pad[%d] = %g outside [0, 2^31 / n_z = %g] atoms per voxel", e,
+                  pad[e], pad_max);
     units[e] = (unsigned long long)std::llrint(pad[e] * 4294967296.0);
   }
   HIP_TRY(hipMemcpy(pl->pad_given, units.data(), sizeof(unsigned long long) * E,

@@ -18,7 +18,8 @@ import torch
 from . import _lib
 from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_points,
                        form_factor_table, form_factor_vectors, half_spectrum_counts, hist_params,
-                       inverse_box_matrix, is_triclinic, point_bins, point_kmax, reciprocal_basis,
+                       inverse_box_matrix, is_triclinic, max_given_pad, point_bins, point_kmax,
+                       reciprocal_basis,
                        shell_bins, triclinic_bins, triclinic_kmax)
 
 _NO_BIN_LIMIT = 2 ** 31 - 1     # bins past num_bins still mark q <= qcut
@@ -77,8 +78,27 @@ class SaxsEngine:
                                            spec.qcut_req, spec.dq_policy)
         self.border = border_planes(spec.n[2])
         self.moduli = [bspline_moduli(spec.nn[a], spec.order) for a in range(3)]
+        self._tile = (int(tile[0]), int(tile[1]))
+        self._flags = int(flags)
+        self._timing = False
         bins, ff, kmax = self._q_tables(self.hist, self.cell)
         self._box_key = tuple(float(v) for v in self.cell.ravel())
+        self._plan = C.c_void_p()
+        self._create_plan(self.cell, bins, ff, kmax)
+        nb = self.hist.num_bins
+        self.H = torch.zeros(nb, dtype=torch.float64, device=self.device)
+        self.C = torch.zeros(nb, dtype=torch.float64, device=self.device)
+        self.frames_done = 0
+        self._generation = 0
+        self._last = None
+        self.pad_given: Optional[Dict[str, float]] = None
+        if pad_given is not None:
+            self.set_padding(pad_given)
+
+    def _create_plan(self, cell, bins, ff, kmax):
+        """saxs_plan_create for `cell` with its q tables (the previous plan,
+        if any, is destroyed first; H, C and the frame count are kept)."""
+        spec = self.spec
         mods = [np.ascontiguousarray(m, dtype=np.float32) for m in self.moduli]
         desc = _lib.SaxsDesc()
         desc.n[:] = list(spec.n)
@@ -95,23 +115,31 @@ class SaxsEngine:
         desc.mod_z = mods[2].ctypes.data_as(C.POINTER(C.c_float))
         desc.max_atoms = self.max_atoms
         desc.max_frames = self.max_frames
-        desc.tile_x, desc.tile_y = int(tile[0]), int(tile[1])
-        desc.flags = int(flags)
+        desc.tile_x, desc.tile_y = self._tile
+        desc.flags = self._flags
         desc.kmax[:] = list(kmax) if kmax is not None else [0, 0, 0]
-        desc.cell[:] = [float(v) for v in self.cell.ravel()] if is_triclinic(self.cell) else [0.0] * 9
+        desc.cell[:] = [float(v) for v in cell.ravel()] if is_triclinic(cell) else [0.0] * 9
         plan = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.saxs_plan_create(C.byref(desc), C.byref(plan)), "saxs_plan_create")
+        self.close()
         self._plan = plan
-        nb = self.hist.num_bins
-        self.H = torch.zeros(nb, dtype=torch.float64, device=self.device)
-        self.C = torch.zeros(nb, dtype=torch.float64, device=self.device)
-        self.frames_done = 0
-        self._generation = 0
+        self.plans_created = getattr(self, "plans_created", 0) + 1
+
+    def _rebuild_plan(self, cell, bins, ff, kmax):
+        """A box whose q tables do not fit the workspace (it keeps more
+        frequencies than the first box): a plan sized for it replaces the
+        current one; accumulated H, C, frame count, padding and timing carry
+        over."""
+        torch.cuda.current_stream().synchronize()
+        self._create_plan(cell, bins, ff, kmax)
+        if self.pad_given is not None:
+            pads = self.pad_given
+            self.pad_given = None
+            self.set_padding(pads)
+        if self._timing:
+            self.set_timing(True)
         self._last = None
-        self.pad_given: Optional[Dict[str, float]] = None
-        if pad_given is not None:
-            self.set_padding(pad_given)
 
     # -- tables ---------------------------------------------------------
     def _q_tables(self, hist: Histogram, cell):
@@ -201,21 +229,24 @@ class SaxsEngine:
         else:
             bins, ff, kmax = self._q_tables_binned(h, cell, self._hist_for(cell0), cell0)
         torch.cuda.current_stream().synchronize()
-        if kmax is None:
-            _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
-                                                     bins.ctypes.data_as(C.c_void_p),
-                                                     ff.ctypes.data_as(C.c_void_p)),
-                       "saxs_plan_set_tables")
-        else:
-            km = np.array(kmax, dtype=np.int32)
-            cm = np.ascontiguousarray(cell, dtype=np.float32)
-            cptr = cm.ctypes.data_as(C.c_void_p) if is_triclinic(cell) else None
-            _lib.check(self.lib.saxs_plan_set_point_tables(self._plan, cptr,
-                                                           km.ctypes.data_as(C.c_void_p),
-                                                           len(bins),
-                                                           bins.ctypes.data_as(C.c_void_p),
-                                                           ff.ctypes.data_as(C.c_void_p)),
-                       "saxs_plan_set_point_tables")
+        try:
+            if kmax is None:
+                _lib.check(self.lib.saxs_plan_set_tables(self._plan, len(bins),
+                                                         bins.ctypes.data_as(C.c_void_p),
+                                                         ff.ctypes.data_as(C.c_void_p)),
+                           "saxs_plan_set_tables")
+            else:
+                km = np.array(kmax, dtype=np.int32)
+                cm = np.ascontiguousarray(cell, dtype=np.float32)
+                cptr = cm.ctypes.data_as(C.c_void_p) if is_triclinic(cell) else None
+                _lib.check(self.lib.saxs_plan_set_point_tables(self._plan, cptr,
+                                                               km.ctypes.data_as(C.c_void_p),
+                                                               len(bins),
+                                                               bins.ctypes.data_as(C.c_void_p),
+                                                               ff.ctypes.data_as(C.c_void_p)),
+                           "saxs_plan_set_point_tables")
+        except _lib.CapacityError:
+            self._rebuild_plan(cell, bins, ff, kmax)
         self._box_key = key
         self.oc = oc
         self.cell = cell
@@ -234,6 +265,10 @@ class SaxsEngine:
             return
         pads = {str(k): float(v) for k, v in dict(pad_given).items()}
         arr = np.array([pads.get(s, 0.0) for s in self.spec.symbols], dtype=np.float64)
+        limit = max_given_pad(self.spec.n[2])
+        bad = [s for s, v in zip(self.spec.symbols, arr) if not 0.0 <= v <= limit]
+        if bad:
+            raise ValueError(f"given pad of {bad} outside [0, 2^31 / n_z = {limit:g}] atoms per voxel")
         _lib.check(self.lib.saxs_plan_set_padding(self._plan, arr.ctypes.data_as(C.c_void_p)),
                    "saxs_plan_set_padding")
         self.pad_given = pads
@@ -338,6 +373,7 @@ class SaxsEngine:
 
     def set_timing(self, on: bool):
         _lib.check(self.lib.saxs_plan_set_timing(self._plan, 1 if on else 0), "saxs_plan_set_timing")
+        self._timing = bool(on)
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since the last call."""

@@ -67,6 +67,13 @@ def border_planes(nz: int) -> int:
     return int(F32(F32(_SHELL * F32(nz)) / F32(2.0)))
 
 
+def max_given_pad(nz: int) -> float:
+    """Largest constant pad (atoms per cell voxel) the engine accepts: the
+    fixed-point pad (2^32 units per atom) summed over a column of n_z voxels
+    stays below 2^63."""
+    return 2.0 ** 31 / float(nz)
+
+
 def inverse_box_diag(box) -> np.ndarray:
     """oc diagonal as float32(1/L_a); raises for a triclinic box."""
     b = np.asarray(box, dtype=np.float64).reshape(3, 3)

@@ -273,3 +273,23 @@ def test_triclinic_bins_match_oracle():
                 for v in range(-km[(a + 2) % 3], km[(a + 2) % 3] + 1):
                     m[(a + 2) % 3] = v
                     assert O.bin_of_vec(tuple(m), basis, h.dq, h.qcut, h.num_bins) == -1
+
+
+def test_capacity_error_maps_to_value_error_subclass():
+    """SAXS_ECAPACITY (a box whose q tables need more workspace) is a
+    ValueError to callers of the ABI, and a distinct class the engine
+    catches to create a plan for that box."""
+    with pytest.raises(_lib.CapacityError):
+        _lib.check(_lib.SAXS_ECAPACITY, "saxs_plan_set_tables")
+    assert issubclass(_lib.CapacityError, ValueError)
+    txt = open(HEADER).read()
+    assert "#define SAXS_ECAPACITY (-5)" in txt
+
+
+@pytest.mark.parametrize("nz", [16, 128, 512])
+def test_given_pad_bound(nz):
+    """A given pad times 2^32 fixed-point units, summed over a column of n_z
+    voxels, stays below 2^63: the bound is 2^31 / n_z atoms per voxel."""
+    lim = G.max_given_pad(nz)
+    assert lim * 2.0 ** 32 * nz <= 2.0 ** 63
+    assert np.nextafter(lim, np.inf) * 2.0 ** 32 * nz > 2.0 ** 63
