@@ -166,6 +166,22 @@ int saxs_forward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t
                  int32_t n_frames, const float* oc_diag, double* H, double* C,
                  void* stream);
 
+/* saxs_forward that allocates nothing (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * per-call buffer (sort keys and records, z and y spectra, block
+ * histograms) comes from one file-scope device array of 128 MiB that each
+ * GPU holds from the library's load until the process ends; the plan
+ * supplies only its tables, and its max_atoms / max_frames do not bound the
+ * call.  A call whose buffers exceed 128 MiB returns SAXS_EINVAL (about 5
+ * frames of 100,000 atoms at a 128^3 cell, q <= 0.6; call with fewer
+ * frames).  Results equal saxs_forward's.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (other streams, other threads, other
+ * plans) race and must be serialised by the caller. */
+int saxs_forward_static(saxs_plan* plan, const float* xyz, const int32_t* elem,
+                        int32_t n_atoms, int32_t n_frames, const float* oc_diag, double* H,
+                        double* C, void* stream);
+
 /* Per-frame histograms instead of an accumulated one: Hf, Cf are device
  * float64 [n_frames][num_bins] and are overwritten.  Used by the adjoint
  * (per-frame binned intensities). */

@@ -19,7 +19,7 @@ EXPORTED = (
     "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
     "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version", "saxs_plan_set_timing",
     "saxs_plan_kernel_times", "saxs_backward_ex", "saxs_plan_set_padding",
-    "saxs_plan_set_point_tables",
+    "saxs_plan_set_point_tables", "saxs_forward_static",
 )
 KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
                 "adj_x", "adj_y", "adj_z", "grad_sum", "plain")
@@ -90,6 +90,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_plan_set_point_tables.argtypes = [vp, vp, vp, i32, vp, vp]
     lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_forward_static.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, vp]
     lib.saxs_forward_plain.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
@@ -104,7 +105,7 @@ def load(path: Path = LIB_PATH):
     for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
                  "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy",
                  "saxs_plan_set_timing", "saxs_plan_kernel_times", "saxs_backward_ex",
-                 "saxs_plan_set_padding", "saxs_plan_set_point_tables"):
+                 "saxs_plan_set_padding", "saxs_plan_set_point_tables", "saxs_forward_static"):
         getattr(lib, name).restype = C.c_int
     if lib.saxs_abi_version() != ABI_VERSION:
         raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -16,6 +17,14 @@
 #include "saxs_kernels.hpp"
 #include "saxs_adjoint.hpp"
 #include "saxs_plain.hpp"
+
+// Device scratch of saxs_forward_static, the entry point that allocates
+// nothing: every per-call buffer of the forward chain is carved from this
+// array.  Each GPU holds its own copy from the library's load until the
+// process ends.  In host code the name is only a symbol handle: the address
+// of a device's copy comes from hipGetSymbolAddress (static_scratch below).
+constexpr size_t kStaticScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
 
 namespace {
 
@@ -495,20 +504,89 @@ void free_plan(saxs_plan* pl) {
   delete pl;
 }
 
-saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem) {
+// Per-call scratch of the forward chain: the plan's workspace, or carved
+// from the static scratch array by saxs_forward_static.
+struct Work {
+  int* counts;
+  int* offsets;
+  int2* keyslot;
+  float4* sorted;
+  unsigned long long* pad_sum;
+  int* segoff;
+  float2* zbuf;
+  float2* ybuf;
+  double* partial;
+};
+
+Work plan_work(const saxs_plan* pl) {
+  return Work{pl->counts, pl->offsets, pl->keyslot, pl->sorted, pl->pad_sum,
+              pl->segoff, pl->zbuf,    pl->ybuf,    pl->partial};
+}
+
+// Address of the current device's copy of `scratch`, looked up once per
+// device (with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SAXS_EUNSUP, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SAXS_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return SAXS_OK;
+}
+
+// Carves the forward chain's buffers for n_atoms x n_frames of the plan's
+// current geometry out of `base` (256-byte aligned pieces); returns the
+// bytes needed (base may be NULL to only measure).
+size_t carve_work(const saxs_plan* pl, int n_atoms, int n_frames, unsigned char* base, Work& w) {
+  const saxs::Params& p = pl->p;
+  const size_t F = n_frames, N = n_atoms, E = p.E;
+  const size_t kframe = E * p.nty * p.ntx, nkeys = F * kframe;
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* q = base ? (void*)(base + off) : nullptr;
+    off += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  w.counts = (int*)take((nkeys + 1) * sizeof(int));
+  w.offsets = (int*)take((nkeys + 1) * sizeof(int));
+  w.keyslot = (int2*)take(F * N * sizeof(int2));
+  w.sorted = (float4*)take(F * N * sizeof(float4));
+  w.pad_sum = (unsigned long long*)take(F * E * sizeof(unsigned long long));
+  w.segoff = pl->seg_sort ? (int*)take((512 + F) * kframe * sizeof(int)) : nullptr;
+  w.zbuf = (float2*)take(F * E * p.n[0] * p.n[1] * p.kz * sizeof(float2));
+  w.ybuf = (float2*)take(F * E * p.ky * p.n[0] * p.kz * sizeof(float2));
+  w.partial = (double*)take(F * p.ky * p.nchx * 2 * p.num_bins * sizeof(double));
+  return off;
+}
+
+saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem,
+                           const Work* w = nullptr) {
+  const Work W = w ? *w : plan_work(pl);
   saxs::Buffers b{};
   b.xyz = xyz;
   b.elem = elem;
-  b.counts = pl->counts;
-  b.offsets = pl->offsets;
-  b.keyslot = pl->keyslot;
-  b.sorted = pl->sorted;
-  b.pad_sum = pl->pad_sum;
+  b.counts = W.counts;
+  b.offsets = W.offsets;
+  b.keyslot = W.keyslot;
+  b.sorted = W.sorted;
+  b.pad_sum = W.pad_sum;
   b.pad_given = pl->pad_given_host.empty() ? nullptr : pl->pad_given;
   b.srcx = pl->srcx;
   b.srcy = pl->srcy;
-  b.zbuf = pl->zbuf;
-  b.ybuf = pl->ybuf;
+  b.zbuf = W.zbuf;
+  b.ybuf = W.ybuf;
   b.twz = pl->twz;
   b.twy = pl->twy;
   b.twx = pl->twx;
@@ -517,8 +595,8 @@ saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem)
   b.modz = pl->modz;
   b.lut_bin = pl->lut_bin;
   b.lut_ff = pl->lut_ff;
-  b.partial = pl->partial;
-  b.segoff = pl->segoff;
+  b.partial = W.partial;
+  b.segoff = W.segoff;
   b.nseg = 0;
   b.seglen = 1;
   b.segkr = 0;
@@ -556,9 +634,11 @@ saxs::Params call_params(saxs_plan* pl, int n_atoms, int n_frames, const float* 
 // Enqueues the forward pipeline up to the per-block histograms (or, with
 // with_x false, up to the y-pass output for the adjoint).
 int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms,
-                        int n_frames, const float* oc_diag, hipStream_t st, bool with_x = true) {
+                        int n_frames, const float* oc_diag, hipStream_t st, bool with_x = true,
+                        const Work* work = nullptr) {
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
-  saxs::Buffers b = make_buffers(pl, xyz, elem);
+  const Work W = work ? *work : plan_work(pl);
+  saxs::Buffers b = make_buffers(pl, xyz, elem, &W);
   const int kframe = p.E * p.nty * p.ntx;               // keys per frame
   const int nkeys = n_frames * kframe;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
@@ -579,8 +659,8 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     b.segkr = pl->seg_kr;
   }
   { Probe pr(pl, SAXS_K_CLEAR, st);
-  hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, pl->counts,
-                     nkeys + 1, pl->pad_sum, n_frames * p.E); }
+  hipLaunchKernelGGL(saxs::k_clear, dim3((nclear + 255) / 256), dim3(256), 0, st, W.counts,
+                     nkeys + 1, W.pad_sum, n_frames * p.E); }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_PREP, st);
     if (seg) {
@@ -609,7 +689,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   if (seg)
     hipLaunchKernelGGL(saxs::k_seg_keys, dim3((kframe + saxs::kWG - 1) / saxs::kWG, n_frames),
                        dim3(saxs::kWG), 0, st, p, b);
-  hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, pl->counts, pl->offsets,
+  hipLaunchKernelGGL(saxs::k_scan, dim3(n_frames), dim3(1024), 0, st, W.counts, W.offsets,
                      kframe, n_atoms); }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_SCATTER, st);
@@ -1023,6 +1103,42 @@ int saxs_forward(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n
     hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
                        pl->partial, nblk, n_frames, pl->p.num_bins, H, C, 0);
   }
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+int saxs_forward_static(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                        int32_t n_frames, const float* oc_diag, double* H, double* C,
+                        void* stream) {
+  g_err.clear();
+  if (!pl) return fail(SAXS_EINVAL, "plan is NULL");
+  if (n_atoms < 0) return fail(SAXS_EINVAL, "n_atoms %d < 0", n_atoms);
+  if (n_frames < 1 || (long long)n_frames * pl->p.E > 65535)
+    return fail(SAXS_EINVAL, "n_frames %d outside [1, 65535 / num_elements]", n_frames);
+  // the plan's capacity does not bound this call; the static scratch does
+  int rc = check_call(pl, xyz, elem, 0, 1, oc_diag);
+  if (rc) return rc;
+  if (n_atoms > 0 && (!xyz || !elem)) return fail(SAXS_EINVAL, "xyz/elem pointer is NULL");
+  if (!H || !C) return fail(SAXS_EINVAL, "H/C pointer is NULL");
+  Work w{};
+  const size_t need = carve_work(pl, n_atoms, n_frames, nullptr, w);
+  if (need > kStaticScratchBytes)
+    return fail(SAXS_EINVAL,
+                "%d frames of %d atoms need %zu bytes of static scratch, more than its %zu "
+                "(call with fewer frames)", n_frames, n_atoms, need, kStaticScratchBytes);
+  void* base = nullptr;
+  if ((rc = static_scratch(&base))) return rc;
+  carve_work(pl, n_atoms, n_frames, (unsigned char*)base, w);
+  hipStream_t st = (hipStream_t)stream;
+  pl->last_frames_valid = false;
+  const bool timing = pl->timer.on;      // no event creation on this path
+  pl->timer.on = false;
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, true, &w);
+  if (!rc)
+    hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
+                       w.partial, pl->p.ky * pl->p.nchx, n_frames, pl->p.num_bins, H, C, 0);
+  pl->timer.on = timing;
+  if (rc) return rc;
   HIP_TRY(hipGetLastError());
   return SAXS_OK;
 }
