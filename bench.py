@@ -193,10 +193,12 @@ def restraint_grad_check(keep, threads):
 class _MemoryTopology:
     """The duck-typed topology callbacks pycusaxs_cuda.run uses
     (RunSaxs.cu:109,124-125,137-138) over frames already in host memory, so
-    the drop-in path is timed without trajectory decoding."""
+    the drop-in path is timed without trajectory decoding.  With `boxes`,
+    frame i carries boxes[i] (a box that changes every frame, as in NPT)."""
 
-    def __init__(self, frames, symbols, box):
+    def __init__(self, frames, symbols, box, boxes=None):
         self.frames, self.box, self.ts = frames, np.asarray(box, dtype=np.float32), None
+        self.boxes = boxes
         self.index = {}
         for i, s in enumerate(symbols):
             self.index.setdefault(s, []).append(i)
@@ -208,31 +210,52 @@ class _MemoryTopology:
         self.ts = i
 
     def get_box(self):
-        return self.box
+        return self.box if self.boxes is None else self.boxes[self.ts or 0]
 
     def iter_frames_stream(self, start, stop, step=1):
         for i in range(start, min(stop, len(self.frames)), step):
-            yield {"frame": i, "time": float(i), "positions": self.frames[i], "box": self.box}
+            yield {"frame": i, "time": float(i), "positions": self.frames[i],
+                   "box": self.box if self.boxes is None else self.boxes[i]}
 
 
-def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps):
+def changing_boxes(kind, F, L=BOX):
+    """A box that changes every frame (NPT-like fluctuations of ~0.5 %):
+    cubic, orthorhombic (each axis its own), or triclinic (tilted b and c)."""
+    out = np.empty((F, 3, 3), dtype=np.float32)
+    for f in range(F):
+        a = 1.0 + 0.005 * np.sin(0.37 * f + np.array([0.0, 1.1, 2.3]))
+        if kind == "cubic":
+            out[f] = np.diag([L * a[0]] * 3)
+        elif kind == "orthorhombic":
+            out[f] = np.diag(L * a)
+        else:
+            out[f] = [[L * a[0], 0.0, 0.0], [0.03 * L * a[1], L * a[1], 0.0],
+                      [-0.02 * L * a[2], 0.025 * L * a[2], L * a[2]]]
+    return out
+
+
+def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simulation=""):
     """pycusaxs_cuda.run end to end on the trajectory workload: host frames
     (the 100 frames cycled to 1,000) staged through pinned double buffers and
     a copy stream, element gather on the device, batches of 16 frames.
     Steady-state rate from runs of 200 and 1,000 frames (the difference
-    removes plan setup and output)."""
+    removes plan setup and output).  With box_kind, the box changes every
+    frame (changing_boxes)."""
     import tempfile
     import pycusaxs_cuda
     syms = [symbols[i] for i in ids_h]
     F = 10 * len(xyz_h)                 # the host frames cycled, so setup is a small share
-    topo = _MemoryTopology([xyz_h[i % len(xyz_h)] for i in range(F)], syms, np.diag([BOX] * 3))
+    boxes = changing_boxes(box_kind, F) if box_kind else None
+    topo = _MemoryTopology([xyz_h[i % len(xyz_h)] for i in range(F)], syms, np.diag([BOX] * 3),
+                           boxes)
     out = os.path.join(tempfile.mkdtemp(), "saxs.dat")
 
     def run(end):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pycusaxs_cuda.run(topo, "memory", "memory", [GRID], end=end, output=out,
-                          scale_factor=SIGMA, bin_size=DQ, qcut=QCUT, quiet=True)
+                          scale_factor=SIGMA, bin_size=DQ, qcut=QCUT, quiet=True,
+                          simulation=simulation)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
@@ -245,7 +268,9 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps):
             "engine_only_frames_per_s": round(engine_fps, 1),
             "fraction_of_engine_rate": round(steady / engine_fps, 3),
             "workload": "pycusaxs_cuda.run on the trajectory workload from host memory "
-                        "(pinned double buffer, copy stream, 16 frames per launch)"}
+                        "(pinned double buffer, copy stream, 16 frames per launch)"
+                        + (f", {box_kind} box changing every frame ({simulation or 'npt'})"
+                           if box_kind else "")}
 
 
 def kernel_bytes(eng, frames_per_launch, n_atoms):
@@ -344,7 +369,9 @@ def main():
                          "(known HBM bytes, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the forward as eager launches only (no hipGraph replay)")
-    ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6"),
+    ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6",
+                                      "run_path_npt_cubic", "run_path_npt_orthorhombic",
+                                      "run_path_npt_triclinic"),
                     help="run only this secondary workload (for rocprofv3) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -356,6 +383,9 @@ def main():
     if args.leg:
         legs = {"batched_uneven": leg_batched_uneven, "large_1M_512": leg_large,
                 "restraint_50k_order6": lambda t, n: leg_restraint(t, n, {})}
+        for kind in ("cubic", "orthorhombic", "triclinic"):
+            legs[f"run_path_npt_{kind}"] = (
+                lambda t, n, k=kind: leg_run_path(t, *make_inputs()[:3], float("nan"), k))
         print(json.dumps({"leg": args.leg, **legs[args.leg](torch, max(1, args.steps))}), flush=True)
         return
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
@@ -511,7 +541,15 @@ def main():
         restraint = {}
         legs = (("batched_uneven", leg_batched_uneven), ("large_1M_512", leg_large),
                 ("restraint_50k_order6", lambda t, n: leg_restraint(t, n, restraint)),
-                ("run_path", lambda t, n: leg_run_path(t, xyz_h, symbols, ids_h, value)))
+                ("run_path", lambda t, n: leg_run_path(t, xyz_h, symbols, ids_h, value)),
+                ("run_path_npt_cubic", lambda t, n: leg_run_path(t, xyz_h, symbols, ids_h, value,
+                                                                 "cubic")),
+                ("run_path_npt_orthorhombic", lambda t, n: leg_run_path(
+                    t, xyz_h, symbols, ids_h, value, "orthorhombic")),
+                ("run_path_npt_triclinic", lambda t, n: leg_run_path(
+                    t, xyz_h, symbols, ids_h, value, "triclinic")),
+                ("run_path_nvt_triclinic", lambda t, n: leg_run_path(
+                    t, xyz_h, symbols, ids_h, value, "triclinic", "nvt")))
         for name, fn in legs:
             try:
                 line[name] = fn(torch, max(1, min(args.steps, 3)))

@@ -182,6 +182,47 @@ int saxs_forward_static(saxs_plan* plan, const float* xyz, const int32_t* elem,
                         int32_t n_atoms, int32_t n_frames, const float* oc_diag, double* H,
                         double* C, void* stream);
 
+/* Frames whose boxes differ, in one launch chain (NPT trajectories, where
+ * the box changes every frame; NVT with the first frame's binning).
+ * Replaces the per-frame runPKernel(..., oc) of RunSaxs.cu:180-193 with
+ * each frame's own inverse cell (saxsDeviceKernels.cu:355-361) and q
+ * (:84-91, 196-205), dq / qcut / bin count fixed by the plan (those of the
+ * first frame, saxsKernel.cu:489-526).  The per-frame q tables are built on
+ * the device (exact bin decisions on the float32 values).
+ *   cells      host [n_frames][9] row-major inverse cell M of each frame
+ *   qgeom      host [n_frames][9] q geometry of each frame: key_type 0 (all
+ *              reciprocal spacings equal): [0] = s; 1 (orthorhombic): [0..2]
+ *              = s_a; 2 (triclinic): rows b_aj of q = sum_a m_a b_a
+ *   bin_qgeom  host [9] geometry the points are binned with (NVT: the first
+ *              frame's), or NULL: each frame's own
+ *   key_type, kmax, lut_len  the table keys as for the plan's tables
+ *              (shell n = |m|^2 < lut_len; or the point box kmax), covering
+ *              every frame of the batch
+ *   dq, qcut   float32 bin width and cutoff of the run
+ *   form_coef  host [num_elements][9] Cromer-Mann a1..a4, b1..b4, c
+ *              (Scattering.cpp:9-189)
+ * The nonzero entries of a geometry must span at most 2^20 (the exact
+ * comparisons stay within 128 bits).  Accumulates into H, C like
+ * saxs_forward.  SAXS_ECAPACITY when the batch keeps more frequencies than
+ * the plan's workspace holds (make the plan with the largest box).  The
+ * first call allocates the per-frame tables (max_frames x the plan's table
+ * capacity) and a pinned staging buffer; a call waits for the previous
+ * call's upload before reusing the staging buffer. */
+typedef struct saxs_frame_boxes {
+  const float* cells;
+  const float* qgeom;
+  const float* bin_qgeom;
+  int32_t key_type;
+  int32_t kmax[3];
+  int32_t lut_len;
+  float dq;
+  float qcut;
+  const float* form_coef;
+} saxs_frame_boxes;
+int saxs_forward_boxes(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                       int32_t n_frames, const saxs_frame_boxes* boxes, double* H, double* C,
+                       void* stream);
+
 /* Per-frame histograms instead of an accumulated one: Hf, Cf are device
  * float64 [n_frames][num_bins] and are overwritten.  Used by the adjoint
  * (per-frame binned intensities). */

@@ -11,7 +11,7 @@ from pathlib import Path
 from ._build import LIB_PATH
 
 SAXS_OK, SAXS_EINVAL, SAXS_ENOMEM, SAXS_EHIP, SAXS_EUNSUP, SAXS_ECAPACITY = 0, -1, -2, -3, -4, -5
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # every symbol include/saxs_engine.h declares
 EXPORTED = (
@@ -19,7 +19,7 @@ EXPORTED = (
     "saxs_backward", "saxs_forward_plain", "saxs_plan_workspace_bytes", "saxs_plan_kept",
     "saxs_plan_destroy", "saxs_last_error", "saxs_abi_version", "saxs_plan_set_timing",
     "saxs_plan_kernel_times", "saxs_backward_ex", "saxs_plan_set_padding",
-    "saxs_plan_set_point_tables", "saxs_forward_static",
+    "saxs_plan_set_point_tables", "saxs_forward_static", "saxs_forward_boxes",
 )
 KERNEL_NAMES = ("clear", "prep", "scan", "scatter", "spread_z", "yfft", "xfft_epi", "reduce",
                 "adj_x", "adj_y", "adj_z", "grad_sum", "plain")
@@ -46,6 +46,20 @@ class SaxsDesc(C.Structure):
         ("flags", C.c_int32),
         ("kmax", C.c_int32 * 3),
         ("cell", C.c_float * 9),
+    ]
+
+
+class SaxsFrameBoxes(C.Structure):
+    _fields_ = [
+        ("cells", C.POINTER(C.c_float)),
+        ("qgeom", C.POINTER(C.c_float)),
+        ("bin_qgeom", C.POINTER(C.c_float)),
+        ("key_type", C.c_int32),
+        ("kmax", C.c_int32 * 3),
+        ("lut_len", C.c_int32),
+        ("dq", C.c_float),
+        ("qcut", C.c_float),
+        ("form_coef", C.POINTER(C.c_float)),
     ]
 
 
@@ -91,6 +105,7 @@ def load(path: Path = LIB_PATH):
     lib.saxs_forward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_frames.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_forward_static.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
+    lib.saxs_forward_boxes.argtypes = [vp, vp, vp, i32, i32, C.POINTER(SaxsFrameBoxes), vp, vp, vp]
     lib.saxs_backward.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.saxs_backward_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, vp]
     lib.saxs_forward_plain.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
@@ -105,7 +120,8 @@ def load(path: Path = LIB_PATH):
     for name in ("saxs_plan_create", "saxs_plan_set_tables", "saxs_forward", "saxs_forward_frames",
                  "saxs_backward", "saxs_forward_plain", "saxs_plan_kept", "saxs_plan_destroy",
                  "saxs_plan_set_timing", "saxs_plan_kernel_times", "saxs_backward_ex",
-                 "saxs_plan_set_padding", "saxs_plan_set_point_tables", "saxs_forward_static"):
+                 "saxs_plan_set_padding", "saxs_plan_set_point_tables", "saxs_forward_static",
+                 "saxs_forward_boxes"):
         getattr(lib, name).restype = C.c_int
     if lib.saxs_abi_version() != ABI_VERSION:
         raise ImportError(f"engine ABI {lib.saxs_abi_version()} != expected {ABI_VERSION}")

@@ -262,7 +262,7 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
     for fr in frames_iter:
         frame, t, pos, box = _frame_fields(fr)
         log(f"--> Frame: {frame:<7}  Time Step: {t:.2f} ps")
-        if stager.count and (not np.array_equal(box, stager.box) or stager.count >= batch):
+        if stager.count >= batch:
             flush()
         stager.add(pos, box)
         n_done += 1
@@ -285,7 +285,9 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
 class FrameStager:
     """Double-buffered frame upload (the reference's double buffer,
     RunSaxs.cu:141-170, without its per-atom host copies): each frame is
-    copied whole into one of two pinned buffers [batch, atoms, 3]; a batch is
+    copied whole into one of two pinned buffers [batch, atoms, 3], its box
+    kept beside it (a batch whose boxes differ runs with per-frame cells and
+    q tables, saxs_forward_boxes); a batch is
     uploaded to its device buffer on a separate copy stream, and the compute
     stream waits for that copy only, gathers the atoms into element order
     (one index_select) and runs the engine.  So batch k+1 is read and uploaded
@@ -307,7 +309,7 @@ class FrameStager:
         self.consumed = [None, None]
         self.slot = 0
         self.count = 0
-        self.box = None
+        self.boxes = np.zeros((batch, 3, 3), dtype=np.float32)
 
     def _allocate(self, n_frame_atoms: int):
         import torch
@@ -328,8 +330,8 @@ class FrameStager:
         if self.count == 0 and self.copied[self.slot] is not None:
             self.copied[self.slot].synchronize()          # host buffer is free again
         np.copyto(self.host_np[self.slot][self.count], pos)
+        self.boxes[self.count] = box
         self.count += 1
-        self.box = box
 
     def launch(self, eng, ids_t, bin_box=None):
         import torch
@@ -349,7 +351,14 @@ class FrameStager:
             xyz = self.sel[:k]
             if self.n_atoms:
                 torch.index_select(self.dev[s][:k], 1, self.idx_t, out=xyz)
-            eng.forward(xyz, ids_t, box=self.box, bin_box=bin_box)
+            boxes = self.boxes[:k]
+            same = bool(np.all(boxes == boxes[0]))
+            if same and (bin_box is None or np.array_equal(boxes[0], bin_box)):
+                eng.forward(xyz, ids_t, box=boxes[0], bin_box=bin_box)
+            else:
+                # boxes that change within the batch (or, in NVT, differ from
+                # the first frame's): one launch chain with per-frame cells
+                eng.forward_boxes(xyz, ids_t, boxes.copy(), bin_box=bin_box)
         finally:
             done = torch.cuda.Event()
             done.record(compute)

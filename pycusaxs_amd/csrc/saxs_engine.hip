@@ -17,6 +17,7 @@
 #include "saxs_kernels.hpp"
 #include "saxs_adjoint.hpp"
 #include "saxs_plain.hpp"
+#include "saxs_tables.hpp"
 
 // Device scratch of saxs_forward_static, the entry point that allocates
 // nothing: every per-call buffer of the forward chain is carved from this
@@ -348,6 +349,16 @@ struct saxs_plan {
   bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
   int x_gen_h = 0;             // x pass by k_xfft_g<x_gen_h>, its adjoint by k_adj_x_g (0: Stockham)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
+  // saxs_forward_boxes: per-frame cells, q geometries, coefficients and
+  // tables (allocated on first use), and the pinned staging of the uploads
+  float* fb_cells = nullptr;   // [max_frames][9]
+  float* fb_geom = nullptr;    // [max_frames + 1][9] (the last row: binning geometry)
+  float* fb_coef = nullptr;    // [E][9]
+  int* fb_bin = nullptr;       // [max_frames][lut_cap]
+  float* fb_ff = nullptr;      // [max_frames][E][lut_cap]
+  float* fb_host = nullptr;    // pinned [max_frames * 18 + 9 + 9 E]
+  hipEvent_t fb_copied = nullptr;
+  bool fb_pending = false;
 };
 
 namespace {
@@ -494,6 +505,11 @@ void free_plan(saxs_plan* pl) {
     (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
+  if (pl->fb_copied) (void)hipEventDestroy(pl->fb_copied);
+  if (pl->fb_host) (void)hipHostFree(pl->fb_host);
+  void* fbp[] = {pl->fb_cells, pl->fb_geom, pl->fb_coef, pl->fb_bin, pl->fb_ff};
+  for (void* q : fbp)
+    if (q) (void)hipFree(q);
   void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
@@ -516,6 +532,11 @@ struct Work {
   float2* zbuf;
   float2* ybuf;
   double* partial;
+  // per-frame geometry of a batch of boxes (saxs_forward_boxes), else NULL / 0
+  const float* cells = nullptr;
+  const int* lut_bin = nullptr;
+  const float* lut_ff = nullptr;
+  int lut_fstride = 0;
 };
 
 Work plan_work(const saxs_plan* pl) {
@@ -593,8 +614,10 @@ saxs::Buffers make_buffers(saxs_plan* pl, const float* xyz, const int32_t* elem,
   b.modx = pl->modx;
   b.mody = pl->mody;
   b.modz = pl->modz;
-  b.lut_bin = pl->lut_bin;
-  b.lut_ff = pl->lut_ff;
+  b.lut_bin = W.lut_bin ? W.lut_bin : pl->lut_bin;
+  b.lut_ff = W.lut_ff ? W.lut_ff : pl->lut_ff;
+  b.cells = W.cells;
+  b.lut_fstride = W.lut_fstride;
   b.partial = W.partial;
   b.segoff = W.segoff;
   b.nseg = 0;
@@ -845,7 +868,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
 
 extern "C" {
 
-int32_t saxs_abi_version(void) { return 5; }
+int32_t saxs_abi_version(void) { return 6; }
 
 const char* saxs_last_error(void) { return g_err.c_str(); }
 
@@ -1138,6 +1161,119 @@ int saxs_forward_static(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
                        w.partial, pl->p.ky * pl->p.nchx, n_frames, pl->p.num_bins, H, C, 0);
   pl->timer.on = timing;
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return SAXS_OK;
+}
+
+int saxs_forward_boxes(saxs_plan* pl, const float* xyz, const int32_t* elem, int32_t n_atoms,
+                       int32_t n_frames, const saxs_frame_boxes* fb, double* H, double* C,
+                       void* stream) {
+  g_err.clear();
+  if (!pl || !fb) return fail(SAXS_EINVAL, "plan/boxes is NULL");
+  if (n_atoms < 0 || n_atoms > pl->max_atoms)
+    return fail(SAXS_EINVAL, "n_atoms %d outside [0, %d]", n_atoms, pl->max_atoms);
+  if (n_frames < 1 || n_frames > pl->max_frames)
+    return fail(SAXS_EINVAL, "n_frames %d outside [1, %d]", n_frames, pl->max_frames);
+  if (n_atoms > 0 && (!xyz || !elem)) return fail(SAXS_EINVAL, "xyz/elem pointer is NULL");
+  if (!H || !C || !fb->cells || !fb->qgeom || !fb->form_coef)
+    return fail(SAXS_EINVAL, "H/C/cells/qgeom/form_coef pointer is NULL");
+  if (fb->key_type < 0 || fb->key_type > 2) return fail(SAXS_EINVAL, "key_type %d", fb->key_type);
+  if (fb->lut_len < 1) return fail(SAXS_EINVAL, "lut_len %d < 1", fb->lut_len);
+  if (fb->key_type > 0) {
+    const int rc = check_point_box(fb->kmax, fb->lut_len, fb->key_type == 2);
+    if (rc) return rc;
+  }
+  if (fb->lut_len > pl->lut_cap)
+    return fail(SAXS_ECAPACITY, "lut_len %d exceeds the plan's %d (create a plan for the "
+                "largest box)", fb->lut_len, pl->lut_cap);
+  for (int f = 0; f < n_frames; ++f)
+    for (int a = 0; a < 3; ++a)
+      if (!(fb->cells[9 * f + 4 * a] > 0.0f) || !std::isfinite(fb->cells[9 * f + 4 * a]))
+        return fail(SAXS_EINVAL, "frame %d: cell diagonal %d = %g is not a positive inverse "
+                    "length", f, a, (double)fb->cells[9 * f + 4 * a]);
+  // kept ranges of the batch's key box (restored when the launches are queued)
+  const saxs::Params saved = pl->p;
+  const bool saved_xp = pl->x_pruned;
+  const int saved_xg = pl->x_gen_h;
+  auto restore = [&]() {
+    pl->p = saved;
+    pl->x_pruned = saved_xp;
+    pl->x_gen_h = saved_xg;
+  };
+  pl->p.point_keys = fb->key_type;
+  for (int a = 0; a < 3; ++a) pl->p.kmax[a] = fb->key_type ? fb->kmax[a] : 0;
+  int rc = derive_ranges(pl, fb->lut_len);
+  if (!rc && (pl->p.kz > pl->kz_cap || pl->p.ky > pl->ky_cap || pl->p.nchx > pl->nchx_cap))
+    rc = fail(SAXS_ECAPACITY, "the batch keeps kz %d, ky %d, %d x chunks; the workspace holds "
+              "%d, %d, %d (create a plan for the largest box)", pl->p.kz, pl->p.ky, pl->p.nchx,
+              pl->kz_cap, pl->ky_cap, pl->nchx_cap);
+  if (rc) {
+    restore();
+    return rc;
+  }
+  const size_t MF = pl->max_frames, E = pl->p.E;
+  size_t tot = pl->bytes;
+  int r = 0;
+  if (!pl->fb_cells) {
+    if ((r = dev_alloc(&pl->fb_cells, MF * 9, tot)) || (r = dev_alloc(&pl->fb_geom, (MF + 1) * 9, tot)) ||
+        (r = dev_alloc(&pl->fb_coef, E * 9, tot)) ||
+        (r = dev_alloc(&pl->fb_bin, MF * (size_t)pl->lut_cap, tot)) ||
+        (r = dev_alloc(&pl->fb_ff, MF * E * (size_t)pl->lut_cap, tot))) {
+      restore();
+      return r;
+    }
+    if (hipHostMalloc((void**)&pl->fb_host, sizeof(float) * (MF * 18 + 9 + 9 * E)) != hipSuccess ||
+        hipEventCreateWithFlags(&pl->fb_copied, hipEventDisableTiming) != hipSuccess) {
+      restore();
+      return fail(SAXS_ENOMEM, "pinned staging for per-frame boxes failed");
+    }
+    pl->bytes = tot;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  // the previous batch's upload must have left the staging buffer
+  if (pl->fb_pending && hipEventSynchronize(pl->fb_copied) != hipSuccess) {
+    restore();
+    return fail(SAXS_EHIP, "waiting for the previous upload failed");
+  }
+  float* hc = pl->fb_host;
+  float* hg = hc + MF * 9;
+  float* hk = hg + (MF + 1) * 9;
+  std::memcpy(hc, fb->cells, sizeof(float) * 9 * n_frames);
+  std::memcpy(hg, fb->qgeom, sizeof(float) * 9 * n_frames);
+  if (fb->bin_qgeom) std::memcpy(hg + MF * 9, fb->bin_qgeom, sizeof(float) * 9);
+  std::memcpy(hk, fb->form_coef, sizeof(float) * 9 * E);
+  if (hipMemcpyAsync(pl->fb_cells, hc, sizeof(float) * 9 * n_frames, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(pl->fb_geom, hg, sizeof(float) * 9 * n_frames, hipMemcpyHostToDevice, st) ||
+      (fb->bin_qgeom && hipMemcpyAsync(pl->fb_geom + MF * 9, hg + MF * 9, sizeof(float) * 9,
+                                       hipMemcpyHostToDevice, st)) ||
+      hipMemcpyAsync(pl->fb_coef, hk, sizeof(float) * 9 * E, hipMemcpyHostToDevice, st) ||
+      hipEventRecord(pl->fb_copied, st)) {
+    restore();
+    return fail(SAXS_EHIP, "upload of the per-frame boxes failed");
+  }
+  pl->fb_pending = true;
+  pl->last_frames_valid = false;
+  saxs::Params tp = pl->p;
+  tp.F = n_frames;
+  tp.lut_len = fb->lut_len;
+  hipLaunchKernelGGL(saxs::k_frame_tables, dim3((fb->lut_len + saxs::kWG - 1) / saxs::kWG, n_frames),
+                     dim3(saxs::kWG), 0, st, tp, pl->fb_geom,
+                     fb->bin_qgeom ? pl->fb_geom + MF * 9 : nullptr, pl->fb_coef, fb->dq, fb->qcut,
+                     pl->p.num_bins, pl->fb_bin, pl->fb_ff);
+  Work w = plan_work(pl);
+  w.cells = pl->fb_cells;
+  w.lut_bin = pl->fb_bin;
+  w.lut_ff = pl->fb_ff;
+  w.lut_fstride = fb->lut_len;
+  const float oc1[3] = {fb->cells[0], fb->cells[4], fb->cells[8]};
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc1, st, true, &w);
+  if (!rc) {
+    Probe pr(pl, SAXS_K_REDUCE, st);
+    hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, 1), dim3(saxs::kWG), 0, st,
+                       pl->partial, pl->p.ky * pl->p.nchx, n_frames, pl->p.num_bins, H, C, 0);
+  }
+  restore();
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   return SAXS_OK;

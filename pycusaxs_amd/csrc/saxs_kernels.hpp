@@ -74,6 +74,9 @@ struct Buffers {
   const float* lut_ff;   // [E][lut_len]
   double* partial;       // [F][ky][nchx][2*num_bins]
   int* segoff;           // [F][nseg][keys per frame] segmented sort (nseg > 0)
+  const float* cells;    // [F][9] inverse cell of each frame (row-major), or NULL: the plan's
+  int lut_fstride;       // 0: one q table for all frames; else frame f's tables start at
+                         // lut_bin + f lut_fstride and lut_ff + f E lut_fstride
   int nseg, seglen;      // segments per frame and atoms per segment
   int segkr;             // keys per k_prep_seg pass
 };
@@ -142,6 +145,24 @@ __device__ __forceinline__ void cell_coords(const float* x, const Params& p, flo
   for (int a = 0; a < 3; ++a) {
     const float u = __fadd_rn(__fadd_rn(__fmul_rn(p.cm[3 * a], x[0]), __fmul_rn(p.cm[3 * a + 1], x[1])),
                               __fmul_rn(p.cm[3 * a + 2], x[2]));
+    r[a] = cell_coord_u(u, p.n[a]);
+  }
+}
+
+// Cell coordinates of position x in frame f: the frame's own inverse cell
+// when the batch carries one per frame (u_a = (M_a0 x + M_a1 y) + M_a2 z,
+// which equals M_aa x_a bit for bit when M is diagonal), else the plan's.
+__device__ __forceinline__ void frame_cell_coords(const float* x, const Params& p,
+                                                  const Buffers& b, int f, float r[3]) {
+  if (!b.cells) {
+    cell_coords(x, p, r);
+    return;
+  }
+  const float* m = b.cells + (size_t)f * 9;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float u = __fadd_rn(__fadd_rn(__fmul_rn(m[3 * a], x[0]), __fmul_rn(m[3 * a + 1], x[1])),
+                              __fmul_rn(m[3 * a + 2], x[2]));
     r[a] = cell_coord_u(u, p.n[a]);
   }
 }
@@ -239,7 +260,7 @@ __device__ __forceinline__ void atom_key_border(const Params& p, const Buffers& 
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
   int base[3];
   float w[3], rr[3];
-  cell_coords(x, p, rr);
+  frame_cell_coords(x, p, b, f, rr);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float r = rr[a];
@@ -443,7 +464,7 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   if (ks.x < 0) return;
   float4 rec;
   float rr[3];
-  cell_coords(x, p, rr);
+  frame_cell_coords(x, p, b, f, rr);
   rec.x = rr[0];
   rec.y = rr[1];
   rec.z = rr[2];
@@ -1605,7 +1626,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
     }
     __syncthreads();
     fft_lines<kMaxC, POW2>(buf, p.lsx, kc, p.fx, tw);
-    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+    const float* ff = b.lut_ff + ((size_t)f * p.E * b.lut_fstride + (size_t)e * p.lut_len);
 #pragma unroll
     for (int u = 0; u < kOutMax; ++u) {
       const int idx = threadIdx.x + u * blockDim.x;
@@ -1635,7 +1656,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
       const int kz = k0 + k;
       const int nsq = shell_key(mx, my, kz, p);
       if (nsq < p.lut_len) {
-        const int bin = b.lut_bin[nsq];
+        const int bin = b.lut_bin[(size_t)f * b.lut_fstride + nsq];
         if (bin >= 0 && bin < nb) {
           const float mod = b.modx[ix] * b.mody[iy] * b.modz[kz];
           const float I = (A[u].x * A[u].x + A[u].y * A[u].y) * mod;
@@ -1716,7 +1737,7 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
       for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
     __syncthreads();
-    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+    const float* ff = b.lut_ff + ((size_t)f * p.E * b.lut_fstride + (size_t)e * p.lut_len);
 #pragma unroll
     for (int u = 0; u < kXItems; ++u) {
       const int item = threadIdx.x + u * blockDim.x;
@@ -1764,7 +1785,7 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
           const int mx = m < p.nf[0] ? m : m - N;
           const int nsq = shell_key(mx, my, mz, p);
           if (nsq < p.lut_len) {
-            const int bin = b.lut_bin[nsq];
+            const int bin = b.lut_bin[(size_t)f * b.lut_fstride + nsq];
             if (bin >= 0 && bin < nb) {
               const float mod = b.modx[m] * b.mody[iy] * b.modz[mz];
               const float I = (A[u][o].x * A[u][o].x + A[u][o].y * A[u][o].y) * mod;
@@ -1852,7 +1873,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
       }
     }
     __syncthreads();
-    const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+    const float* ff = b.lut_ff + ((size_t)f * p.E * b.lut_fstride + (size_t)e * p.lut_len);
 #pragma unroll
     for (int u = 0; u < kXgItems; ++u) {
       const int item = threadIdx.x + u * blockDim.x;
@@ -1903,7 +1924,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
         const int ix = o == 0 ? m : N - m, mx = o == 0 ? m : -m;
         const int nsq = shell_key(mx, my, mz, p);
         if (nsq < p.lut_len) {
-          const int bin = b.lut_bin[nsq];
+          const int bin = b.lut_bin[(size_t)f * b.lut_fstride + nsq];
           if (bin >= 0 && bin < nb) {
             const float mod = b.modx[ix] * b.mody[iy] * b.modz[mz];
             const float I = (A[u][o].x * A[u][o].x + A[u][o].y * A[u][o].y) * mod;

@@ -18,8 +18,8 @@ import torch
 from . import _lib
 from .geometry import (Histogram, border_planes, bspline_moduli, fft_size, form_factor_points,
                        form_factor_table, form_factor_vectors, half_spectrum_counts, hist_params,
-                       inverse_box_matrix, is_triclinic, max_given_pad, point_bins, point_kmax,
-                       reciprocal_basis,
+                       inverse_box_matrices, inverse_box_matrix, is_triclinic, max_given_pad,
+                       point_bins, point_kmax, reciprocal_bases, reciprocal_basis,
                        shell_bins, triclinic_bins, triclinic_kmax)
 
 _NO_BIN_LIMIT = 2 ** 31 - 1     # bins past num_bins still mark q <= qcut
@@ -300,6 +300,128 @@ class SaxsEngine:
                 oc.ctypes.data_as(C.c_void_p), _ptr(self.H), _ptr(self.C), _stream_handle()),
                 "saxs_forward")
         self.frames_done += F
+
+    # -- frames whose boxes differ ----------------------------------------
+    def _batch_geometry(self, cells, bin_cell=None):
+        """Key type, kept box, table length and per-frame q geometry for a
+        stack of inverse cells [F, 3, 3] (saxs_forward_boxes), or None when a
+        geometry spans more than 2^19 in magnitude (the device's exact
+        comparisons)."""
+        n, nn = self.spec.n, self.spec.nn
+        h = self.hist
+        F = len(cells)
+        allc = cells if bin_cell is None else np.concatenate([cells, bin_cell[None]])
+        bases = reciprocal_bases(allc, n, nn)
+        a = np.abs(bases.reshape(len(bases), 9)).astype(np.float64)
+        big = a.max(axis=1)
+        small = np.where(a > 0, a, np.inf).min(axis=1)
+        if np.any(np.log2(big / small) > 19.0):
+            return None
+        offd = bases.copy()
+        offd[:, [0, 1, 2], [0, 1, 2]] = 0
+        diag = np.diagonal(bases, axis1=1, axis2=2)
+        bgeom = None
+        qn, qd = float(h.qcut).as_integer_ratio()
+        if np.any(offd != 0):
+            kmax = [0, 0, 0]
+            for bm in bases[:F]:
+                kmax = [max(x, y) for x, y in zip(kmax, triclinic_kmax(bm, h.qcut, nn))]
+            geom = bases[:F].reshape(F, 9)
+            if bin_cell is not None:
+                bgeom = bases[F].ravel()
+            return 2, kmax, int(np.prod([2 * k + 1 for k in kmax])), geom, bgeom
+        if np.all((diag[:, 0] == diag[:, 1]) & (diag[:, 1] == diag[:, 2])):
+            n_max = 0
+            for s in set(float(v) for v in diag[:F, 0]):
+                sn, sd = s.as_integer_ratio()
+                n_max = max(n_max, (qn * qn * sd * sd) // (sn * sn * qd * qd))
+            geom = np.zeros((F, 9), dtype=np.float32)
+            geom[:, 0] = diag[:F, 0]
+            if bin_cell is not None:
+                bgeom = np.zeros(9, dtype=np.float32)
+                bgeom[0] = diag[F, 0]
+            return 0, [0, 0, 0], int(n_max) + 1, geom, bgeom
+        kmax = [0, 0, 0]
+        for d in set(tuple(float(v) for v in row) for row in diag[:F]):
+            km = point_kmax(d, h.qcut)
+            kmax = [max(x, min(y, v // 2)) for x, y, v in zip(kmax, km, nn)]
+        geom = np.zeros((F, 9), dtype=np.float32)
+        geom[:, :3] = diag[:F]
+        if bin_cell is not None:
+            bgeom = np.zeros(9, dtype=np.float32)
+            bgeom[:3] = diag[F]
+        return 1, kmax, int(np.prod([k + 1 for k in kmax])), geom, bgeom
+
+    def _form_coefficients(self) -> np.ndarray:
+        if getattr(self, "_coef", None) is None:
+            from .formfactors import coefficients
+            self._coef = np.ascontiguousarray([coefficients(s) for s in self.spec.symbols],
+                                              dtype=np.float32)
+        return self._coef
+
+    def forward_boxes(self, xyz: torch.Tensor, elem: torch.Tensor, boxes, bin_box=None):
+        """Accumulates frames with one box each ([frames, 3, 3]) into H, C in
+        one launch chain per max_frames chunk (saxs_forward_boxes): each frame
+        is spread with its own inverse cell and its points take its own q,
+        binned with the run's dq, qcut and bins (NVT: binned by bin_box's q).
+        A batch that needs more kept frequencies than the plan holds makes a
+        plan for its largest box; geometries outside the device's exact range
+        go frame by frame through forward(box=...)."""
+        self._last = None
+        xyz, elem = self._prep(xyz, elem)
+        F = xyz.shape[0]
+        boxes = np.asarray(boxes, dtype=np.float32).reshape(F, 3, 3)
+        cells = inverse_box_matrices(boxes)
+        bin_cell = None if bin_box is None else inverse_box_matrix(bin_box)
+        geo = self._batch_geometry(cells, bin_cell)
+        if geo is None:
+            for f in range(F):
+                self.forward(xyz[f:f + 1], elem, box=boxes[f], bin_box=bin_box)
+            return
+        key_type, kmax, lut_len, geom, bgeom = geo
+        coef = self._form_coefficients()
+        cm = np.ascontiguousarray(cells.reshape(F, 9), dtype=np.float32)
+        geom = np.ascontiguousarray(geom, dtype=np.float32)
+        bg = None if bgeom is None else np.ascontiguousarray(bgeom, dtype=np.float32)
+        for f0 in range(0, F, self.max_frames):
+            k = min(self.max_frames, F - f0)
+            fb = _lib.SaxsFrameBoxes()
+            fb.cells = cm[f0:f0 + k].ctypes.data_as(C.POINTER(C.c_float))
+            fb.qgeom = geom[f0:f0 + k].ctypes.data_as(C.POINTER(C.c_float))
+            fb.bin_qgeom = bg.ctypes.data_as(C.POINTER(C.c_float)) if bg is not None else None
+            fb.key_type = key_type
+            fb.kmax[:] = [int(v) for v in kmax]
+            fb.lut_len = lut_len
+            fb.dq = float(self.hist.dq)
+            fb.qcut = float(self.hist.qcut)
+            fb.form_coef = coef.ctypes.data_as(C.POINTER(C.c_float))
+            chunk = xyz[f0:f0 + k]
+            args = (self._plan, _ptr(chunk), _ptr(elem), int(chunk.shape[1]), int(k), C.byref(fb),
+                    _ptr(self.H), _ptr(self.C), _stream_handle())
+            try:
+                _lib.check(self.lib.saxs_forward_boxes(*args), "saxs_forward_boxes")
+            except _lib.CapacityError:
+                # a plan for the batch's largest box (its most kept frequencies)
+                big = f0 + int(np.argmax([np.prod(np.diag(b)) for b in boxes[f0:f0 + k]]))
+                self._rebuild_for_box(boxes[big])
+                args = (self._plan,) + args[1:]
+                try:
+                    _lib.check(self.lib.saxs_forward_boxes(*args), "saxs_forward_boxes")
+                except _lib.CapacityError:
+                    frames_done = self.frames_done
+                    for f in range(f0, f0 + k):
+                        self.forward(xyz[f:f + 1], elem, box=boxes[f], bin_box=bin_box)
+                    self.frames_done = frames_done
+        self.frames_done += F
+
+    def _rebuild_for_box(self, box):
+        """A plan whose workspace is sized for `box` (its q tables)."""
+        cell = inverse_box_matrix(box)
+        bins, ff, kmax = self._q_tables(self._hist_for(cell), cell)
+        self._rebuild_plan(cell, bins, ff, kmax)
+        self._box_key = tuple(float(v) for v in cell.ravel())
+        self.cell = cell
+        self.oc = np.ascontiguousarray(np.diag(cell), dtype=np.float32)
 
     def forward_frames(self, xyz: torch.Tensor, elem: torch.Tensor, box=None):
         """Per-frame (H, C) as [frames, num_bins] float64 tensors."""

@@ -114,6 +114,38 @@ def inverse_box_matrix(box) -> np.ndarray:
     return M
 
 
+def inverse_box_matrices(boxes) -> np.ndarray:
+    """inverse_box_matrix of a stack of boxes [F, 3, 3], the same float32
+    operations elementwise (bit for bit the per-box result)."""
+    b = np.asarray(boxes, dtype=np.float32).reshape(-1, 3, 3)
+    if np.any(b[:, 0, 1] != 0) or np.any(b[:, 0, 2] != 0) or np.any(b[:, 1, 2] != 0):
+        raise ValueError("box must be lower-triangular: rows a = (ax, 0, 0), b = (bx, by, 0), "
+                         "c = (cx, cy, cz)")
+    d = np.diagonal(b, axis1=1, axis2=2)
+    if np.any(~np.isfinite(b)) or np.any(d <= 0):
+        raise ValueError("box diagonal must be positive and finite")
+    t = np.transpose(b, (0, 2, 1))
+    M = np.zeros_like(b)
+    one = np.float64(1.0)
+    M[:, 0, 0] = (one / t[:, 0, 0].astype(np.float64)).astype(np.float32)
+    M[:, 1, 1] = (one / t[:, 1, 1].astype(np.float64)).astype(np.float32)
+    M[:, 2, 2] = (one / t[:, 2, 2].astype(np.float64)).astype(np.float32)
+    M[:, 0, 1] = -t[:, 0, 1] / (t[:, 0, 0] * t[:, 1, 1])
+    M[:, 1, 2] = -t[:, 1, 2] / (t[:, 1, 1] * t[:, 2, 2])
+    M[:, 0, 2] = (t[:, 0, 1] * t[:, 1, 2] - t[:, 0, 2] * t[:, 1, 1]) / ((t[:, 0, 0] * t[:, 1, 1])
+                                                                       * t[:, 2, 2])
+    return M
+
+
+def reciprocal_bases(Ms, n: Sequence[int], nn: Sequence[int]) -> np.ndarray:
+    """reciprocal_basis of a stack of inverse cells [F, 3, 3] (float32,
+    the same operations elementwise)."""
+    Ms = np.asarray(Ms, dtype=np.float32).reshape(-1, 3, 3)
+    tp = F32(F32(2.0) * F32(math.pi))
+    ratio = np.array([F32(F32(nn[a]) / F32(n[a])) for a in range(3)], dtype=np.float32)
+    return ((tp * Ms) / ratio[None, :, None]).astype(np.float32)
+
+
 def is_triclinic(M) -> bool:
     M = np.asarray(M)
     return bool(np.any(M != np.diag(np.diag(M))))

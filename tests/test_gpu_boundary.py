@@ -205,18 +205,33 @@ def test_engine_box_growth_makes_a_plan_and_keeps_the_accumulators(gpu):
     assert torch.all(H0[1:] <= torch.from_numpy(H[1:]).to(H0.device) + 1e-9 * H0[1:].abs())
 
 
-def test_run_nvt_rectangular_after_triclinic_raises(gpu, tmp_path):
-    """NVT bins every frame with the first frame's (triclinic) signed keys; a
-    rectangular frame after it cannot carry them, and run() raises instead
-    of writing a saxs.dat without that frame."""
-    base, syms = O.synthetic_system(1000, 37.0, seed=33)
+def test_run_nvt_rectangular_after_triclinic(gpu, tmp_path):
+    """NVT bins every frame with the first frame's (triclinic) q; a
+    rectangular frame after it runs in the same batch with signed point keys
+    (per-frame tables on the device), against the oracle's NVT restatement."""
+    boxes = [TRIC_A, np.diag([37.0] * 3).astype(np.float32), TRIC_B]
+    base, syms = O.synthetic_system(1500, 37.0, seed=33)
     frames = [O.frame_positions(base, 33, f, 0.3) for f in range(3)]
-    topo = FakeTopology(frames, syms, TRIC_A)
-    topo.boxes = [TRIC_A, np.diag([37.0] * 3).astype(np.float32), TRIC_A]
+    topo = FakeTopology(frames, syms, boxes[0])
+    topo.boxes = boxes
     out = tmp_path / "x.dat"
-    with pytest.raises(ValueError, match="triclinic"):
-        pycusaxs_cuda.run(topo, "sys.tpr", "traj.xtc", [48], end=2, output=str(out),
-                          scale_factor=2.0, bin_size=0.02, qcut=0.8, simulation="nvt")
+    pycusaxs_cuda.run(topo, "sys.tpr", "traj.xtc", [48], end=2, output=str(out),
+                      scale_factor=2.0, bin_size=0.02, qcut=0.8, simulation="nvt")
+    symbols, ids = O.element_ids(syms)
+    plan0 = O.make_plan((48,) * 3, boxes[0], 4, 2.0, None, 0.02, 0.8)
+
+    def plan_of(box):
+        p = O.make_plan((48,) * 3, box, 4, 2.0, None, 0.02, 0.8)
+        p.hp = O.HistParams(plan0.hp.dq, plan0.hp.qcut, plan0.hp.num_bins, p.hp.spacing)
+        return p
+
+    H, C = O.trajectory_histogram_nvt(plan0, list(zip(frames, boxes)), ids, symbols, plan_of)
+    want = O.saxs_rows(plan0, H, C)
+    got = parse(out)
+    assert len(got) == len(want) and len(got) >= 8
+    for (qg, ig), (qw, iw) in zip(got, want):
+        assert ("%10.5f" % qg) == ("%10.5f" % qw)
+        assert abs(ig - iw) <= 1e-5 * abs(iw) + 1e-5 * 10 ** np.floor(np.log10(abs(iw)))
 
 
 def test_given_pad_above_bound_rejected(gpu):
