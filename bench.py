@@ -234,7 +234,8 @@ def changing_boxes(kind, F, L=BOX):
     return out
 
 
-def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simulation=""):
+def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simulation="",
+                 defaults=False):
     """pycusaxs_cuda.run end to end on the trajectory workload: host frames
     (the 100 frames cycled to 1,000) staged through pinned double buffers and
     a copy stream, element gather on the device, batches of 16 frames.
@@ -253,9 +254,13 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simula
     def run(end):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pycusaxs_cuda.run(topo, "memory", "memory", [GRID], end=end, output=out,
-                          scale_factor=SIGMA, bin_size=DQ, qcut=QCUT, quiet=True,
-                          simulation=simulation)
+        if defaults:      # the reference CLI's defaults (saxs_defaults.py:17-31)
+            pycusaxs_cuda.run(topo, "memory", "memory", [GRID], end=end, output=out,
+                              scale_factor=2.5, bin_size=0.01, qcut=2.0, quiet=True)
+        else:
+            pycusaxs_cuda.run(topo, "memory", "memory", [GRID], end=end, output=out,
+                              scale_factor=SIGMA, bin_size=DQ, qcut=QCUT, quiet=True,
+                              simulation=simulation)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
@@ -271,6 +276,40 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simula
                         "(pinned double buffer, copy stream, 16 frames per launch)"
                         + (f", {box_kind} box changing every frame ({simulation or 'npt'})"
                            if box_kind else "")}
+
+
+def leg_reference_defaults(torch, xyz_h, symbols, ids_h, steps):
+    """The drop-in at the reference CLI's defaults (saxs_defaults.py:17-31:
+    grid 128, scale 2.5 -> 320^3 FFT, bin 0.01, qcut 2.0, order 4) on the
+    trajectory workload: the engine on HBM-resident frames (50 per launch,
+    with per-kernel times) and pycusaxs_cuda.run from host memory."""
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    F = 50
+    xyz = torch.from_numpy(xyz_h[:F]).cuda()
+    ids = torch.from_numpy(ids_h).cuda()
+    n3, nn = resolve_grid((GRID,) * 3, 2.5)
+    eng = SaxsEngine(SystemSpec(n3, nn, ORDER, tuple(symbols), 2.5, 0.01, 2.0),
+                     np.diag([BOX] * 3), max_atoms=N_ATOMS, max_frames=F)
+    eng.forward(xyz, ids)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.forward(xyz, ids)
+    torch.cuda.synchronize()
+    fps = steps * F / (time.perf_counter() - t0)
+    eng.set_timing(True)
+    eng.forward(xyz, ids)
+    torch.cuda.synchronize()
+    kt = {k: round(v[0] / v[1], 4) for k, v in eng.kernel_times().items()}
+    eng.set_timing(False)
+    out = {"engine_frames_per_s": round(fps, 1), "kept": list(eng.kept()), "fft_grid": nn[0],
+           "kernel_ms_per_50_frames": kt,
+           "workload": "reference CLI defaults: 100,000 atoms, 100 A, grid 128, scale 2.5 "
+                       "(320^3 FFT), bin 0.01 (reset to 0.02513), qcut 2.0, 80 bins"}
+    eng.close()
+    out["run_path_frames_per_s"] = leg_run_path(torch, xyz_h, symbols, ids_h, fps,
+                                                defaults=True)["frames_per_s"]
+    return out
 
 
 def kernel_bytes(eng, frames_per_launch, n_atoms):
@@ -371,7 +410,7 @@ def main():
                     help="time the forward as eager launches only (no hipGraph replay)")
     ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6",
                                       "run_path_npt_cubic", "run_path_npt_orthorhombic",
-                                      "run_path_npt_triclinic"),
+                                      "run_path_npt_triclinic", "reference_defaults"),
                     help="run only this secondary workload (for rocprofv3) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -383,6 +422,7 @@ def main():
     if args.leg:
         legs = {"batched_uneven": leg_batched_uneven, "large_1M_512": leg_large,
                 "restraint_50k_order6": lambda t, n: leg_restraint(t, n, {})}
+        legs["reference_defaults"] = lambda t, n: leg_reference_defaults(t, *make_inputs()[:3], n)
         for kind in ("cubic", "orthorhombic", "triclinic"):
             legs[f"run_path_npt_{kind}"] = (
                 lambda t, n, k=kind: leg_run_path(t, *make_inputs()[:3], float("nan"), k))
@@ -549,7 +589,9 @@ def main():
                 ("run_path_npt_triclinic", lambda t, n: leg_run_path(
                     t, xyz_h, symbols, ids_h, value, "triclinic")),
                 ("run_path_nvt_triclinic", lambda t, n: leg_run_path(
-                    t, xyz_h, symbols, ids_h, value, "triclinic", "nvt")))
+                    t, xyz_h, symbols, ids_h, value, "triclinic", "nvt")),
+                ("reference_defaults", lambda t, n: leg_reference_defaults(
+                    t, xyz_h, symbols, ids_h, n)))
         for name, fn in legs:
             try:
                 line[name] = fn(torch, max(1, min(args.steps, 3)))

@@ -197,3 +197,28 @@ def test_large_1M_512_pruned_matches_full_transforms(gpu):
     rel = np.linalg.norm(g1 - g2) / np.linalg.norm(g2)
     print(f"[1M atoms, 512^3] adjoint pruned vs full rel-L2 {rel:.3e}", flush=True)
     assert rel <= 1e-5
+
+
+def test_reference_cli_defaults_plan(gpu, threads):
+    """The plan the reference CLI's defaults give (grid 128, scale 2.5 ->
+    320^3 FFT, bin 0.01 reset to the grid spacing, qcut 2.0: kz = 80,
+    ky = kx = 159, the full-length mixed-radix y and x transforms) on a
+    sparse system, against the oracle."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    L = 100.0
+    box = np.diag([L] * 3)
+    pos, syms = O.synthetic_system(4000, L, seed=12, mix=(("C", 0.6), ("O", 0.3), ("N", 0.1)))
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((128,) * 3, 2.5)
+    assert nn == (320, 320, 320)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.5, 0.01, 2.0), box,
+                     max_atoms=len(pos), max_frames=1)
+    eng.forward(torch.from_numpy(pos[None]), torch.from_numpy(ids))
+    torch.cuda.synchronize()
+    H, C = eng.H.cpu().numpy(), eng.C.cpu().numpy()
+    assert eng.kept() == (159, 159, 80)
+    eng.close()
+    plan = O.make_plan(n3, box, 4, 2.5, None, 0.01, 2.0)
+    Ho, Co = O.frame_histogram(plan, pos, ids, symbols, box)
+    _rows_close(H, C, Ho, Co, REL_TOL, "reference CLI defaults, 128 -> 320")
