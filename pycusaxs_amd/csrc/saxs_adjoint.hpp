@@ -27,20 +27,33 @@ struct AdjBuffers {
   const double* gH;     // [F][num_bins]
   float* grad;          // [F][N][3]
   double* padg;         // [F][E] sum over the cell of drho~
-  float* gpart;         // [F][nsx*nsy][N][3] per-tile partial gradients
+  float* gpart;         // [F][nsx*nsy][N][3] per-tile partial gradients, entry
+                        // (f, slot, r): record r of frame f's sorted run
   int nsx, nsy;         // tile offsets an atom's stencil reaches along x, y
 };
+
+// A sorted record as the z adjoints stage it: w carries the record's
+// position in the sorted array (pos) instead of the atom index, so the
+// gather writes each partial gradient at (frame, slot, pos - f N): the
+// records of one source segment are consecutive, and so are the partials
+// one tile writes for them.
+__device__ __forceinline__ float4 staged_record(const Buffers& b, int pos) {
+  float4 r = b.sorted[pos];
+  r.w = __int_as_float(pos);
+  return r;
+}
 
 // Slot of the partial gradient that tile (bx, by) writes for an atom in cell
 // (mx, my): the tile's offset from the atom's home tile (the tile holding
 // stencil point o = 0), ox + nsx oy.  Every (frame, slot, atom) has one
 // writer, so the gather stores instead of adding atomically and the sum
-// over slots (k_grad_finalize) is in a fixed order.
+// over slots (k_grad_finalize) is in a fixed order.  The tile divisions are
+// exact float-reciprocal quotients (div_small: coordinates < 2^22).
 template <int P>
 __device__ __forceinline__ int partial_slot(int mx, int my, int bx, int by, const Params& p,
                                             const AdjBuffers& g) {
-  int ox = bx - wrap_base(mx, P, p.n[0]) / p.tx;
-  int oy = by - wrap_base(my, P, p.n[1]) / p.ty;
+  int ox = bx - div_small(wrap_base(mx, P, p.n[0]), p.tx, 1.0f / (float)p.tx);
+  int oy = by - div_small(wrap_base(my, P, p.n[1]), p.ty, 1.0f / (float)p.ty);
   if (ox < 0) ox += p.ntx;
   if (oy < 0) oy += p.nty;
   return ox + g.nsx * oy;
@@ -561,7 +574,7 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
       int sgm = 0;
 #pragma unroll
       for (int k = 1; k < 9; ++k) sgm += (a >= s_pref[k]) ? 1 : 0;
-      recs[i] = b.sorted[s_start[sgm] + (a - s_pref[sgm])];
+      recs[i] = staged_record(b, s_start[sgm] + (a - s_pref[sgm]));
     }
     __syncthreads();
     for (int i = threadIdx.x; i < chunk; i += blockDim.x) {
@@ -610,9 +623,9 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
         }
       }
       if (hit) {
-        const int idx = __float_as_int(rec.w);
+        const int r = __float_as_int(rec.w) - f * p.N;   // position in the frame's run
         const int slot = partial_slot<P>(mx, my, bx, by, p, g);
-        float* gp = g.gpart + (((size_t)f * g.nsx * g.nsy + slot) * p.N + idx) * 3;
+        float* gp = g.gpart + (((size_t)f * g.nsx * g.nsy + slot) * p.N + r) * 3;
         gp[0] = gx * sx;
         gp[1] = gy * sy;
         gp[2] = gzs * sz;
@@ -945,8 +958,9 @@ __global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers
 // Adjoint of spread_cols: for each staged atom, the in-tile part of
 // sum_stencil drho~ grad(th_x th_y th_z), with drho~ in columns of stride cs
 // whose z halo repeats z < P - 1 after nz.  The stencil is unrolled and
-// predicated on the in-tile ranges; the partial gradient goes to the atom's
-// slot for this tile (partial_slot).
+// predicated on the in-tile ranges; the partial gradient goes to the
+// record's slot for this tile (partial_slot), at the record's position in
+// the frame's sorted run (staged_record).
 template <int P>
 __device__ __forceinline__ void gather_cols(const float4* recs, int count, const float* drho,
                                             const Params& p, int X0, int Y0, int wx, int wy,
@@ -1010,9 +1024,9 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
       gy += thx[o] * ay;
       gzs += thx[o] * az;
     }
-    const int idx = __float_as_int(rec.w);
+    const int r = __float_as_int(rec.w) - f * p.N;     // position in the frame's run
     const int slot = partial_slot<P>(mx, my, bx, by, p, g);
-    float* gp = g.gpart + (((size_t)f * nslot + slot) * p.N + idx) * 3;
+    float* gp = g.gpart + (((size_t)f * nslot + slot) * p.N + r) * 3;
     gp[0] = gx * sx;
     gp[1] = gy * sy;
     gp[2] = gzs * sz;
@@ -1104,7 +1118,7 @@ __global__ void __launch_bounds__(kWG) k_adj_zg(Params p, Buffers b, AdjBuffers 
     int sgm = 0;
 #pragma unroll
     for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
-    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+    return staged_record(b, s_start[e * 9 + sgm] + (a - pref[sgm]));
   };
   const float inv_ty = 1.0f / (float)p.ty;
   const float inv_r2 = 1.0f / (float)R2;
@@ -1291,7 +1305,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
     int sgm = 0;
 #pragma unroll
     for (int k = 1; k < 9; ++k) sgm += (a >= pref[k]) ? 1 : 0;
-    return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
+    return staged_record(b, s_start[e * 9 + sgm] + (a - pref[sgm]));
   };
 
   for (int e = 0; e < E; ++e) {
@@ -1408,45 +1422,61 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   }
 }
 
-// Per atom: the sum of the partial gradients of the tiles its stencil
-// reaches (slots recomputed from the same cell indices as the gather), minus
-// the bulk-solvent mean term (padg_e / count) grad(border weight) when the
-// pad is the border mean, border weight = (sum th_x)(sum th_y)(sum over
-// border z of th_z).  Writes every entry of grad (zero for atoms with an
-// invalid element id).
+// Per sorted record r of frame blockIdx.y (one per atom with a valid
+// element): the sum of the partial gradients of the tiles its stencil
+// reaches (slots recomputed from the record's cell coordinates, as the
+// gather computes them; read at (f, slot, r), consecutive along r), minus
+// the bulk-solvent mean term when the pad is the border mean, written to the
+// record's atom.  The mean term is (padg_e / count) grad(border weight),
+// border weight = (sum th_x)(sum th_y)(sum over border z of th_z); the full
+// sums over x and y are 1 and their derivatives 0 (partition of unity), so
+// only its z component is nonzero: (padg_e / count) (sum over border z of
+// dth_z).  Thread i also writes zeros for atom i when its element id is
+// invalid (such atoms have no record).
 template <int P>
 __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjBuffers g) {
+  __shared__ int s_eb[kMaxElements + 1];             // element runs of the frame, from f N
   const int f = blockIdx.y;
+  const int T = p.nty * p.ntx, K = p.E * T;
+  const int f0 = f * p.N;
+  for (int e = threadIdx.x; e <= p.E; e += blockDim.x)
+    s_eb[e] = e < p.E ? b.offsets[f * K + e * T] - f0
+                      : b.offsets[f * K + K - 1] + b.counts[f * K + K - 1] - f0;
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.N) return;
-  float* gp = g.grad + ((size_t)f * p.N + i) * 3;
-  const int e = b.elem[i];
-  if (e < 0 || e >= p.E) {
-    gp[0] = 0.0f;
-    gp[1] = 0.0f;
-    gp[2] = 0.0f;
-    return;
+  {
+    const int ei = b.elem[i];
+    if (ei < 0 || ei >= p.E) {
+      float* gz = g.grad + ((size_t)f0 + i) * 3;
+      gz[0] = 0.0f;
+      gz[1] = 0.0f;
+      gz[2] = 0.0f;
+    }
   }
-  const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
-  float w[3], rr[3];
+  if (i >= s_eb[p.E]) return;
+  const float4 rec = b.sorted[(size_t)f0 + i];
+  int e = 0;
+  for (int k = 1; k < p.E; ++k) e += (i >= s_eb[k]) ? 1 : 0;
+  const float rr[3] = {rec.x, rec.y, rec.z};
+  float w[3];
   int base[3];
-  cell_coords(x, p, rr);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float r = rr[a];
-    const int m = cell_index(r, p.n[a]);
-    w[a] = r - (float)m;
+    const int m = cell_index(rr[a], p.n[a]);
+    w[a] = rr[a] - (float)m;
     base[a] = wrap_base(m, P, p.n[a]);
   }
   // tile offsets (from the home tile) that the stencil reaches along x, y
+  const float inv_tx = 1.0f / (float)p.tx, inv_ty = 1.0f / (float)p.ty;
   unsigned vx = 0u, vy = 0u;
-  const int hx = base[0] / p.tx, hy = base[1] / p.ty;
+  const int hx = div_small(base[0], p.tx, inv_tx), hy = div_small(base[1], p.ty, inv_ty);
 #pragma unroll
   for (int o = 0; o < P; ++o) {
     int cx = base[0] + o, cy = base[1] + o;
     if (cx >= p.n[0]) cx -= p.n[0];
     if (cy >= p.n[1]) cy -= p.n[1];
-    int ox = cx / p.tx - hx, oy = cy / p.ty - hy;
+    int ox = div_small(cx, p.tx, inv_tx) - hx, oy = div_small(cy, p.ty, inv_ty) - hy;
     if (ox < 0) ox += p.ntx;
     if (oy < 0) oy += p.nty;
     vx |= 1u << ox;
@@ -1465,47 +1495,33 @@ __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjB
       s2 += q[2];
     }
   }
-  float th[P], dth[P];
-  double Sx = 0, Sy = 0, Sz = 0, dSx = 0, dSy = 0, dSz = 0;
-  bspline<P>(w[0], th);
-  bspline_d<P>(w[0], dth);
+  float t2 = s2;
+  if (!b.pad_given) {                                // a given pad is a constant: no mean term
+    float dth[P];
+    bspline_d<P>(w[2], dth);
+    double dSz = 0.0;
 #pragma unroll
-  for (int o = 0; o < P; ++o) {
-    Sx += th[o];
-    dSx += dth[o];
-  }
-  bspline<P>(w[1], th);
-  bspline_d<P>(w[1], dth);
-#pragma unroll
-  for (int o = 0; o < P; ++o) {
-    Sy += th[o];
-    dSy += dth[o];
-  }
-  bspline<P>(w[2], th);
-  bspline_d<P>(w[2], dth);
-#pragma unroll
-  for (int o = 0; o < P; ++o) {
-    int z = base[2] + o;
-    if (z >= p.n[2]) z -= p.n[2];
-    if (z <= p.border || z >= p.n[2] - p.border) {
-      Sz += th[o];
-      dSz += dth[o];
+    for (int o = 0; o < P; ++o) {
+      int z = base[2] + o;
+      if (z >= p.n[2]) z -= p.n[2];
+      if (z <= p.border || z >= p.n[2] - p.border) dSz += (double)dth[o];
+    }
+    if (dSz != 0.0) {
+      const double c = g.padg[(size_t)f * p.E + e] * p.inv_count;
+      t2 = s2 - (float)(c * dSz * (double)p.n[2] * (p.tric ? 1.0 : (double)p.oc[2]));
     }
   }
-  // a given pad is a constant: no mean term
-  const double c = b.pad_given ? 0.0 : g.padg[(size_t)f * p.E + e] * p.inv_count;
+  const int idx = __float_as_int(rec.w);
+  float* gp = g.grad + ((size_t)f0 + idx) * 3;
   if (!p.tric) {
-    gp[0] = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0] * (double)p.oc[0]);
-    gp[1] = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1] * (double)p.oc[1]);
-    gp[2] = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2] * (double)p.oc[2]);
+    gp[0] = s0;
+    gp[1] = s1;
+    gp[2] = t2;
     return;
   }
   // triclinic: the sums are dL/dr_a n_a; dL/dx_j = sum_a (dL/dr_a n_a) M_aj
-  const float t0 = s0 - (float)(c * dSx * Sy * Sz * (double)p.n[0]);
-  const float t1 = s1 - (float)(c * Sx * dSy * Sz * (double)p.n[1]);
-  const float t2 = s2 - (float)(c * Sx * Sy * dSz * (double)p.n[2]);
 #pragma unroll
-  for (int j = 0; j < 3; ++j) gp[j] = t0 * p.cm[j] + t1 * p.cm[3 + j] + t2 * p.cm[6 + j];
+  for (int j = 0; j < 3; ++j) gp[j] = s0 * p.cm[j] + s1 * p.cm[3 + j] + t2 * p.cm[6 + j];
 }
 
 }  // namespace saxs
