@@ -1,7 +1,9 @@
-"""Builds the engine's shared library in-tree with hipcc for gfx950.
+"""Builds the package's shared libraries in-tree: the engine with hipcc for
+gfx950, the trajectory frame reader (host code) with the C++ compiler.
 
-``build_library()`` always recompiles; the loader (``pycusaxs_amd._lib``)
-only loads what is built and never compiles on its own.
+``build_library()`` and ``build_xdr_library()`` always recompile; the
+loaders (``pycusaxs_amd._lib``, ``pycusaxs_amd.trajectory``) only load what
+is built and never compile on their own.
 """
 from __future__ import annotations
 
@@ -16,6 +18,8 @@ SRC_DIR = PKG / "csrc"
 LIB_DIR = PKG / "_lib"
 LIB_PATH = LIB_DIR / "libsaxs_engine.so"
 SOURCES = [SRC_DIR / "saxs_engine.hip"]
+XDR_LIB_PATH = LIB_DIR / "libsaxs_xdr.so"
+XDR_SOURCES = [SRC_DIR / "xdr_frames.cpp"]
 ARCH = os.environ.get("PYCUSAXS_AMD_ARCH", "gfx950")
 
 
@@ -50,5 +54,25 @@ def build_library(verbose: bool = False) -> Path:
     return LIB_PATH
 
 
+def cxx() -> str:
+    for cand in (os.environ.get("CXX"), shutil.which("g++"), shutil.which("c++")):
+        if cand and (Path(cand).exists() or shutil.which(cand)):
+            return cand
+    raise RuntimeError("no C++ compiler found for the trajectory reader")
+
+
+def build_xdr_library() -> Path:
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    tmp = XDR_LIB_PATH.with_suffix(".so.tmp")
+    cmd = [cxx(), "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
+           f"-I{ROOT / 'include'}", "-o", str(tmp), *[str(s) for s in XDR_SOURCES]]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("C++ build of the trajectory reader failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, XDR_LIB_PATH)
+    return XDR_LIB_PATH
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_xdr_library())

@@ -258,16 +258,30 @@ def _run_frames(topo, grid3, scaled3, begin, end, stride, order, sigma, dq_req, 
             raise
         gpu_ms += (time.perf_counter() - t0) * 1e3
 
-    frames_iter = topo.iter_frames_stream(begin, end + 1, stride)
-    for fr in frames_iter:
-        frame, t, pos, box = _frame_fields(fr)
-        log(f"--> Frame: {frame:<7}  Time Step: {t:.2f} ps")
-        if stager.count >= batch:
+    stage = getattr(topo, "stage_frames", None)
+    if callable(stage) and callable(getattr(topo, "frame_indices", None)):
+        # a topology that decodes frames itself (trajectory.XdrTopology):
+        # each batch is written straight into the free pinned buffer
+        chosen = topo.frame_indices(begin, end + 1, stride)
+        for b0 in range(0, len(chosen), batch):
+            chunk = list(chosen[b0:b0 + batch])
+            xyz_host, box_host = stager.reserve(len(chunk), int(topo.n_atoms))
+            times = stage(chunk, xyz_host, box_host)
+            for f, t in zip(chunk, times):
+                log(f"--> Frame: {f:<7}  Time Step: {t:.2f} ps")
             flush()
-        stager.add(pos, box)
-        n_done += 1
-    if stager.count:
-        flush()
+            n_done += len(chunk)
+    else:
+        frames_iter = topo.iter_frames_stream(begin, end + 1, stride)
+        for fr in frames_iter:
+            frame, t, pos, box = _frame_fields(fr)
+            log(f"--> Frame: {frame:<7}  Time Step: {t:.2f} ps")
+            if stager.count >= batch:
+                flush()
+            stager.add(pos, box)
+            n_done += 1
+        if stager.count:
+            flush()
     torch.cuda.synchronize(dev)
     rows = eng.nvt_rows(box0, eng.frames_done) if sim == "nvt" else eng.rows()
     with open(out_path, "w") as fh:
@@ -321,6 +335,24 @@ class FrameStager:
         self.host_np = [h.numpy() for h in self.host]
         self.dev = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
         self.n_frame_atoms = n_frame_atoms
+
+    def reserve(self, k: int, n_frame_atoms: int):
+        """The free pinned buffer's first k frames [k, atoms, 3] and their
+        boxes [k, 3, 3], for a reader to fill in place; the batch then holds
+        k frames."""
+        if not 0 < k <= self.batch:
+            raise ValueError(f"batch of {k} frames (at most {self.batch})")
+        if self.count:
+            raise RuntimeError("reserve() on a batch that already holds frames")
+        if self.n_frame_atoms is None:
+            self._allocate(n_frame_atoms)
+        elif n_frame_atoms != self.n_frame_atoms:
+            raise ValueError(f"frames have {n_frame_atoms} atoms, earlier frames "
+                             f"{self.n_frame_atoms}")
+        if self.copied[self.slot] is not None:
+            self.copied[self.slot].synchronize()          # host buffer is free again
+        self.count = k
+        return self.host_np[self.slot][:k], self.boxes[:k]
 
     def add(self, pos: np.ndarray, box: np.ndarray):
         if self.n_frame_atoms is None:
