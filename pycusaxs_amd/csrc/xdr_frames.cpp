@@ -237,36 +237,66 @@ int xtc_header(Cursor& c, XtcHeader& h) {
   return SAXS_XDR_OK;
 }
 
-// MSB-first bit reader over the packed bytes.
+// MSB-first bit reader over the packed bytes: a 64-bit window loaded at the
+// current byte (big-endian) and shifted by the bit offset, so one call takes
+// up to 57 bits; near the end of the data the window is assembled byte by
+// byte with zeros past the end (reading past it sets `bad`).
 struct Bits {
   const unsigned char* p;
   size_t n;
-  size_t pos = 0;
-  unsigned long long acc = 0;   // the low `avail` bits are unread
-  int avail = 0;
+  size_t bit = 0;
   bool bad = false;
 
-  unsigned get(int k) {         // 0 <= k <= 32
-    if (k == 0) return 0u;
-    while (avail < k) {
-      unsigned char b = 0;
-      if (pos < n) b = p[pos];
-      else bad = true;
-      ++pos;
-      acc = (acc << 8) | b;
-      avail += 8;
+  unsigned long long window() const {
+    const size_t byte = bit >> 3;
+    unsigned long long w = 0;
+    if (byte + 8 <= n) {
+      std::memcpy(&w, p + byte, 8);
+      w = __builtin_bswap64(w);
+    } else {
+      for (size_t k = 0; k < 8; ++k) w = (w << 8) | (byte + k < n ? p[byte + k] : 0u);
     }
-    avail -= k;
-    const unsigned long long v = (acc >> avail) & ((1ull << k) - 1ull);
-    acc &= (1ull << avail) - 1ull;
-    return (unsigned)v;
+    return w << (bit & 7);
+  }
+  unsigned long long get(int k) {   // 0 <= k <= 57
+    if (k == 0) return 0ull;
+    const unsigned long long v = window() >> (64 - k);
+    bit += (size_t)k;
+    if (bit > 8 * n) bad = true;
+    return v;
   }
 };
 
 // Three integers packed as one mixed-radix number of `nbits` bits, stored
 // as bytes least significant first (the last byte holds the top bits):
-// value = (a0 sizes[1] + a1) sizes[2] + a2.
+// value = (a0 sizes[1] + a1) sizes[2] + a2.  Up to 56 bits are read in one
+// window and their bytes reordered; values below 2^32 or 2^64 take native
+// divisions, wider ones 128-bit.
 void get_triplet(Bits& br, int nbits, const unsigned sizes[3], int out[3]) {
+  if (nbits <= 56) {
+    const unsigned long long w = br.get(nbits);        // first byte in the top bits
+    unsigned long long v = 0;
+    int left = nbits, shift = 0;
+    while (left > 8) {
+      v |= ((w >> (left - 8)) & 0xffull) << shift;
+      shift += 8;
+      left -= 8;
+    }
+    v |= (w & ((1ull << left) - 1ull)) << shift;
+    if (v <= 0xffffffffull) {
+      unsigned u = (unsigned)v;
+      out[2] = (int)(u % sizes[2]);
+      u /= sizes[2];
+      out[1] = (int)(u % sizes[1]);
+      out[0] = (int)(u / sizes[1]);
+      return;
+    }
+    out[2] = (int)(unsigned)(v % sizes[2]);
+    v /= sizes[2];
+    out[1] = (int)(unsigned)(v % sizes[1]);
+    out[0] = (int)(unsigned long long)(v / sizes[1]);
+    return;
+  }
   unsigned __int128 v = 0;
   int shift = 0;
   while (nbits > 8) {
