@@ -1094,11 +1094,12 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 // Stage 0 (radix R1 = 2H): butterfly j < R2 takes x[j + R2 r]; only r < H
 // can be nonzero (z < n_z <= N/2), so its length-R1 DFT is two length-H
 // DFTs, Y_j[2t] = DFT_H(x_r)[t] and Y_j[2t+1] = DFT_H(x_r W_R1^r)[t].
-// Stage 1 forms only the kept outputs as direct sums over j,
+// Stage 1 forms only the kept outputs as sums over j,
 //   Z(k)     = sum_j Y_j[k mod R1]  W_N^{jk}
 //   Z(N - k) = sum_j Y_j[-k mod R1] conj(W_N^{jk}),   k < kz,
-// which costs 2 kz R2 products per line against the N log N of a full
-// transform whose outputs are mostly discarded (kz is about N/30 here).
+// evaluated by Horner's rule in W_N^k (2 kz R2 complex multiply-adds per
+// line against the N log N of a full transform whose outputs are mostly
+// discarded; kz is about N/30 here).
 // Y_j[s] sits at (R1 + 1) j + s of its line, and line l occupies the bytes
 // of columns 2l and 2l + 1 as in k_spread_zp.  Each wavefront owns a block
 // of lines and takes them through both stages and the clear for the next
@@ -1121,6 +1122,46 @@ __host__ __device__ inline size_t spread_g_lds_bytes(const Params& p, int R1) {
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^m
   const size_t seg = (size_t)p.E * 28 * sizeof(int);
   return A + B + tw + seg;
+}
+
+// Horner accumulation for the pruned transforms' kept outputs: with y_t the
+// entries p[s - t d] and p[sm - t d] (t = 0 .. T-1, read from the highest
+// term down), returns Z = sum_t y1_t u^(T-1-t) and Zm = sum_t y2_t
+// conj(u)^(T-1-t), i.e. Horner's rule over the terms in descending order.
+// Four terms' loads are issued before their multiply-adds: the chain is
+// bound by LDS latency otherwise.
+__device__ __forceinline__ void horner_pair(const float2* p, int d, int s, int sm, int T, float2 u,
+                                            float2& Z, float2& Zm) {
+  float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+  int t = 0;
+  for (; t + 4 <= T; t += 4) {
+    float2 y1[4], y2[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      y1[v] = p[s - (t + v) * d];
+      y2[v] = p[sm - (t + v) * d];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float ax = fmaf(A.x, u.x, fmaf(-A.y, u.y, y1[v].x));
+      const float ay = fmaf(A.x, u.y, fmaf(A.y, u.x, y1[v].y));
+      const float bx = fmaf(B.x, u.x, fmaf(B.y, u.y, y2[v].x));
+      const float by = fmaf(B.y, u.x, fmaf(-B.x, u.y, y2[v].y));
+      A = make_float2(ax, ay);
+      B = make_float2(bx, by);
+    }
+  }
+  for (; t < T; ++t) {
+    const float2 y1 = p[s - t * d], y2 = p[sm - t * d];
+    const float ax = fmaf(A.x, u.x, fmaf(-A.y, u.y, y1.x));
+    const float ay = fmaf(A.x, u.y, fmaf(A.y, u.x, y1.y));
+    const float bx = fmaf(B.x, u.x, fmaf(B.y, u.y, y2.x));
+    const float by = fmaf(B.y, u.x, fmaf(-B.x, u.y, y2.y));
+    A = make_float2(ax, ay);
+    B = make_float2(bx, by);
+  }
+  Z = A;
+  Zm = B;
 }
 
 // The plan guarantees R2 <= 64 (a stage-0 pass holds a whole line) and
@@ -1300,19 +1341,16 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
         k = q - l * kz;
         l += lw0;
         const int s = k % R1, sm = s == 0 ? 0 : R1 - s;
-        int idx = (part * k) % N;                    // j k mod N
-        const int step = (split * k) % N;
-        const float2* yl = ybuf + l * LY;
-        for (int jj = part; jj < R2; jj += split) {
-          const float2 w = tw[idx];
-          idx += step;
-          if (idx >= N) idx -= N;
-          const float2 y1 = yl[YS * jj + s], y2 = yl[YS * jj + sm];
-          Z.x += y1.x * w.x - y1.y * w.y;
-          Z.y += y1.x * w.y + y1.y * w.x;
-          Zm.x += y2.x * w.x + y2.y * w.y;
-          Zm.y += y2.y * w.x - y2.x * w.y;
-        }
+        // the part's terms j = part + split t, t < T, by Horner's rule in
+        // u = W_N^{split k} from the last term down (conj u for Z(N - k)),
+        // then the factor W_N^{part k}: sum_t y_j W_N^{jk} = W_N^{part k}
+        // sum_t y_{part + split t} u^t
+        const float2 u = tw[(split * k) % N], wp = tw[(part * k) % N];
+        const int T = part < R2 ? (R2 - 1 - part) / split + 1 : 0;
+        if (T > 0)
+          horner_pair(ybuf + l * LY + YS * (part + split * (T - 1)), YS * split, s, sm, T, u, Z, Zm);
+        Z = cmul(Z, wp);
+        Zm = cmul(Zm, make_float2(wp.x, -wp.y));
       }
       for (int m = 1; m < split; m <<= 1) {
         Z.x += __shfl_xor(Z.x, m, 64);
@@ -1485,8 +1523,8 @@ __host__ __device__ inline size_t xfft_lds_bytes(const Params& p) {
 // The z-transformed rows of zbuf are transformed along y keeping only m_y in
 // [0, Ky] and (N - Ky, N).  Stage 0 as in k_spread_zg: butterfly j < R2 of
 // line (x, kz) takes the inputs y = j + R2 r, r < H (the rest is zero
-// padding), as two length-H DFTs.  Stage 1 as direct sums over j, the pair
-// m, N - m sharing its twiddles:
+// padding), as two length-H DFTs.  Stage 1 as sums over j, the pair m,
+// N - m evaluated together by Horner's rule in W_N^m and its conjugate:
 //   Y(m) = sum_j T_j[m mod R1] W_N^{jm},  Y(N-m) = sum_j T_j[-m mod R1] conj(W_N^{jm}).
 // A workgroup takes LW = kWG / R2 lines with consecutive kz, so loads and
 // stores are coalesced along kz.  The plan guarantees 2 Ky + 1 < N.
@@ -1560,19 +1598,14 @@ __global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
     float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
     if (act) {
       const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
-      int idx = (part * m) % N;                      // j m mod N
-      const int step = (split * m) % N;
-      const float2* yl = tbuf + l * LY;
-      for (int jj = part; jj < R2; jj += split) {
-        const float2 w = tw[idx];
-        idx += step;
-        if (idx >= N) idx -= N;
-        const float2 y1 = yl[YS * jj + s], y2 = yl[YS * jj + sm];
-        A.x += y1.x * w.x - y1.y * w.y;
-        A.y += y1.x * w.y + y1.y * w.x;
-        B.x += y2.x * w.x + y2.y * w.y;
-        B.y += y2.y * w.x - y2.x * w.y;
-      }
+      // sum_j y_j W_N^{jm} over j = part + split t by Horner's rule in
+      // u = W_N^{split m} (conj u for the mirrored output), times W_N^{part m}
+      const float2 u = tw[(split * m) % N], wp = tw[(part * m) % N];
+      const int T = part < R2 ? (R2 - 1 - part) / split + 1 : 0;
+      if (T > 0)
+        horner_pair(tbuf + l * LY + YS * (part + split * (T - 1)), YS * split, s, sm, T, u, A, B);
+      A = cmul(A, wp);
+      B = cmul(B, make_float2(wp.x, -wp.y));
     }
     for (int sh = 1; sh < split; sh <<= 1) {
       A.x += __shfl_xor(A.x, sh, 64);
@@ -1812,7 +1845,8 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
 // Same result and accounting as k_xfft_epi over the same (jy, z chunk,
 // frame) workgroups: stage 0 (butterfly j < R2 of line k: inputs x = j + R2 r,
 // r < H, two length-H DFTs) per element, then stage 1 only for the kept
-// pairs m, N - m (m <= Kx) as direct sums over j, each thread accumulating
+// pairs m, N - m (m <= Kx) as sums over j by Horner's rule in W_N^m, each
+// thread accumulating
 // A = sum_e f_e(q) F_e for its pairs in registers; then |A|^2 x moduli into
 // the LDS histogram (2 I, or I at the origin).  The plan guarantees
 // Kx < N/8, kx < N and kcx (Kx + 1) <= kXgItems kWG.
@@ -1882,17 +1916,9 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
         const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
         const float2* yl = lines + k * LY;
         float2 Z = make_float2(0.0f, 0.0f), Zm = make_float2(0.0f, 0.0f);
-        int idx = 0;                                 // j m mod N
-        for (int j = 0; j < R2; ++j) {
-          const float2 w = tw[idx];
-          idx += m;
-          if (idx >= N) idx -= N;
-          const float2 y1 = yl[YS * j + s], y2 = yl[YS * j + sm];
-          Z.x += y1.x * w.x - y1.y * w.y;
-          Z.y += y1.x * w.y + y1.y * w.x;
-          Zm.x += y2.x * w.x + y2.y * w.y;
-          Zm.y += y2.y * w.x - y2.x * w.y;
-        }
+        // sum_j y_j W_N^{jm} by Horner's rule in wm = W_N^m from j = R2 - 1
+        // down (conj wm for the mirrored output)
+        horner_pair(yl + YS * (R2 - 1), YS, s, sm, R2, tw[m], Z, Zm);
         const int mz = k0 + k;
         const int n0 = shell_key(m, my, mz, p);
         if (n0 < p.lut_len) {
