@@ -278,6 +278,74 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simula
                            if box_kind else "")}
 
 
+def _trr_bytes(x_nm, box_nm, step, t):
+    """One float32 TRR frame (the layout include/saxs_xdr.h reads)."""
+    n = len(x_nm)
+    head = np.array([1993, 13, 12], dtype=">i4").tobytes() + b"GMX_trn_file"
+    sizes = np.array([0, 0, 36, 0, 0, 0, 0, 12 * n, 0, 0, n, step, 0], dtype=">i4").tobytes()
+    return (head + sizes + np.array([t, 0.0], dtype=">f4").tobytes()
+            + np.asarray(box_nm, dtype=">f4").tobytes() + np.asarray(x_nm, dtype=">f4").tobytes())
+
+
+def leg_run_file(torch, xyz_h, symbols, ids_h, engine_fps, fmt):
+    """pycusaxs_cuda.run fed by the MDAnalysis-free reader: the trajectory
+    workload written to a TRR or XTC file (the 100 host frames cycled to
+    500; XTC: four frames packed by the test suite's encoder, cycled, since
+    packing in Python is slow; decoding costs the same per frame), read
+    through trajectory.XdrTopology, whose batches are decoded on host
+    threads straight into the pinned staging buffers.  Steady-state rate
+    from runs of 100 and 500 frames, and the reader's decode rate alone."""
+    import tempfile
+    import pycusaxs_cuda
+    from pycusaxs_amd.trajectory import XdrTopology, default_threads
+    syms = [symbols[i] for i in ids_h]
+    F = 5 * len(xyz_h)
+    box_nm = np.diag([BOX / 10.0] * 3)
+    path = os.path.join(tempfile.mkdtemp(), "traj." + fmt)
+    with open(path, "wb") as fh:
+        if fmt == "trr":
+            for f in range(F):
+                fh.write(_trr_bytes(xyz_h[f % len(xyz_h)] / 10.0, box_nm, f, 0.5 * f))
+        else:
+            from tests import xdr_files as X
+            rng = np.random.default_rng(0)
+            blobs = [X.xtc_frame(X.quantize(xyz_h[f] / 10.0, 1000.0), box_nm, f, 0.5 * f, rng=rng)
+                     for f in range(4)]
+            for f in range(F):
+                fh.write(blobs[f % 4])
+    topo = XdrTopology(None, path, elements=syms)
+    buf = np.empty((len(xyz_h), topo.n_atoms, 3), dtype=np.float32)
+    box = np.empty((len(xyz_h), 3, 3), dtype=np.float32)
+    topo.stage_frames(range(len(xyz_h)), buf, box)         # page the file in
+    t0 = time.perf_counter()
+    topo.stage_frames(range(len(xyz_h)), buf, box)
+    decode = len(xyz_h) / (time.perf_counter() - t0)
+    out = os.path.join(tempfile.mkdtemp(), "saxs.dat")
+
+    def run(end):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pycusaxs_cuda.run(topo, "memory", path, [GRID], end=end, output=out,
+                          scale_factor=SIGMA, bin_size=DQ, qcut=QCUT, quiet=True)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run(F - 1)                                              # warm-up
+    short = F // 5
+    t_short = run(short - 1)
+    t_full = run(F - 1)
+    steady = (F - short) / max(t_full - t_short, 1e-9)
+    size = os.path.getsize(path)
+    topo.close()
+    os.remove(path)
+    return {"frames_per_s": round(steady, 1), "frames": F, "decode_frames_per_s": round(decode, 1),
+            "decode_threads": default_threads(), "file_bytes_per_frame": size // F,
+            "engine_only_frames_per_s": round(engine_fps, 1),
+            "workload": f"pycusaxs_cuda.run on the trajectory workload from a {fmt.upper()} file "
+                        "(trajectory.XdrTopology: batches decoded on host threads into the "
+                        "pinned staging buffers, 16 frames per launch)"}
+
+
 def leg_reference_defaults(torch, xyz_h, symbols, ids_h, steps):
     """The drop-in at the reference CLI's defaults (saxs_defaults.py:17-31:
     grid 128, scale 2.5 -> 320^3 FFT, bin 0.01, qcut 2.0, order 4) on the
@@ -410,7 +478,8 @@ def main():
                     help="time the forward as eager launches only (no hipGraph replay)")
     ap.add_argument("--leg", choices=("batched_uneven", "large_1M_512", "restraint_50k_order6",
                                       "run_path_npt_cubic", "run_path_npt_orthorhombic",
-                                      "run_path_npt_triclinic", "reference_defaults"),
+                                      "run_path_npt_triclinic", "reference_defaults",
+                                      "run_path_trr", "run_path_xtc"),
                     help="run only this secondary workload (for rocprofv3) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -426,6 +495,9 @@ def main():
         for kind in ("cubic", "orthorhombic", "triclinic"):
             legs[f"run_path_npt_{kind}"] = (
                 lambda t, n, k=kind: leg_run_path(t, *make_inputs()[:3], float("nan"), k))
+        for fmt in ("trr", "xtc"):
+            legs[f"run_path_{fmt}"] = (
+                lambda t, n, k=fmt: leg_run_file(t, *make_inputs()[:3], float("nan"), k))
         print(json.dumps({"leg": args.leg, **legs[args.leg](torch, max(1, args.steps))}), flush=True)
         return
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
@@ -590,6 +662,7 @@ def main():
                     t, xyz_h, symbols, ids_h, value, "triclinic")),
                 ("run_path_nvt_triclinic", lambda t, n: leg_run_path(
                     t, xyz_h, symbols, ids_h, value, "triclinic", "nvt")),
+                ("run_path_trr", lambda t, n: leg_run_file(t, xyz_h, symbols, ids_h, value, "trr")),
                 ("reference_defaults", lambda t, n: leg_reference_defaults(
                     t, xyz_h, symbols, ids_h, n)))
         for name, fn in legs:

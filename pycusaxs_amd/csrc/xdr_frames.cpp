@@ -2,7 +2,8 @@
 // include/saxs_xdr.h).  Host code only: the file is memory-mapped, its
 // frames are indexed once by walking the frame headers, and frames are
 // decoded independently (each from its own offset), so a batch is split
-// over host threads and written straight into the caller's buffers.
+// over host threads (kept for the handle's life) and written straight into
+// the caller's buffers.  A handle serves one read at a time.
 //
 // Formats (XDR: big-endian 4-byte words, opaque data padded to 4 bytes):
 //
@@ -40,13 +41,88 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
+
+namespace {
+
+// Worker threads kept for the life of a handle (a batch of 16 frames would
+// otherwise pay 16 thread creations per read).  run() hands out items
+// 0..count-1 to the workers and the calling thread and returns when all are
+// done; one run at a time per pool.
+class Pool {
+ public:
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+
+  void run(int nthreads, int count, const std::function<void(int)>& fn) {
+    while ((int)workers_.size() < nthreads - 1) {
+      const int index = (int)workers_.size();
+      workers_.emplace_back([this, index] { loop(index); });
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      fn_ = &fn;
+      count_ = count;
+      next_ = 0;
+      active_ = nthreads - 1;              // workers 0..active_-1 take items
+      pending_ = (int)workers_.size();
+      ++generation_;
+    }
+    wake_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (int k = next_++; k < count_; k = next_++) (*fn_)(k);
+  }
+  void loop(int index) {
+    unsigned long long seen = 0;
+    for (;;) {
+      bool take = false;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        wake_.wait(lk, [&] { return stop_ || generation_ != seen; });
+        if (stop_) return;
+        seen = generation_;
+        take = index < active_;
+      }
+      if (take) work();
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable wake_, done_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int count_ = 0;
+  std::atomic<int> next_{0};
+  int pending_ = 0;
+  int active_ = 0;
+  unsigned long long generation_ = 0;
+  bool stop_ = false;
+};
+
+}  // namespace
 
 struct saxs_xdr {
   int fd = -1;
@@ -55,6 +131,7 @@ struct saxs_xdr {
   int kind = 0;
   int natoms = 0;
   std::vector<size_t> offsets;   // byte offset of each frame
+  Pool pool;                     // decoding threads, started on the first threaded read
 };
 
 namespace {
@@ -555,25 +632,21 @@ int saxs_xdr_read(saxs_xdr* h, const int64_t* frames, int count, float* xyz, flo
     }
     return SAXS_XDR_OK;
   }
-  std::atomic<int> next{0};
   std::atomic<int> first_rc{0};
   std::mutex mu;
   std::string msg;
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&]() {
-      for (int k = next++; k < count && first_rc.load() == 0; k = next++) {
-        const int rc = one(k);
-        if (rc) {
-          std::lock_guard<std::mutex> lock(mu);
-          if (first_rc.load() == 0) {
-            msg = g_error;
-            first_rc = rc;
-          }
-        }
+  const std::function<void(int)> item = [&](int k) {
+    if (first_rc.load() != 0) return;
+    const int rc = one(k);
+    if (rc) {
+      std::lock_guard<std::mutex> lock(mu);
+      if (first_rc.load() == 0) {
+        msg = g_error;
+        first_rc = rc;
       }
-    });
-  for (auto& th : pool) th.join();
+    }
+  };
+  h->pool.run(nt, count, item);
   if (first_rc.load()) return fail(first_rc.load(), "%s", msg.c_str());
   return SAXS_XDR_OK;
 }
