@@ -51,8 +51,10 @@ int saxs_xdr_info(const saxs_xdr* h, int* kind, int* n_atoms, int64_t* n_frames)
  *   box  [count][9] float, Angstrom (NULL: not wanted)
  *   time [count] double, ps (NULL: not wanted)
  *   step [count] int64 (NULL: not wanted)
- * on up to `threads` host threads (<= 0: one).  A TRR frame without
- * positions is an error when xyz is requested. */
+ * on up to `threads` host threads (<= 0: one; the threads are kept for the
+ * handle's life).  Calls on one handle from several threads run one after
+ * another.  A TRR frame without positions is an error when xyz is
+ * requested. */
 int saxs_xdr_read(saxs_xdr* h, const int64_t* frames, int count, float* xyz, float* box,
                   double* time, int64_t* step, int threads);
 

@@ -3,7 +3,7 @@
 // frames are indexed once by walking the frame headers, and frames are
 // decoded independently (each from its own offset), so a batch is split
 // over host threads (kept for the handle's life) and written straight into
-// the caller's buffers.  A handle serves one read at a time.
+// the caller's buffers.  Reads on one handle are serialized.
 //
 // Formats (XDR: big-endian 4-byte words, opaque data padded to 4 bytes):
 //
@@ -131,6 +131,7 @@ struct saxs_xdr {
   int kind = 0;
   int natoms = 0;
   std::vector<size_t> offsets;   // byte offset of each frame
+  std::mutex read_mu;            // reads on one handle run one at a time
   Pool pool;                     // decoding threads, started on the first threaded read
 };
 
@@ -619,6 +620,7 @@ int saxs_xdr_read(saxs_xdr* h, const int64_t* frames, int count, float* xyz, flo
     if (frames[k] < 0 || frames[k] >= nf)
       return fail(SAXS_XDR_EINVAL, "frame %lld out of range [0, %lld)", (long long)frames[k],
                   (long long)nf);
+  std::lock_guard<std::mutex> serial(h->read_mu);
   const size_t n3 = 3 * (size_t)h->natoms;
   auto one = [&](int k) {
     return decode_one(h, frames[k], xyz ? xyz + k * n3 : nullptr, box ? box + 9 * (size_t)k : nullptr,

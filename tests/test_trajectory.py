@@ -147,6 +147,36 @@ def test_threaded_batch_equals_serial_and_any_order(tmp_path):
         assert np.array_equal(a[k], X.expected_angstrom(ints[f], 1000.0))
 
 
+def test_concurrent_reads_on_one_handle(tmp_path):
+    """Reads from several Python threads on one handle (ctypes releases the
+    GIL) are serialized and each gets its own frames."""
+    import threading
+    rng = np.random.default_rng(12)
+    F = 8
+    ints = [X.quantize(X.water_like(100, 2.5, rng), 1000.0) for _ in range(F)]
+    tr = T.XdrTrajectory(_xtc(tmp_path, ints, [np.diag([2.5] * 3)] * F))
+    out, errs = {}, []
+
+    def work(first):
+        try:
+            for rep in range(5):
+                fr = [(first + rep + k) % F for k in range(3)]
+                xyz, _, _, _ = tr.read(fr, threads=4)
+                for k, f in enumerate(fr):
+                    if not np.array_equal(xyz[k], X.expected_angstrom(ints[f], 1000.0)):
+                        errs.append((first, rep, f))
+            out[first] = True
+        except Exception as exc:          # reported below
+            errs.append(exc)
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errs and len(out) == 4
+
+
 def test_read_into_given_buffers(tmp_path):
     rng = np.random.default_rng(8)
     ints = [X.quantize(X.water_like(40, 2.0, rng), 1000.0) for _ in range(3)]
@@ -287,3 +317,38 @@ def test_pdb_structure(tmp_path):
     assert s.elements == ["N", "C", "Na", "O"]
     assert np.array_equal(s.box, np.diag([40.0, 41.0, 42.0]).astype(np.float32))
     assert np.allclose(s.xyz[1], [11.639, 6.071, -5.147])
+
+
+def test_corrupted_files_under_address_sanitizer(tmp_path):
+    """1,500 corrupted copies each of an XTC and a TRR file (bit flips,
+    random bytes, truncations) through open/index/decode in a driver built
+    with AddressSanitizer and UndefinedBehaviorSanitizer (host code): no
+    memory error, out-of-bounds read or undefined behaviour."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no g++")
+    root = os.path.join(os.path.dirname(__file__), "..")
+    exe = str(tmp_path / "xdr_fuzz")
+    build = subprocess.run([cxx, "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                            "-fno-sanitize-recover=all", "-pthread",
+                            "-I" + os.path.join(root, "include"), "-o", exe,
+                            os.path.join(root, "tests", "xdr_fuzz.cpp"),
+                            os.path.join(root, "pycusaxs_amd", "csrc", "xdr_frames.cpp")],
+                           capture_output=True, text=True)
+    if build.returncode != 0 and "sanitize" in build.stderr:
+        pytest.skip("sanitizer runtime unavailable")
+    assert build.returncode == 0, build.stderr
+    rng = np.random.default_rng(13)
+    q = [X.quantize(X.water_like(120, 3.0, rng, 0.05), 1000.0) for _ in range(3)]
+    xtc = _write(tmp_path, "seed.xtc", [X.xtc_frame(q[f], np.diag([3.0] * 3), f, 0.1 * f, rng=rng,
+                                                   start_idx=15) for f in range(3)])
+    x = rng.uniform(0, 3, (40, 3))
+    trr = _write(tmp_path, "seed.trr", [X.trr_frame(x, np.diag([3.0] * 3), f, 0.1 * f,
+                                                   double=(f == 1)) for f in range(3)])
+    for seed in (xtc, trr):
+        run = subprocess.run([exe, seed, "1500", str(tmp_path / "case.bin")],
+                             capture_output=True, text=True, timeout=300)
+        assert run.returncode == 0, run.stderr[-2000:]
+        assert "1500 cases" in run.stdout
