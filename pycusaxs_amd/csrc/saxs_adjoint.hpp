@@ -387,12 +387,18 @@ __global__ void __launch_bounds__(kWG) k_adj_y(Params p, Buffers b) {
 // W_16^{s r} W_R2^{r' j}: stage A item (line, s) loads the kept r' and forms
 // the radix-R2 DFT over r' times W_N^{s j}; one LDS exchange (T_s[j] at
 // 17 j + s); stage B item (line, j) forms the radix-16 DFT over s and writes
-// the y < n_y outputs.  Lines and workgroups as in k_yfft_p.
+// the y < n_y outputs.  Lines as in k_yfft_p; a workgroup takes
+// kAdjYGroups consecutive line groups and issues the next group's loads
+// before the current group's stage B, so they are in flight while it
+// computes and stores (a thread has only its few kept-row loads otherwise).
+constexpr int kAdjYGroups = 2;
+
 template <int N, bool ENDS, bool HALF>
 __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
   constexpr int R2 = N / 16;
   constexpr int LY = 17 * R2 + 1;
   constexpr int LW = yfft_p_lines<N>();
+  constexpr int IPT = (LW * 16 + kWG - 1) / kWG;      // stage-A items per thread
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tbuf = reinterpret_cast<float2*>(smem);
   float2* tw = tbuf + (size_t)LW * LY;                 // W_N^{j s} stored [j][s]
@@ -402,58 +408,77 @@ __global__ void __launch_bounds__(kWG) k_adj_y_p(Params p, Buffers b) {
   const size_t fe = grp / gridDim.x;
   const int nx = p.n[0], ny = p.n[1], kz = p.kz;
   const int nlines = nx * kz;
-  const int g0 = (int)(grp % gridDim.x) * LW;
+  const int gbase = (int)(grp % gridDim.x) * kAdjYGroups * LW;
   const int Ky = p.Ky, ky = p.ky;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[(i >> 4) * (i & 15)];
-  __syncthreads();
-  for (int item = threadIdx.x; item < LW * 16; item += blockDim.x) {
-    const int l = item % LW, s = item / LW;
-    const int g = g0 + l;
-    float2 t[R2];
+  float2 t[IPT][R2];
+  // stage-A inputs of the group starting at line g0
+  auto load = [&](int g0) {
 #pragma unroll
-    for (int rr = 0; rr < R2; ++rr) t[rr] = make_float2(0.0f, 0.0f);
-    if (g < nlines) {
-      const int x = g / kz, k = g - x * kz;
-      const float2* in = b.ybuf + (fe * ky * nx + x) * kz + k;   // + jy nx kz
+    for (int u = 0; u < IPT; ++u) {
+      const int item = threadIdx.x + u * kWG;
+      const int l = item % LW, s = item / LW;
+      const int g = g0 + l;
 #pragma unroll
-      for (int rr = 0; rr < R2; ++rr) {
-        // ENDS: Ky < N / 8, kept m only at r' < KL or r' >= R2 - KL
-        if (ENDS && rr >= N / 128 && rr < R2 - N / 128) continue;
-        const int m = s + 16 * rr;
-        if (m <= Ky || m >= N - Ky) {
-          const int jy = m <= Ky ? m : ky - (N - m);
-          const float2 v = in[(size_t)jy * nx * kz];
-          t[rr] = make_float2(v.x, -v.y);
+      for (int rr = 0; rr < R2; ++rr) t[u][rr] = make_float2(0.0f, 0.0f);
+      if (item < LW * 16 && g < nlines) {
+        const int x = g / kz, k = g - x * kz;
+        const float2* in = b.ybuf + (fe * ky * nx + x) * kz + k;   // + jy nx kz
+#pragma unroll
+        for (int rr = 0; rr < R2; ++rr) {
+          // ENDS: Ky < N / 8, kept m only at r' < KL or r' >= R2 - KL
+          if (ENDS && rr >= N / 128 && rr < R2 - N / 128) continue;
+          const int m = s + 16 * rr;
+          if (m <= Ky || m >= N - Ky) {
+            const int jy = m <= Ky ? m : ky - (N - m);
+            const float2 v = in[(size_t)jy * nx * kz];
+            t[u][rr] = make_float2(v.x, -v.y);
+          }
         }
       }
     }
-    if constexpr (R2 == 8) Radix<8>::dft(t);
-    else if constexpr (ENDS && R2 == 16) dft16_in_ends(t);    // rows rr < 2 or >= 14 only
-    else if constexpr (ENDS && R2 == 32) dft32_in_ends(t);    // rows rr < 4 or >= 28 only
-    else if constexpr (R2 == 16) dft16<false>(t);
-    else dft32(t);
-    float2* dst = tbuf + l * LY + s;
-#pragma unroll
-    for (int j = 0; j < R2; ++j) dst[17 * j] = (j == 0) ? t[0] : cmul(t[j], tw[j * 16 + s]);
-  }
+  };
+  load(gbase);
   __syncthreads();
-  {
-    const int l = threadIdx.x % LW, j = threadIdx.x / LW;     // j < R2
-    const int g = g0 + l;
-    float2 v[16];
-    const float2* srcl = tbuf + l * LY + 17 * j;
+  for (int gi = 0; gi < kAdjYGroups; ++gi) {
+    const int g0 = gbase + gi * LW;
+    if (g0 >= nlines) break;                            // the same for the whole workgroup
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = srcl[q];
-    dft16<false>(v);
-    if (g < nlines) {
-      const int x = g / kz, k = g - x * kz;
-      float2* dst = b.zbuf + (fe * nx + x) * (size_t)ny * kz + k;
+    for (int u = 0; u < IPT; ++u) {
+      const int item = threadIdx.x + u * kWG;
+      if (item < LW * 16) {
+        const int l = item % LW, s = item / LW;
+        if constexpr (R2 == 8) Radix<8>::dft(t[u]);
+        else if constexpr (ENDS && R2 == 16) dft16_in_ends(t[u]);    // rows rr < 2 or >= 14 only
+        else if constexpr (ENDS && R2 == 32) dft32_in_ends(t[u]);    // rows rr < 4 or >= 28 only
+        else if constexpr (R2 == 16) dft16<false>(t[u]);
+        else dft32(t[u]);
+        float2* dst = tbuf + l * LY + s;
 #pragma unroll
-      for (int r = 0; r < (HALF ? 8 : 16); ++r) {      // HALF: 2 n_y <= N
-        const int y = j + R2 * r;
-        if (y < ny) dst[(size_t)y * kz] = make_float2(v[r].x, -v[r].y);
+        for (int j = 0; j < R2; ++j) dst[17 * j] = (j == 0) ? t[u][0] : cmul(t[u][j], tw[j * 16 + s]);
       }
     }
+    __syncthreads();
+    if (gi + 1 < kAdjYGroups) load(g0 + LW);           // in flight during stage B
+    {
+      const int l = threadIdx.x % LW, j = threadIdx.x / LW;     // j < R2
+      const int g = g0 + l;
+      float2 v[16];
+      const float2* srcl = tbuf + l * LY + 17 * j;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = srcl[q];
+      dft16<false>(v);
+      if (g < nlines) {
+        const int x = g / kz, k = g - x * kz;
+        float2* dst = b.zbuf + (fe * nx + x) * (size_t)ny * kz + k;
+#pragma unroll
+        for (int r = 0; r < (HALF ? 8 : 16); ++r) {      // HALF: 2 n_y <= N
+          const int y = j + R2 * r;
+          if (y < ny) dst[(size_t)y * kz] = make_float2(v[r].x, -v[r].y);
+        }
+      }
+    }
+    __syncthreads();                                    // tbuf is rewritten next group
   }
 }
 

@@ -1390,7 +1390,8 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     const unsigned nfe = (unsigned)(n_frames * p.E);
 #define SAXS_AYP(NN)                                                                         \
   do {                                                                                       \
-    const dim3 gay((p.n[0] * p.kz + saxs::yfft_p_lines<NN>() - 1) / saxs::yfft_p_lines<NN>(), nfe); \
+    const int lw_ = saxs::yfft_p_lines<NN>() * saxs::kAdjYGroups;                           \
+    const dim3 gay((p.n[0] * p.kz + lw_ - 1) / lw_, nfe);                                    \
     const bool ends = p.Ky < NN / 8 && p.ky < NN, yhalf = 2 * p.n[1] <= NN;                  \
     const size_t lq = saxs::yfft_p_lds_bytes<NN>();                                          \
     if (ends && yhalf)                                                                       \
