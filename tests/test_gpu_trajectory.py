@@ -2,8 +2,9 @@
 GRO + XTC/TRR pair written by tests/xdr_files.py): its staged path (frames
 decoded on host threads straight into the pinned staging buffers) writes the
 same saxs.dat, byte for byte, as the per-frame iterator path over the same
-positions and boxes, for NPT boxes that change every frame and batches that
-cross the run's batch size; and against the FP64 oracle for a fixed box."""
+positions and boxes, for cubic and triclinic NPT boxes that change every
+frame, float and double TRR, and batches that cross the run's batch size;
+and against the FP64 oracle for a fixed box."""
 import numpy as np
 import pytest
 
@@ -16,33 +17,41 @@ from tests.fake_topology import FakeTopology
 pytestmark = pytest.mark.gpu
 
 
-def _system(tmp_path, F, fmt, grow, seed=31, nmol=700):
+TRIC = np.array([[3.6, 0.0, 0.0], [0.4, 3.5, 0.0], [-0.3, 0.2, 3.7]])
+
+
+def _system(tmp_path, F, fmt, grow, seed=31, nmol=700, tric=False, double=False):
     rng = np.random.default_rng(seed)
     L = 3.6
     x0 = X.water_like(nmol, L, rng, 0.08)
     names = ["OW", "HW1", "HW2"] * nmol + ["NA", "CL", "CA", "N", "C", "O"]
     res = ["SOL"] * (3 * nmol) + ["NA", "CL", "ALA", "ALA", "ALA", "ALA"]
     x0 = np.vstack([x0, rng.uniform(0, L, size=(6, 3))])
+    base = TRIC if tric else np.diag([L] * 3)
     gro = tmp_path / "sys.gro"
-    gro.write_text(X.gro_text(names, res, x0, np.diag([L] * 3)))
-    boxes = [np.diag([L * (1 + grow * f)] * 3) for f in range(F)]
-    xs = [(x0 * (1 + grow * f) + rng.normal(0, 0.01, x0.shape)) % (L * (1 + grow * f))
-          for f in range(F)]
+    gro.write_text(X.gro_text(names, res, x0, base))
+    boxes = [base * (1 + grow * f) for f in range(F)]
+    # fractional coordinates in each frame's cell (wrapped), then Cartesian
+    frac = (x0 / L) % 1.0
+    xs = [((frac + rng.normal(0, 0.003, frac.shape)) % 1.0) @ b for b in boxes]
     if fmt == "xtc":
         blobs = [X.xtc_frame(X.quantize(x, 1000.0), b, 5 * f, 0.1 * f, rng=rng)
                  for f, (x, b) in enumerate(zip(xs, boxes))]
     else:
-        blobs = [X.trr_frame(x, b, 5 * f, 0.1 * f) for f, (x, b) in enumerate(zip(xs, boxes))]
+        blobs = [X.trr_frame(x, b, 5 * f, 0.1 * f, double=double)
+                 for f, (x, b) in enumerate(zip(xs, boxes))]
     traj = tmp_path / f"traj.{fmt}"
     traj.write_bytes(b"".join(blobs))
     return str(gro), str(traj)
 
 
-@pytest.mark.parametrize("fmt,grow,stride", [("xtc", 0.002, 1), ("trr", 0.0, 2),
-                                             ("xtc", 0.0, 3)])
-def test_run_staged_reader_matches_iterator_path(gpu, tmp_path, fmt, grow, stride):
+@pytest.mark.parametrize("fmt,grow,stride,tric,double",
+                         [("xtc", 0.002, 1, False, False), ("trr", 0.0, 2, False, False),
+                          ("xtc", 0.0, 3, False, False), ("xtc", 0.002, 1, True, False),
+                          ("trr", 0.003, 1, True, True)])
+def test_run_staged_reader_matches_iterator_path(gpu, tmp_path, fmt, grow, stride, tric, double):
     F = 37                                             # crosses the batch size (16)
-    gro, traj = _system(tmp_path, F, fmt, grow)
+    gro, traj = _system(tmp_path, F, fmt, grow, tric=tric, double=double)
     topo = T.XdrTopology(gro, traj)
     xyz, box, time, _ = topo.traj.read(range(F))
     elements = topo.elements
@@ -55,7 +64,7 @@ def test_run_staged_reader_matches_iterator_path(gpu, tmp_path, fmt, grow, strid
     pycusaxs_cuda.run(topo, gro, traj, output=str(out_a), **kw)
     pycusaxs_cuda.run(fake, gro, traj, output=str(out_b), **kw)
     a, b = out_a.read_text(), out_b.read_text()
-    print(f"[{fmt} grow {grow} stride {stride}] {len(a.splitlines())} rows", flush=True)
+    print(f"[{fmt} grow {grow} stride {stride} tric {tric}] {len(a.splitlines())} rows", flush=True)
     assert len(a.splitlines()) >= 8
     assert a == b
 
