@@ -93,11 +93,17 @@ typedef struct saxs_plan saxs_plan;
  *               such a plan return SAXS_EUNSUP when the z adjoint does not
  *               fit that tile (no counterpart in the reference, whose plans
  *               have no backward; for forward-only batches).
+ *               SAXS_FLAG_NO_KEY_CLASSES: the counting sort keys atoms by
+ *               home tile only, and each tile fetches every atom of the
+ *               tiles whose stencils can reach it; by default atoms whose
+ *               stencil leaves the home tile are keyed apart, so a tile
+ *               fetches only atoms that reach it (identical results).
  */
 #define SAXS_FLAG_PER_ELEMENT_SPREAD 1
 #define SAXS_FLAG_NO_PRUNED_Z 2
 #define SAXS_FLAG_WIDE_RADIX 4
 #define SAXS_FLAG_FORWARD_ONLY 8
+#define SAXS_FLAG_NO_KEY_CLASSES 16
 typedef struct saxs_desc {
   int32_t n[3];
   int32_t nn[3];

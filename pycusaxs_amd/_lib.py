@@ -67,6 +67,7 @@ FLAG_PER_ELEMENT_SPREAD = 1
 FLAG_NO_PRUNED_Z = 2
 FLAG_WIDE_RADIX = 4
 FLAG_FORWARD_ONLY = 8     # tile fitted to the forward kernels alone (no backward)
+FLAG_NO_KEY_CLASSES = 16  # sort keys by home tile only (tiles fetch every source atom)
 BWD_REUSE_FORWARD = 1
 
 

@@ -513,18 +513,9 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
   __shared__ int s_start[9], s_cnt[9], s_pref[10];
   __shared__ double s_red[kWG / 64];
   if (threadIdx.x < 9) {
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[threadIdx.x] = st;
-    s_cnt[threadIdx.x] = cnt;
+    const int2 r = tile_slot(p, b, f, e, bx, by, threadIdx.x);
+    s_start[threadIdx.x] = r.x;
+    s_cnt[threadIdx.x] = r.y;
   }
   for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
   for (int i = threadIdx.x; i < nl * nnz; i += blockDim.x) {
@@ -1113,19 +1104,10 @@ __global__ void __launch_bounds__(kWG) k_adj_zg(Params p, Buffers b, AdjBuffers 
   __shared__ double s_red[kWG / 64];
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
-    const int e = t / 9, sg = t - e * 9;
-    const int4 sxv = b.srcx[bx], syv = b.srcy[by];
-    const int ix = sg % 3, iy = sg / 3;
-    int st = 0, cnt = 0;
-    if (ix < sxv.x && iy < syv.x) {
-      const int kx = ix == 0 ? sxv.y : (ix == 1 ? sxv.z : sxv.w);
-      const int ky = iy == 0 ? syv.y : (iy == 1 ? syv.z : syv.w);
-      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[t] = st;
-    s_cnt[t] = cnt;
+    const int e = t / 9;
+    const int2 r = tile_slot(p, b, f, e, bx, by, t - e * 9);
+    s_start[t] = r.x;
+    s_cnt[t] = r.y;
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[i];
   __syncthreads();
@@ -1299,19 +1281,10 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   __shared__ double s_red[kWG / 64];
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
-    const int e = t / 9, sg = t - e * 9;
-    const int4 sxv = b.srcx[bx], syv = b.srcy[by];
-    const int ix = sg % 3, iy = sg / 3;
-    int st = 0, cnt = 0;
-    if (ix < sxv.x && iy < syv.x) {
-      const int kx = ix == 0 ? sxv.y : (ix == 1 ? sxv.z : sxv.w);
-      const int ky = iy == 0 ? syv.y : (iy == 1 ? syv.z : syv.w);
-      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[t] = st;
-    s_cnt[t] = cnt;
+    const int e = t / 9;
+    const int2 r = tile_slot(p, b, f, e, bx, by, t - e * 9);
+    s_start[t] = r.x;
+    s_cnt[t] = r.y;
   }
   // stage-1 twiddles W_N^{j r} stored [r][j]
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[(i >> 4) * (i & 15)];
@@ -1462,7 +1435,7 @@ template <int P>
 __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjBuffers g) {
   __shared__ int s_eb[kMaxElements + 1];             // element runs of the frame, from f N
   const int f = blockIdx.y;
-  const int T = p.nty * p.ntx, K = p.E * T;
+  const int K = frame_keys(p), T = K / p.E;   // keys per frame, per element
   const int f0 = f * p.N;
   for (int e = threadIdx.x; e <= p.E; e += blockDim.x)
     s_eb[e] = e < p.E ? b.offsets[f * K + e * T] - f0

@@ -121,30 +121,67 @@ std::vector<float2> twiddles(int N) {
   return t;
 }
 
-// For each column tile of one axis: the source bins (by wrapped stencil
-// base) whose B-spline stencils reach the tile.
-bool tile_sources(int n, int t, int P, std::vector<int4>& out) {
+// For each column tile of one axis of n cells in tiles of t: the key units
+// (home tile S of the stencil's first point, times C, plus its class c; see
+// saxs::frame_keys) whose atoms reach the tile, found by enumerating every
+// stencil base.  x axis: the units as at most two runs of consecutive units
+// {runs, first unit of run 0, of run 1, len0 | len1 << 16}; y axis: at most
+// three units {count, u0, u1, u2}.  False when a tile needs more.
+bool tile_units(int n, int t, int P, int C, bool x_axis, std::vector<int4>& out) {
   const int nt = (n + t - 1) / t;
+  std::vector<std::vector<int>> units(nt);
+  for (int base = 0; base < n; ++base) {
+    const int S = base / t, X0 = S * t, w = std::min(t, n - X0);
+    const int u = C == 1 ? S : 2 * S + (base - X0 + P - 1 >= w ? 1 : 0);
+    for (int o = 0; o < P; ++o) {
+      auto& v = units[((base + o) % n) / t];
+      if (std::find(v.begin(), v.end(), u) == v.end()) v.push_back(u);
+    }
+  }
   out.assign(nt, make_int4(0, 0, 0, 0));
   for (int T = 0; T < nt; ++T) {
-    std::vector<int> bins;
-    for (int base = 0; base < n; ++base) {
-      bool hit = false;
-      for (int o = 0; o < P && !hit; ++o) {
-        const int g = (base + o) % n;
-        hit = g >= T * t && g < std::min(n, (T + 1) * t);
-      }
-      const int bin = base / t;
-      if (hit && std::find(bins.begin(), bins.end(), bin) == bins.end()) bins.push_back(bin);
+    std::vector<int>& v = units[T];
+    std::sort(v.begin(), v.end());
+    if (!x_axis) {
+      if (v.size() > 3) return false;
+      int4 r = make_int4((int)v.size(), 0, 0, 0);
+      if (v.size() > 0) r.y = v[0];
+      if (v.size() > 1) r.z = v[1];
+      if (v.size() > 2) r.w = v[2];
+      out[T] = r;
+      continue;
     }
-    if (bins.size() > 3) return false;
-    int4 v = make_int4((int)bins.size(), 0, 0, 0);
-    if (bins.size() > 0) v.y = bins[0];
-    if (bins.size() > 1) v.z = bins[1];
-    if (bins.size() > 2) v.w = bins[2];
-    out[T] = v;
+    std::vector<std::pair<int, int>> runs;            // (first unit, length)
+    for (int u : v) {
+      if (!runs.empty() && runs.back().first + runs.back().second == u)
+        ++runs.back().second;
+      else
+        runs.push_back({u, 1});
+    }
+    if (runs.empty() || runs.size() > 2) return false;
+    int4 r = make_int4((int)runs.size(), runs[0].first, 0, runs[0].second);
+    if (runs.size() > 1) {
+      r.z = runs[1].first;
+      r.w |= runs[1].second << 16;
+    }
+    out[T] = r;
   }
   return true;
+}
+
+// Passes of the segmented counting sort over a frame's keys: 1 with 32-bit
+// LDS counters, else the number of 16-bit passes over key ranges (at most
+// 4), else a large number (the global-atomic sort).
+constexpr size_t kSortLdsBudget = 48 * 1024;
+int seg_kr16(int E) {
+  int kr = 2 * (int)((kSortLdsBudget - 16 - E * sizeof(unsigned long long)) / sizeof(unsigned));
+  while (kr > 2 && saxs::prep_seg_lds_bytes(kr, E, true) > kSortLdsBudget) kr -= 2;
+  return kr;
+}
+int sort_passes(int kframe, int E) {
+  if (saxs::prep_seg_lds_bytes(kframe, E) <= kSortLdsBudget) return 1;
+  const int passes = (kframe + seg_kr16(E) - 1) / seg_kr16(E);
+  return passes <= 4 ? passes : 1000;
 }
 
 // Largest kept |m| per axis: from the point-key box, or K with K^2 <= lut_len - 1
@@ -573,7 +610,7 @@ int static_scratch(void** out) {
 size_t carve_work(const saxs_plan* pl, int n_atoms, int n_frames, unsigned char* base, Work& w) {
   const saxs::Params& p = pl->p;
   const size_t F = n_frames, N = n_atoms, E = p.E;
-  const size_t kframe = E * p.nty * p.ntx, nkeys = F * kframe;
+  const size_t kframe = saxs::frame_keys(p), nkeys = F * kframe;
   size_t off = 0;
   auto take = [&](size_t bytes) -> void* {
     void* q = base ? (void*)(base + off) : nullptr;
@@ -662,7 +699,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   const Work W = work ? *work : plan_work(pl);
   saxs::Buffers b = make_buffers(pl, xyz, elem, &W);
-  const int kframe = p.E * p.nty * p.ntx;               // keys per frame
+  const int kframe = saxs::frame_keys(p);               // keys per frame
   const int nkeys = n_frames * kframe;
   const int nclear = std::max(nkeys + 1, n_frames * p.E);
   // 16-bit segment counters need segments of at most kSeg16MaxLen atoms
@@ -972,10 +1009,31 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   }
   p.ntx = (p.n[0] + p.tx - 1) / p.tx;
   p.nty = (p.n[1] + p.ty - 1) / p.ty;
+  // Sort-key classes per axis: the most that keep every tile's fetch within
+  // its nine record ranges and cost the counting sort no extra pass.
   std::vector<int4> sx, sy;
-  if (!tile_sources(p.n[0], p.tx, p.order, sx) || !tile_sources(p.n[1], p.ty, p.order, sy)) {
-    free_plan(pl);
-    return fail(SAXS_EUNSUP, "spreading tile %dx%d too narrow for order %d", p.tx, p.ty, p.order);
+  {
+    const int base_passes = sort_passes(p.E * p.nty * p.ntx, p.E);
+    const bool classes = (d->flags & SAXS_FLAG_NO_KEY_CLASSES) == 0;
+    const int cand[4][2] = {{2, 2}, {1, 2}, {2, 1}, {1, 1}};
+    bool found = false;
+    for (const auto& c : cand) {
+      if (!classes && (c[0] != 1 || c[1] != 1)) continue;
+      if ((c[0] != 1 || c[1] != 1) &&
+          sort_passes(p.E * p.nty * c[1] * p.ntx * c[0], p.E) > base_passes)
+        continue;
+      if (!tile_units(p.n[0], p.tx, p.order, c[0], true, sx) ||
+          !tile_units(p.n[1], p.ty, p.order, c[1], false, sy))
+        continue;
+      p.ncx = c[0];
+      p.ncy = c[1];
+      found = true;
+      break;
+    }
+    if (!found) {
+      free_plan(pl);
+      return fail(SAXS_EUNSUP, "spreading tile %dx%d too narrow for order %d", p.tx, p.ty, p.order);
+    }
   }
   pl->nsx = slot_count(p.n[0], p.tx, p.order);
   pl->nsy = slot_count(p.n[1], p.ty, p.order);
@@ -999,7 +1057,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   }
   pl->max_frames = d->max_frames;
   pl->max_atoms = d->max_atoms;
-  pl->nkeys_max = d->max_frames * p.E * p.nty * p.ntx;
+  pl->nkeys_max = d->max_frames * saxs::frame_keys(p);
 
   const size_t F = d->max_frames, N = d->max_atoms, E = p.E;
   const size_t n3 = (size_t)p.n[0] * p.n[1] * p.n[2];
@@ -1026,17 +1084,15 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->offsets, (size_t)pl->nkeys_max + 1);
   ALLOC(pl->keyslot, F * N);
   {
-    const int kframe = E * p.nty * p.ntx;
+    const int kframe = saxs::frame_keys(p);
     // LDS histogram of a frame's keys: 32-bit counters in one pass, else
     // 16-bit counters in passes over key ranges (up to 4), else the global-
     // atomic sort (k_prep); k_prep_seg<P, false> always runs a single pass
-    const size_t budget = 48 * 1024;
-    if (saxs::prep_seg_lds_bytes(kframe, E) <= budget) {
+    if (saxs::prep_seg_lds_bytes(kframe, E) <= kSortLdsBudget) {
       pl->seg_sort = true;
       pl->seg_kr = kframe;
     } else {
-      int kr = 2 * (int)((budget - 16 - E * sizeof(unsigned long long)) / sizeof(unsigned));
-      while (kr > 2 && saxs::prep_seg_lds_bytes(kr, E, true) > budget) kr -= 2;
+      const int kr = seg_kr16((int)E);
       if ((kframe + kr - 1) / kr <= 4) {
         pl->seg_sort = pl->seg16 = true;
         pl->seg_kr = std::min(kr, kframe);

@@ -36,6 +36,7 @@ struct Params {
   int n[3], nn[3];
   int order, E, border;
   int tx, ty, ntx, nty;
+  int ncx = 1, ncy = 1;  // sort-key classes per axis (1, or 2: stencil leaves its home tile)
   int kz, ky, Ky, kx, Kx;
   int nf[3];
   int num_bins, lut_len;
@@ -54,14 +55,14 @@ struct Params {
 struct Buffers {
   const float* xyz;
   const int* elem;
-  int* counts;           // [F*E*nty*ntx + 1]
-  int* offsets;          // [F*E*nty*ntx + 1]
+  int* counts;           // [F * frame_keys + 1]
+  int* offsets;          // [F * frame_keys + 1]
   int2* keyslot;         // [F*N]
   float4* sorted;        // [F*N]  (r_x, r_y, r_z, atom index bits)
   unsigned long long* pad_sum;   // [F*E] border-slab weight, fixed point (to_fixed_d)
   const unsigned long long* pad_given;  // [E] given pad in units of 2^-32, or NULL
-  const int4* srcx;      // [ntx] {count, bin0, bin1, bin2}
-  const int4* srcy;      // [nty]
+  const int4* srcx;      // [ntx] {runs (1, 2), first x unit of run 0, of run 1, len0 | len1 << 16}
+  const int4* srcy;      // [nty] {count (1..3), y unit 0, 1, 2}
   float2* zbuf;
   float2* ybuf;
   const float2* twz;
@@ -250,8 +251,44 @@ __global__ void k_clear(int* counts, int nkeys, unsigned long long* pad_sum, int
   if (i < npad) pad_sum[i] = 0ull;
 }
 
-// Per atom of frame f: its column-tile key within the frame,
-// (e ny_tile + ty) ntx + tx, and its border-slab share of unit weight in
+// Sort keys of one frame.  A key is (element, y unit, x unit) with the unit
+// of an axis S C + c: S the tile holding the stencil's first point (its home
+// tile), c (with C = 2 classes) 1 when the stencil leaves that tile, else 0.
+// A tile then fetches only atoms that reach it: its home units and the
+// class-1 unit of the tile before it (C = 2), or every atom of each source
+// tile (C = 1, axes whose stencils can reach three tiles).
+__host__ __device__ inline int frame_keys(const Params& p) {
+  return p.E * p.nty * p.ncy * p.ntx * p.ncx;
+}
+
+// Unit of stencil base `base` along an axis of n cells in tiles of t.
+__device__ __forceinline__ int key_unit(int base, int t, int n, int C, int P) {
+  const int S = base / t;
+  if (C == 1) return S;
+  const int X0 = S * t, w = min(t, n - X0);
+  return 2 * S + (base - X0 + P - 1 >= w ? 1 : 0);
+}
+
+// Record range {first sorted position, count} of fetch slot sg < 9 of column
+// tile (bx, by), element e, frame f: slot (i, j) = (x-unit run i < 2, y unit
+// j < 3) as sg = i + 3 j.  A run of consecutive keys is one range of the
+// sorted records (their offsets are an exclusive scan within the frame).
+__device__ __forceinline__ int2 tile_slot(const Params& p, const Buffers& b, int f, int e, int bx,
+                                          int by, int sg) {
+  const int i = sg % 3, j = sg / 3;
+  const int4 sx = b.srcx[bx], sy = b.srcy[by];
+  if (i >= sx.x || j >= sy.x) return make_int2(0, 0);
+  const int a = i == 0 ? sx.y : sx.z;
+  const int len = i == 0 ? (sx.w & 0xffff) : (sx.w >> 16);
+  const int yu = j == 0 ? sy.y : (j == 1 ? sy.z : sy.w);
+  const int k0 = ((f * p.E + e) * (p.nty * p.ncy) + yu) * (p.ntx * p.ncx) + a;
+  const int k1 = k0 + len - 1;
+  const int st = b.offsets[k0];
+  return make_int2(st, b.offsets[k1] + b.counts[k1] - st);
+}
+
+// Per atom of frame f: its sort key within the frame (frame_keys) and its
+// border-slab share of unit weight in
 // fixed point: the sum over the stencil of th_x th_y th_z restricted to
 // z <= d or z >= n_z - d.
 template <int P>
@@ -268,7 +305,8 @@ __device__ __forceinline__ void atom_key_border(const Params& p, const Buffers& 
     w[a] = r - (float)m;
     base[a] = wrap_base(m, P, p.n[a]);
   }
-  key = (e * p.nty + base[1] / p.ty) * p.ntx + base[0] / p.tx;
+  key = (e * (p.nty * p.ncy) + key_unit(base[1], p.ty, p.n[1], p.ncy, P)) * (p.ntx * p.ncx) +
+        key_unit(base[0], p.tx, p.n[0], p.ncx, P);
   float th[P];
   double sx = 0.0, sy = 0.0, sz = 0.0;
   bspline<P>(w[0], th);
@@ -302,7 +340,7 @@ __global__ void __launch_bounds__(kWG) k_prep(Params p, Buffers b) {
     int key;
     unsigned long long border;
     atom_key_border<P>(p, b, f, i, e, key, border);
-    key += f * p.E * p.nty * p.ntx;
+    key += f * frame_keys(p);
     const int slot = atomicAdd(&b.counts[key], 1);
     b.keyslot[(size_t)f * p.N + i] = make_int2(key, slot);
     if (border != 0ull) atomicAdd(&padacc[e], border);
@@ -338,7 +376,7 @@ constexpr int kSeg16MaxLen = 65535;   // atoms per segment with 16-bit counters
 template <int P, bool HW>
 __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int K = p.E * p.nty * p.ntx;
+  const int K = frame_keys(p);
   const int KR = HW ? b.segkr : K;                   // keys per pass (LDS): 32-bit, one pass
   const int KRW = HW ? (KR + 1) / 2 : KR;
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
@@ -395,7 +433,7 @@ __global__ void __launch_bounds__(kSortWG) k_prep_seg(Params p, Buffers b) {
 // run) and the key's total.  One thread per key, loads and stores coalesced
 // along the keys.
 __global__ void __launch_bounds__(kWG) k_seg_keys(Params p, Buffers b) {
-  const int K = p.E * p.nty * p.ntx;
+  const int K = frame_keys(p);
   const int k = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
   if (k >= K) return;
   int* seg = b.segoff + (size_t)f * b.nseg * K + k;
@@ -469,7 +507,7 @@ __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
   rec.y = rr[1];
   rec.z = rr[2];
   rec.w = __int_as_float(i);
-  const int K = p.E * p.nty * p.ntx;
+  const int K = frame_keys(p);
   int pos = b.offsets[ks.x] + ks.y;
   if (b.nseg > 0) pos += b.segoff[((size_t)f * b.nseg + i / b.seglen) * K + (ks.x - f * K)];
   b.sorted[pos] = rec;
@@ -606,18 +644,9 @@ __global__ void __launch_bounds__(kWG) k_spread_z(Params p, Buffers b) {
   // source segments: up to 3x3 bins whose stencils reach this tile
   __shared__ int s_start[9], s_cnt[9], s_pref[10];
   if (threadIdx.x < 9) {
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = threadIdx.x % 3, iy = threadIdx.x / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * p.E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[threadIdx.x] = st;
-    s_cnt[threadIdx.x] = cnt;
+    const int2 r = tile_slot(p, b, f, e, bx, by, threadIdx.x);
+    s_start[threadIdx.x] = r.x;
+    s_cnt[threadIdx.x] = r.y;
   }
   for (int i = threadIdx.x; i < ncol * cs; i += blockDim.x) rho[i] = 0ull;
   for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
@@ -747,19 +776,10 @@ __global__ void __launch_bounds__(kWG) k_spread_zall(Params p, Buffers b) {
   int* s_pref = s_cnt + 9 * E;                        // [E][10]
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
-    const int e = t / 9, sg = t - e * 9;
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = sg % 3, iy = sg / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[t] = st;
-    s_cnt[t] = cnt;
+    const int e = t / 9;
+    const int2 r = tile_slot(p, b, f, e, bx, by, t - e * 9);
+    s_start[t] = r.x;
+    s_cnt[t] = r.y;
   }
   for (int i = threadIdx.x; i < nnz; i += blockDim.x) tw[i] = b.twz[i];
   __syncthreads();
@@ -914,19 +934,10 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   int* s_pref = s_cnt + 9 * E;
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
-    const int e = t / 9, sg = t - e * 9;
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = sg % 3, iy = sg / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[t] = st;
-    s_cnt[t] = cnt;
+    const int e = t / 9;
+    const int2 r = tile_slot(p, b, f, e, bx, by, t - e * 9);
+    s_start[t] = r.x;
+    s_cnt[t] = r.y;
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[(i >> 4) * (i & 15)];
   for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
@@ -1191,19 +1202,10 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   int* s_pref = s_cnt + 9 * E;
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
-    const int e = t / 9, sg = t - e * 9;
-    const int4 sx = b.srcx[bx], sy = b.srcy[by];
-    const int ix = sg % 3, iy = sg / 3;
-    int st = 0, cnt = 0;
-    if (ix < sx.x && iy < sy.x) {
-      const int kx = ix == 0 ? sx.y : (ix == 1 ? sx.z : sx.w);
-      const int ky = iy == 0 ? sy.y : (iy == 1 ? sy.z : sy.w);
-      const int key = ((f * E + e) * p.nty + ky) * p.ntx + kx;
-      st = b.offsets[key];
-      cnt = b.counts[key];
-    }
-    s_start[t] = st;
-    s_cnt[t] = cnt;
+    const int e = t / 9;
+    const int2 r = tile_slot(p, b, f, e, bx, by, t - e * 9);
+    s_start[t] = r.x;
+    s_cnt[t] = r.y;
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[i];
   for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
