@@ -245,11 +245,15 @@ int adjg_half(bool spread_all, bool pruned_z, const saxs::Params& p) {
 
 // Half-length H of the first stage of the generic output-pruned y transform
 // (k_yfft_g, N = 2H R2), or 0 where it does not apply: it needs n_y <= N/2,
-// few kept outputs (Ky < N/8), an even R1 = 2H dividing N with R2 <= 64,
-// and the LDS (the same order of preference as zg_half).
+// not every output kept (ky < N), an even R1 = 2H dividing N with R2 <= 64,
+// and the LDS (the same order of preference as zg_half).  Its stage-1
+// direct sums cost about twice a full transform's arithmetic when half the
+// outputs are kept (the reference CLI's q <= 2 on a 320-point axis), and in
+// one LDS exchange it still beats the multi-stage Stockham kernel there
+// (2.37 against 3.16 ms per 50 frames of 128^3 -> 320^3).
 int yg_half(bool pruned_z, const saxs::Params& p) {
   const int Ny = p.nn[1];
-  if (!pruned_z || 2 * p.n[1] > Ny || p.ky >= Ny || p.Ky >= Ny / 8) return 0;
+  if (!pruned_z || 2 * p.n[1] > Ny || p.ky >= Ny) return 0;
   for (int H : {8, 7, 6, 5, 9, 10}) {
     const int R1 = 2 * H;
     if (Ny % R1 != 0 || Ny / R1 > 64) continue;
@@ -447,10 +451,11 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
     }
   }
   // generic pruned x pass and its adjoint (over the same z chunks): n_x <=
-  // N/2, few kept m_x, an even R1 = 2H dividing N, every kept pair of a
-  // chunk in kXgItems per thread
+  // N/2, not every m_x kept, an even R1 = 2H dividing N, every kept pair of
+  // a chunk in kXgItems per thread (at the reference CLI's q <= 2 on a
+  // 320-point axis 3.04 against 4.31 ms per 50 frames for the Stockham pass)
   pl->x_gen_h = 0;
-  if (pl->pruned_z && !pl->x_pruned && 2 * p.n[0] <= Nx && p.kx < Nx && p.Kx < Nx / 8 &&
+  if (pl->pruned_z && !pl->x_pruned && 2 * p.n[0] <= Nx && p.kx < Nx &&
       p.kcx * (p.Kx + 1) <= saxs::kXgItems * saxs::kWG) {
     for (int H : {8, 7, 6, 5, 9, 10}) {
       if (Nx % (2 * H) != 0 || Nx / (2 * H) > 64) continue;

@@ -264,6 +264,15 @@ def test_adjoint_generic_pruned_z_against_oracle(gpu, n, L, order, qcut):
     run_case(n, L, max(600, int(0.1 * L ** 3)), order=order, qcut=qcut, seed=n + 5)
 
 
+@pytest.mark.parametrize("n,L,qcut", [(36, 27.0, 1.6),    # N 72 = 12 x 6: K = 13 >= N/8
+                                      (40, 30.0, 1.5)])   # N 80 = 16 x 5: K = 14 >= N/8
+def test_adjoint_generic_y_x_many_kept_against_oracle(gpu, n, L, qcut):
+    """The generic y and x transforms and their adjoints where most outputs
+    are kept (N/8 <= K < N/2, the q range of the reference CLI's defaults),
+    gradient against the FP64 oracle."""
+    run_case(n, L, max(600, int(0.1 * L ** 3)), qcut=qcut, seed=n + 7)
+
+
 def test_adjoint_generic_pruned_z_matches_stockham(gpu):
     """189^3 cell (N = 378 = 14 x 27): the generic pruned z adjoint against
     the Stockham adjoint (flag 2) on the same inputs and dL/dH."""

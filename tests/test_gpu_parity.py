@@ -341,7 +341,8 @@ def test_generic_pruned_z_matches_full_transform(gpu, n, tile):
 
 @pytest.mark.parametrize("n,qcut", [((48, 70, 64), 0.6),     # y: N 140 = 20 x 7; z 128 radix 16
                                     ((40, 81, 45), 0.5),     # y: 162 = 18 x 9; z: 90 = 18 x 5
-                                    ((36, 105, 50), 0.45)])  # y: 210 = 14 x 15
+                                    ((36, 105, 50), 0.45),   # y: 210 = 14 x 15
+                                    ((48, 70, 64), 2.0)])    # K_y = 33 >= N/8: most outputs kept
 def test_generic_pruned_y_against_oracle(gpu, n, qcut):
     """The output-pruned y transform for N = 2H R2 on non-cubic cells (each
     axis its own transform length), against the oracle."""
@@ -358,7 +359,8 @@ def test_generic_pruned_y_against_oracle(gpu, n, qcut):
 
 @pytest.mark.parametrize("n,qcut", [((70, 64, 48), 0.6),     # x: N 140 = 14 x 10
                                     ((81, 40, 45), 0.5),     # x: 162 = 18 x 9
-                                    ((105, 36, 50), 0.45)])  # x: 210 = 14 x 15
+                                    ((105, 36, 50), 0.45),   # x: 210 = 14 x 15
+                                    ((70, 64, 48), 2.0)])    # K_x = 33 >= N/8: most outputs kept
 def test_generic_pruned_x_against_oracle(gpu, n, qcut):
     """The output-pruned x transform fused with the form-factor sum, moduli
     and histogram, for N = 2H R2 on non-cubic cells, against the oracle."""
