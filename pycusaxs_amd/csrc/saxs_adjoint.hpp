@@ -1011,7 +1011,7 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
     for (int o2 = 0; o2 < P; ++o2) {
       int ly = dy + o2;
       if (ly >= p.n[1]) ly -= p.n[1];
-      yo[o2] = ly * cs + gz0;
+      yo[o2] = (int)__umul24((unsigned)ly, (unsigned)cs) + gz0;   // LDS offsets < 2^24
     }
     float gx = 0.0f, gy = 0.0f, gzs = 0.0f;
 #pragma unroll
@@ -1019,7 +1019,7 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
       if (o < xlo || o >= xhi) continue;
       int lx = dx + o;
       if (lx >= p.n[0]) lx -= p.n[0];
-      const float* xp = drho + lx * plane;
+      const float* xp = drho + __umul24((unsigned)lx, (unsigned)plane);
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
       for (int o2 = 0; o2 < P; ++o2) {

@@ -170,8 +170,25 @@ __device__ __forceinline__ void frame_cell_coords(const float* x, const Params& 
 
 // Cardinal B-spline weights th[k] = M_P(w + P - 1 - k) by the recursion of
 // Splines.cu:47-80 (Init then OnePass for k = 3..P).
+// Cubic weights in closed form, times `scale`: with u = 1 - w,
+//   th0 = u^3/6, th1 = (4 - 6w^2 + 3w^3)/6, th2 = (1 + 3w + 3w^2 - 3w^3)/6,
+//   th3 = w^3/6
+// (the order-4 recursion below expanded), in 12 operations with the scale
+// folded into the constants.
+__device__ __forceinline__ void bspline4(float w, float scale, float* th) {
+  const float u = 1.0f - w, w2 = w * w, hw = w * (0.5f * scale);
+  th[3] = w2 * (w * (scale / 6.0f));
+  th[0] = (u * u) * (u * (scale / 6.0f));
+  th[1] = fmaf(w2, fmaf(w, 0.5f * scale, -scale), scale * (2.0f / 3.0f));
+  th[2] = fmaf(hw, fmaf(w, u, 1.0f), scale / 6.0f);
+}
+
 template <int P>
 __device__ __forceinline__ void bspline(float w, float* th) {
+  if constexpr (P == 4) {
+    bspline4(w, 1.0f, th);
+    return;
+  }
   th[0] = 1.0f - w;
   th[1] = w;
 #pragma unroll
@@ -201,8 +218,7 @@ __device__ __forceinline__ void bspline_d(float w, float* dth) {
 // Cell index m = int(r), clamped to [0, n] so that non-finite coordinates
 // cannot index outside the grid (finite inputs always give 0 <= r <= n).
 __device__ __forceinline__ int cell_index(float r, int n) {
-  const int m = (int)r;
-  return m < 0 ? 0 : (m > n ? n : m);
+  return min(max((int)r, 0), n);
 }
 
 __device__ __forceinline__ int wrap_base(int m, int P, int n) {
@@ -573,9 +589,13 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
     float thx[P], thy[P], thz[P];
     bspline<P>(rec.x - (float)mx, thx);
     bspline<P>(rec.y - (float)my, thy);
-    bspline<P>(rec.z - (float)mz, thz);
+    if constexpr (P == 4) {
+      bspline4(rec.z - (float)mz, 4294967296.0f, thz);   // 2^32 (exact): to_fixed's scale
+    } else {
+      bspline<P>(rec.z - (float)mz, thz);
 #pragma unroll
-    for (int q = 0; q < P; ++q) thz[q] *= 4294967296.0f;   // 2^32 (exact): to_fixed's scale
+      for (int q = 0; q < P; ++q) thz[q] *= 4294967296.0f;
+    }
     int gz0 = mz - P;
     if (gz0 < 0) gz0 += nz;
     int yo[P];
@@ -583,7 +603,7 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
     for (int o2 = 0; o2 < P; ++o2) {
       int ly = dy + o2;
       if (ly >= p.n[1]) ly -= p.n[1];
-      yo[o2] = ly * cs + gz0;
+      yo[o2] = (int)__umul24((unsigned)ly, (unsigned)cs) + gz0;   // LDS offsets < 2^24
     }
 #pragma unroll
     for (int oo = 0; oo < P; oo += LPR) {
@@ -591,7 +611,7 @@ __device__ __forceinline__ void spread_cols(const float4* recs, int count,
       if (o >= P || o < xlo || o >= xhi) continue;
       int lx = dx + o;
       if (lx >= p.n[0]) lx -= p.n[0];
-      unsigned long long* xp = rho + lx * plane;
+      unsigned long long* xp = rho + __umul24((unsigned)lx, (unsigned)plane);
       const float fx = thx[o];
 #pragma unroll
       for (int o2 = 0; o2 < P; ++o2) {
@@ -1136,22 +1156,28 @@ __host__ __device__ inline size_t spread_g_lds_bytes(const Params& p, int R1) {
 }
 
 // Horner accumulation for the pruned transforms' kept outputs: with y_t the
-// entries p[s - t d] and p[sm - t d] (t = 0 .. T-1, read from the highest
-// term down), returns Z = sum_t y1_t u^(T-1-t) and Zm = sum_t y2_t
-// conj(u)^(T-1-t), i.e. Horner's rule over the terms in descending order.
-// Four terms' loads are issued before their multiply-adds: the chain is
-// bound by LDS latency otherwise.
-__device__ __forceinline__ void horner_pair(const float2* p, int d, int s, int sm, int T, float2 u,
-                                            float2& Z, float2& Zm) {
+// entries p[s - t D] and p[sm - t D] (t = 0 .. T-1, read from the highest
+// term down; D a compile-time stride, so the loads take immediate LDS
+// offsets from two running pointers), returns Z = sum_t y1_t u^(T-1-t) and
+// Zm = sum_t y2_t conj(u)^(T-1-t), i.e. Horner's rule over the terms in
+// descending order.  Four terms' loads are issued before their multiply-
+// adds: the chain is bound by LDS latency otherwise.
+template <int D>
+__device__ __forceinline__ void horner_pair_c(const float2* p, int s, int sm, int T, float2 u,
+                                              float2& Z, float2& Zm) {
   float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+  const float2* p1 = p + s;
+  const float2* p2 = p + sm;
   int t = 0;
   for (; t + 4 <= T; t += 4) {
     float2 y1[4], y2[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      y1[v] = p[s - (t + v) * d];
-      y2[v] = p[sm - (t + v) * d];
+      y1[v] = p1[-v * D];
+      y2[v] = p2[-v * D];
     }
+    p1 -= 4 * D;
+    p2 -= 4 * D;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const float ax = fmaf(A.x, u.x, fmaf(-A.y, u.y, y1[v].x));
@@ -1163,7 +1189,9 @@ __device__ __forceinline__ void horner_pair(const float2* p, int d, int s, int s
     }
   }
   for (; t < T; ++t) {
-    const float2 y1 = p[s - t * d], y2 = p[sm - t * d];
+    const float2 y1 = *p1, y2 = *p2;
+    p1 -= D;
+    p2 -= D;
     const float ax = fmaf(A.x, u.x, fmaf(-A.y, u.y, y1.x));
     const float ay = fmaf(A.x, u.y, fmaf(A.y, u.x, y1.y));
     const float bx = fmaf(B.x, u.x, fmaf(B.y, u.y, y2.x));
@@ -1173,6 +1201,11 @@ __device__ __forceinline__ void horner_pair(const float2* p, int d, int s, int s
   }
   Z = A;
   Zm = B;
+}
+
+// a mod n for 0 <= a < 2^22 (div_small's range).
+__device__ __forceinline__ int mod_small(int a, int n, float inv_n) {
+  return a - n * div_small(a, n, inv_n);
 }
 
 // The plan guarantees R2 <= 64 (a stage-0 pass holds a whole line) and
@@ -1247,7 +1280,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   int lgs = 0;
   while (lgs < 3 && (kz << (lgs + 1)) <= 64) ++lgs;
   const int split = 1 << lgs;
-  const float inv_kz = 1.0f / (float)kz;
+  const int T0 = (R2 + split - 1) / split;           // terms of one part of a stage-1 sum
+  const float inv_kz = 1.0f / (float)kz, inv_n = 1.0f / (float)N;
   float4 pre[kZpPre];
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
@@ -1343,14 +1377,13 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
         k = q - l * kz;
         l += lw0;
         const int s = k % R1, sm = s == 0 ? 0 : R1 - s;
-        // the part's terms j = part + split t, t < T, by Horner's rule in
-        // u = W_N^{split k} from the last term down (conj u for Z(N - k)),
-        // then the factor W_N^{part k}: sum_t y_j W_N^{jk} = W_N^{part k}
-        // sum_t y_{part + split t} u^t
-        const float2 u = tw[(split * k) % N], wp = tw[(part * k) % N];
-        const int T = part < R2 ? (R2 - 1 - part) / split + 1 : 0;
-        if (T > 0)
-          horner_pair(ybuf + l * LY + YS * (part + split * (T - 1)), YS * split, s, sm, T, u, Z, Zm);
+        // the part's terms j = j0 .. j0 + T - 1 (j0 = part T0) by Horner's
+        // rule in u = W_N^k from the last term down (conj u for Z(N - k)),
+        // then the factor W_N^{j0 k}: sum_j y_j W_N^{jk} = W_N^{j0 k} sum_t
+        // y_{j0 + t} u^t (k < kz <= N/2 + 1, so u = tw[k])
+        const int j0 = part * T0, T = min(T0, R2 - j0);
+        const float2 u = tw[k], wp = tw[mod_small(j0 * k, N, inv_n)];
+        if (T > 0) horner_pair_c<YS>(ybuf + l * LY + YS * (j0 + T - 1), s, sm, T, u, Z, Zm);
         Z = cmul(Z, wp);
         Zm = cmul(Zm, make_float2(wp.x, -wp.y));
       }
@@ -1592,6 +1625,8 @@ __global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
   int lgs = 0;
   while (lgs < 3 && (nq << (lgs + 1)) <= kWG) ++lgs;
   const int split = 1 << lgs;
+  const int T0 = (R2 + split - 1) / split;           // terms of one part of a stage-1 sum
+  const float inv_n = 1.0f / (float)N;
   for (int t0 = 0; t0 < (nq << lgs); t0 += kWG) {
     const int t = t0 + threadIdx.x;
     const int part = t & (split - 1), qq = t >> lgs;
@@ -1600,12 +1635,12 @@ __global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
     float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
     if (act) {
       const int s = m % R1, sm = s == 0 ? 0 : R1 - s;
-      // sum_j y_j W_N^{jm} over j = part + split t by Horner's rule in
-      // u = W_N^{split m} (conj u for the mirrored output), times W_N^{part m}
-      const float2 u = tw[(split * m) % N], wp = tw[(part * m) % N];
-      const int T = part < R2 ? (R2 - 1 - part) / split + 1 : 0;
-      if (T > 0)
-        horner_pair(tbuf + l * LY + YS * (part + split * (T - 1)), YS * split, s, sm, T, u, A, B);
+      // sum_j y_j W_N^{jm} over the part's j = j0 .. j0 + T - 1 (j0 = part T0)
+      // by Horner's rule in u = W_N^m (conj u for the mirrored output), times
+      // W_N^{j0 m} (m <= Ky < N/2, so u = tw[m])
+      const int j0 = part * T0, T = min(T0, R2 - j0);
+      const float2 u = tw[m], wp = tw[mod_small(j0 * m, N, inv_n)];
+      if (T > 0) horner_pair_c<YS>(tbuf + l * LY + YS * (j0 + T - 1), s, sm, T, u, A, B);
       A = cmul(A, wp);
       B = cmul(B, make_float2(wp.x, -wp.y));
     }
@@ -1920,7 +1955,7 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
         float2 Z = make_float2(0.0f, 0.0f), Zm = make_float2(0.0f, 0.0f);
         // sum_j y_j W_N^{jm} by Horner's rule in wm = W_N^m from j = R2 - 1
         // down (conj wm for the mirrored output)
-        horner_pair(yl + YS * (R2 - 1), YS, s, sm, R2, tw[m], Z, Zm);
+        horner_pair_c<YS>(yl + YS * (R2 - 1), s, sm, R2, tw[m], Z, Zm);
         const int mz = k0 + k;
         const int n0 = shell_key(m, my, mz, p);
         if (n0 < p.lut_len) {
