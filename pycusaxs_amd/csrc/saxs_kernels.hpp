@@ -1277,8 +1277,11 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   // split depends on kz alone, so every output is summed in the same order
   // whatever the tile (and the spectra do not depend on it)
   const int nq = (lw1 - lw0) * kz;
+  // (kz split <= 32: a wavefront's two or more lines then fill its lanes
+  // without more iterations than an unsplit sum, which costs fewer
+  // instructions than the shorter parts plus their shuffles)
   int lgs = 0;
-  while (lgs < 3 && (kz << (lgs + 1)) <= 64) ++lgs;
+  while (lgs < 3 && (kz << (lgs + 1)) <= 32) ++lgs;
   const int split = 1 << lgs;
   const int T0 = (R2 + split - 1) / split;           // terms of one part of a stage-1 sum
   const float inv_kz = 1.0f / (float)kz, inv_n = 1.0f / (float)N;
