@@ -1306,6 +1306,34 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
     return staged_record(b, s_start[e * 9 + sgm] + (a - pref[sgm]));
   };
 
+  // Stage-0 inputs of thread (line l, butterfly j): the spectrum gradients
+  // of the line's two columns at k = j + R2 r (r = 0, 1, k < kz) and
+  // N - k = N - (j + R2 r) (r = 14, 15, < kz), loaded one element ahead so
+  // that the loads are in flight during the previous element's stage 1 and
+  // gather.
+  const bool act0 = (int)threadIdx.x < nl * R2;
+  const int l0 = act0 ? (int)threadIdx.x / R2 : 0, j0 = (int)threadIdx.x - l0 * R2;
+  const int lx0 = (2 * l0) / p.ty, ly0 = 2 * l0 - lx0 * p.ty;
+  const bool ok0 = act0 && lx0 < wx && ly0 < wy, ok1 = ok0 && ly0 + 1 < wy;
+  float2 za_v[4], zb_v[4];
+  auto zload = [&](int e) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) za_v[q] = zb_v[q] = make_float2(0.0f, 0.0f);
+    if (ok0) {
+      const float2* za =
+          b.zbuf + ((((size_t)f * E + e) * p.n[0] + (X0 + lx0)) * p.n[1] + (Y0 + ly0)) * kz;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = q < 2 ? j0 + R2 * q : N - (j0 + R2 * (12 + q));
+        if (k < kz) {
+          za_v[q] = za[k];
+          if (ok1) zb_v[q] = za[kz + k];
+        }
+      }
+    }
+  };
+  zload(0);
+
   for (int e = 0; e < E; ++e) {
     const size_t fe = (size_t)f * E + e;
     const int total = s_pref[e * 10 + 9];
@@ -1317,24 +1345,18 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
       const int a = threadIdx.x + u * blockDim.x;
       if (a < staged) pre[u] = fetch(e, a);
     }
-    // stage 0 (item = line l, butterfly j): sparse conj(S) from zbuf, DFT16
-    if (threadIdx.x < nl * R2) {
-      const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;
-      const int c0 = 2 * l;
-      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
-      const bool ok0 = lx < wx && ly < wy, ok1 = lx < wx && ly + 1 < wy;
+    // stage 0 (item = line l, butterfly j): sparse conj(S), DFT16
+    if (act0) {
+      const int l = l0, j = j0;
       float2 v[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
       if (ok0) {
-        const float2* za = b.zbuf + ((fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly)) * kz;
-        const float2* zb = za + kz;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {                 // k = j + R2 r < kz
           const int k = j + R2 * r;
           if (k < kz) {
-            const float2 a = za[k];
-            const float2 c = ok1 ? zb[k] : make_float2(0.0f, 0.0f);
+            const float2 a = za_v[r], c = zb_v[r];
             v[r] = k == 0 ? make_float2(a.x, -c.x)
                           : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
           }
@@ -1343,12 +1365,12 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
         for (int r = 14; r < 16; ++r) {               // N - k < kz
           const int m = N - (j + R2 * r);
           if (m < kz) {
-            const float2 a = za[m];
-            const float2 c = ok1 ? zb[m] : make_float2(0.0f, 0.0f);
+            const float2 a = za_v[r - 12], c = zb_v[r - 12];
             v[r] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
           }
         }
       }
+      if (e + 1 < E) zload(e + 1);                     // the next element's inputs
       dft16_in_ends(v);                                // nonzero inputs r = 0, 1, 14, 15
       float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll

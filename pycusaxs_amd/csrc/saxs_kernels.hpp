@@ -921,7 +921,7 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
   const size_t B = (size_t)kZpRecs * sizeof(float4);
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j s} as [j][s]
-  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  const size_t seg = (size_t)p.E * (28 * sizeof(int) + sizeof(unsigned long long));
   return A + B + tw + seg;
 }
 
@@ -952,6 +952,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   int* s_start = reinterpret_cast<int*>(tw + N);
   int* s_cnt = s_start + 9 * E;
   int* s_pref = s_cnt + 9 * E;
+  // bulk-solvent mean of each element in the columns' fixed-point units (a
+  // column outside the cell keeps its zeros), read once for the elements
+  unsigned long long* s_pad = reinterpret_cast<unsigned long long*>(s_pref + 10 * E);
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
     const int e = t / 9;
@@ -959,6 +962,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     s_start[t] = r.x;
     s_cnt[t] = r.y;
   }
+  for (int e = threadIdx.x; e < E; e += blockDim.x) s_pad[e] = pad_units(b, p, (size_t)f * E + e, e);
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[(i >> 4) * (i & 15)];
   for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
     reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
@@ -1010,9 +1014,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 
     const size_t fe = (size_t)f * E + e;
     float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
-    // bulk-solvent mean in the columns' fixed-point units (a column outside
-    // the cell keeps its zeros)
-    const unsigned long long pad = pad_units(b, p, fe, e);
+    const unsigned long long pad = s_pad[e];
     // stage 0 (item = line l, butterfly j): columns 2l, 2l+1 -> line l in place
     if (threadIdx.x < nl * R2) {
       const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;
@@ -1151,7 +1153,7 @@ __host__ __device__ inline size_t spread_g_lds_bytes(const Params& p, int R1) {
   const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
   const size_t B = (size_t)kZpRecs * sizeof(float4);
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^m
-  const size_t seg = (size_t)p.E * 28 * sizeof(int);
+  const size_t seg = (size_t)p.E * (28 * sizeof(int) + sizeof(unsigned long long));
   return A + B + tw + seg;
 }
 
@@ -1233,6 +1235,9 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   int* s_start = reinterpret_cast<int*>(tw + N);
   int* s_cnt = s_start + 9 * E;
   int* s_pref = s_cnt + 9 * E;
+  // bulk-solvent mean of each element in the columns' fixed-point units (a
+  // column outside the cell keeps its zeros), read once for the elements
+  unsigned long long* s_pad = reinterpret_cast<unsigned long long*>(s_pref + 10 * E);
 
   for (int t = threadIdx.x; t < 9 * E; t += blockDim.x) {
     const int e = t / 9;
@@ -1240,6 +1245,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
     s_start[t] = r.x;
     s_cnt[t] = r.y;
   }
+  for (int e = threadIdx.x; e < E; e += blockDim.x) s_pad[e] = pad_units(b, p, (size_t)f * E + e, e);
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twz[i];
   for (int i = threadIdx.x; i < ncol * cs / 2; i += blockDim.x)
     reinterpret_cast<uint4*>(rho)[i] = make_uint4(0u, 0u, 0u, 0u);
@@ -1307,7 +1313,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
 
     const size_t fe = (size_t)f * E + e;
     float2* zt = b.zbuf + ((fe * p.n[0] + X0) * p.n[1] + Y0) * kz;   // tile origin
-    const unsigned long long pad = pad_units(b, p, fe, e);
+    const unsigned long long pad = s_pad[e];
     const int lane = threadIdx.x & 63;
     // stage 0, a pass of g0 whole lines: lane -> (line, butterfly j)
     {
