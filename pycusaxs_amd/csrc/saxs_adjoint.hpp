@@ -1308,9 +1308,14 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
 
   // Stage-0 inputs of thread (line l, butterfly j): the spectrum gradients
   // of the line's two columns at k = j + R2 r (r = 0, 1, k < kz) and
-  // N - k = N - (j + R2 r) (r = 14, 15, < kz), loaded one element ahead so
-  // that the loads are in flight during the previous element's stage 1 and
-  // gather.
+  // N - k = N - (j + R2 r) (r = 14, 15, < kz), loaded one element ahead
+  // (N <= 256) so that the loads are in flight during the previous
+  // element's stage 1 and gather: 1.80 -> 1.66 ms per 100 frames at N = 256.
+  // At N = 512, whose stage 1 holds 32 complex values a thread, the extra
+  // registers cost more than the hidden latency (5.83 -> 7.15 ms per 16
+  // frames of the 1M-atom system), and the element's own inputs are loaded
+  // at its stage 0.
+  constexpr bool kAhead = N <= 256;
   const bool act0 = (int)threadIdx.x < nl * R2;
   const int l0 = act0 ? (int)threadIdx.x / R2 : 0, j0 = (int)threadIdx.x - l0 * R2;
   const int lx0 = (2 * l0) / p.ty, ly0 = 2 * l0 - lx0 * p.ty;
@@ -1332,7 +1337,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
       }
     }
   };
-  zload(0);
+  if constexpr (kAhead) zload(0);
 
   for (int e = 0; e < E; ++e) {
     const size_t fe = (size_t)f * E + e;
@@ -1346,31 +1351,60 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
       if (a < staged) pre[u] = fetch(e, a);
     }
     // stage 0 (item = line l, butterfly j): sparse conj(S), DFT16
-    if (act0) {
-      const int l = l0, j = j0;
+    if (threadIdx.x < nl * R2) {
+      const int l = threadIdx.x / R2, j = threadIdx.x - l * R2;
       float2 v[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
-      if (ok0) {
+      if constexpr (kAhead) {
+        if (ok0) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {                 // k = j + R2 r < kz
-          const int k = j + R2 * r;
-          if (k < kz) {
-            const float2 a = za_v[r], c = zb_v[r];
-            v[r] = k == 0 ? make_float2(a.x, -c.x)
-                          : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
+          for (int r = 0; r < 2; ++r) {               // k = j + R2 r < kz
+            const int k = j + R2 * r;
+            if (k < kz) {
+              const float2 a = za_v[r], c = zb_v[r];
+              v[r] = k == 0 ? make_float2(a.x, -c.x)
+                            : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
+            }
+          }
+#pragma unroll
+          for (int r = 14; r < 16; ++r) {             // N - k < kz
+            const int m = N - (j + R2 * r);
+            if (m < kz) {
+              const float2 a = za_v[r - 12], c = zb_v[r - 12];
+              v[r] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+            }
           }
         }
+        if (e + 1 < E) zload(e + 1);                   // the next element's inputs
+      } else {
+        const int c0 = 2 * l;
+        const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+        const bool okA = lx < wx && ly < wy, okB = lx < wx && ly + 1 < wy;
+        if (okA) {
+          const float2* za = b.zbuf + ((fe * p.n[0] + (X0 + lx)) * p.n[1] + (Y0 + ly)) * kz;
+          const float2* zb = za + kz;
 #pragma unroll
-        for (int r = 14; r < 16; ++r) {               // N - k < kz
-          const int m = N - (j + R2 * r);
-          if (m < kz) {
-            const float2 a = za_v[r - 12], c = zb_v[r - 12];
-            v[r] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+          for (int r = 0; r < 2; ++r) {               // k = j + R2 r < kz
+            const int k = j + R2 * r;
+            if (k < kz) {
+              const float2 a = za[k];
+              const float2 c = okB ? zb[k] : make_float2(0.0f, 0.0f);
+              v[r] = k == 0 ? make_float2(a.x, -c.x)
+                            : make_float2(0.5f * (a.x - c.y), -0.5f * (a.y + c.x));
+            }
+          }
+#pragma unroll
+          for (int r = 14; r < 16; ++r) {             // N - k < kz
+            const int m = N - (j + R2 * r);
+            if (m < kz) {
+              const float2 a = za[m];
+              const float2 c = okB ? zb[m] : make_float2(0.0f, 0.0f);
+              v[r] = make_float2(0.5f * (a.x + c.y), -0.5f * (c.x - a.y));
+            }
           }
         }
       }
-      if (e + 1 < E) zload(e + 1);                     // the next element's inputs
       dft16_in_ends(v);                                // nonzero inputs r = 0, 1, 14, 15
       float2* dst = ybuf + l * LY + 17 * j;
 #pragma unroll
