@@ -46,17 +46,17 @@ HBM_PEAK_GBS = 8000.0
 # TCC_EA0_RDREQ_128B + 64 B x TCC_EA0_RDREQ_64B, writes = WRITE_SIZE; the same
 # counters give a 512 MiB elementwise copy exactly.  Commands and the
 # per-kernel breakdown: profiles/hbm_traffic_trajectory_100k_128.md.
-MEASURED_TRAFFIC = {"prep": 211.2e6, "scatter": 572.5e6, "spread_z": 1675.8e6, "yfft": 1710.1e6,
-                    "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1710.2e6, "adj_z": 2112.0e6,
-                    "grad_sum": 1042.2e6}
+MEASURED_TRAFFIC = {"prep": 229.7e6, "scatter": 734.7e6, "spread_z": 1710.1e6, "yfft": 1710.1e6,
+                    "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1828.7e6, "adj_z": 2045.5e6,
+                    "grad_sum": 921.5e6}
 
 # Wave64 VALU instructions per launch of each kernel for the default workload,
 # from rocprofv3 SQ_INSTS_VALU on MI355X (commands and the per-kernel table:
 # profiles/sq_counters_trajectory_100k_128.md).  A CU issues at most one wave64
 # VALU instruction per clock, so the chip's issue peak is 256 x 2.4e9 per second.
-MEASURED_VALU = {"prep": 37.3e6, "scatter": 12.2e6, "spread_z": 919.4e6, "yfft": 145.8e6,
-                 "xfft_epi": 41.0e6, "adj_x": 77.2e6, "adj_y": 158.7e6, "adj_z": 906.2e6,
-                 "grad_sum": 59.0e6}
+MEASURED_VALU = {"prep": 35.2e6, "scatter": 12.5e6, "spread_z": 739.9e6, "yfft": 145.8e6,
+                 "xfft_epi": 41.1e6, "adj_x": 77.2e6, "adj_y": 162.5e6, "adj_z": 811.7e6,
+                 "grad_sum": 57.6e6}
 VALU_ISSUE_PEAK = 256 * 2.4e9          # wave64 VALU instructions per second
 
 
