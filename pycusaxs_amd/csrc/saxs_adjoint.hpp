@@ -984,6 +984,7 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
   const int nz = p.n[2];
   const int plane = p.ty * cs;
   const int nslot = g.nsx * g.nsy;
+  float* gframe = g.gpart + (size_t)f * nslot * p.N * 3;   // this frame's slots
   const float sx = p.tric ? (float)p.n[0] : (float)p.n[0] * p.oc[0],
               sy = p.tric ? (float)p.n[1] : (float)p.n[1] * p.oc[1],
               sz = p.tric ? (float)p.n[2] : (float)p.n[2] * p.oc[2];
@@ -1042,7 +1043,8 @@ __device__ __forceinline__ void gather_cols(const float4* recs, int count, const
     }
     const int r = __float_as_int(rec.w) - f * p.N;     // position in the frame's run
     const int slot = partial_slot<P>(mx, my, bx, by, p, g);
-    float* gp = g.gpart + (((size_t)f * nslot + slot) * p.N + r) * 3;
+    // (slot N + r) 3 < 2^32 for every plan (N < 2^24, nslot <= 9)
+    float* gp = gframe + ((__umul24((unsigned)slot, (unsigned)p.N) + (unsigned)r) * 3u);
     gp[0] = gx * sx;
     gp[1] = gy * sy;
     gp[2] = gzs * sz;
@@ -1268,6 +1270,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   const int ncol = p.tx * p.ty;
   const int nl = ncol / 2;
   const int E = p.E, kz = p.kz;
+  const float inv_ty = 1.0f / (float)p.ty;
   const int LY = adj_line_stride(nz, P, N);            // line stride = drho~ column stride
   const size_t A_b = ((size_t)nl * LY * sizeof(float2) + 15) & ~size_t(15);
   const size_t B_b = ((size_t)kZpRecs * sizeof(float4) + 15) & ~size_t(15);
@@ -1428,7 +1431,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
         else dft16<false>(t);
       } else dft32(t);
       const int c0 = 2 * l1;
-      const int lx = c0 / p.ty, ly = c0 - lx * p.ty;
+      const int lx = div_small(c0, p.ty, inv_ty), ly = c0 - lx * p.ty;
       float* col0 = drho + c0 * LY;
       float* col1 = col0 + LY;
       __builtin_amdgcn_wave_barrier();

@@ -204,9 +204,19 @@ __device__ __forceinline__ void bspline(float w, float* th) {
   }
 }
 
-// dth/dw: order P-1 weights differenced (Splines.cu:84-92).
+// dth/dw: order P-1 weights differenced (Splines.cu:84-92); for order 4 in
+// closed form, dth0 = -u^2/2, dth1 = w (3w/2 - 2), dth2 = 1/2 + w - 3w^2/2,
+// dth3 = w^2/2 with u = 1 - w (the quadratic weights differenced).
 template <int P>
 __device__ __forceinline__ void bspline_d(float w, float* dth) {
+  if constexpr (P == 4) {
+    const float u = 1.0f - w, w2 = w * w;
+    dth[0] = -0.5f * (u * u);
+    dth[1] = w * fmaf(w, 1.5f, -2.0f);
+    dth[2] = fmaf(w2, -1.5f, w + 0.5f);
+    dth[3] = 0.5f * w2;
+    return;
+  }
   float t[P];
   bspline<P - 1>(w, t);
   dth[0] = -t[0];
@@ -246,8 +256,9 @@ __device__ __forceinline__ unsigned long long to_fixed_d(double v) {
 // correction each way (exact; cheaper than integer division by a runtime d).
 __device__ __forceinline__ int div_small(int a, int d, float inv_d) {
   int q = (int)((float)a * inv_d);
-  if (q * d > a) --q;
-  if ((q + 1) * d <= a) ++q;
+  const int qd = (int)__umul24((unsigned)q, (unsigned)d);   // q d <= a + d < 2^23
+  if (qd > a) --q;
+  else if (qd + d <= a) ++q;
   return q;
 }
 
