@@ -169,8 +169,22 @@ def leg_restraint(torch, steps, keep):
     ms = float(np.median(times)) * 1e3
     keep.update(T=T[0].cpu().numpy(), s=s.cpu().numpy(), sel=sel.cpu().numpy(),
                 grad=g[0].cpu().numpy(), loss=loss.item(), w=w)
+    # the same step captured once as a graph and replayed (autograd.RestraintStep)
+    from pycusaxs_amd.autograd import RestraintStep
+    gstep = RestraintStep(eng, idt, x0, lambda I: chi2(I[0], T[0], s, sel))
+    gstep(x0)
+    torch.cuda.synchronize()
+    gtimes = []
+    for _ in range(max(steps, 20)):
+        t0 = time.perf_counter()
+        gstep(x0)
+        torch.cuda.synchronize()
+        gtimes.append(time.perf_counter() - t0)
+    gms = float(np.median(gtimes)) * 1e3
+    del gstep
     eng.close()
-    return {"fwd_bwd_ms": round(ms, 3), "chi2": loss.item(), "bins_used": int(sel.sum().item()),
+    return {"fwd_bwd_ms": round(ms, 3), "fwd_bwd_graph_ms": round(gms, 3), "chi2": loss.item(),
+            "bins_used": int(sel.sum().item()),
             "workload": "50,000 atoms, 80 A, 128^3 -> 256^3, order 6, 100 requested bins "
                         "0.005-0.5, chi^2 vs displaced (1 A) structure, 1 % errors"}
 

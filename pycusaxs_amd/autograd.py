@@ -60,3 +60,59 @@ def chi2(I: torch.Tensor, target: torch.Tensor, sigma: torch.Tensor, bins=None) 
         return (r ** 2).sum()
     I, target, sigma = I[..., bins], target[..., bins], sigma[..., bins]
     return (((I - target) / sigma) ** 2).sum()
+
+
+class RestraintStep:
+    """One restraint step on fixed shapes, captured once as a graph.
+
+        step = RestraintStep(engine, elem, x0, lambda I: chi2(I[0], T, s, sel))
+        loss, grad = step(x)        # x [frames, atoms, 3] on the engine's GPU
+
+    Each call copies x into the captured input and replays the engine's
+    forward (saxs_forward_frames), the loss, its backward and the engine's
+    adjoint (saxs_backward on the forward's spectra): no allocation, no host
+    synchronisation and one graph launch per step, where the eager step
+    issues some forty launches through autograd.  The returned tensors are
+    the graph's own and are overwritten by the next call (clone to keep
+    them).  The loss must keep static shapes and synchronise nothing (chi2
+    with a boolean bin mask does).  Construction runs `warmup` eager steps
+    on a side stream first (the first backward allocates the engine's
+    partial-gradient slots); the engine and the loss's tensors must outlive
+    the step."""
+
+    def __init__(self, engine, elem: torch.Tensor, x_example: torch.Tensor, loss_fn,
+                 warmup: int = 2):
+        x = x_example.detach()
+        if x.dim() == 2:
+            x = x.unsqueeze(0)
+        self.x = x.to(device=engine.device, dtype=torch.float32).contiguous().clone()
+        self.x.requires_grad_(True)
+        self.elem = elem.to(device=engine.device, dtype=torch.int32).contiguous()
+        self.engine = engine
+
+        def run():
+            I, _ = saxs_profile(self.x, engine, self.elem)
+            loss = loss_fn(I)
+            loss.backward()
+            return loss
+
+        side = torch.cuda.Stream(device=engine.device)
+        side.wait_stream(torch.cuda.current_stream(engine.device))
+        with torch.cuda.stream(side):
+            for _ in range(max(1, warmup)):
+                self.x.grad = None
+                run()
+        torch.cuda.current_stream(engine.device).wait_stream(side)
+        self.x.grad = None
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.loss = run()
+        self.grad = self.x.grad
+
+    def __call__(self, x: torch.Tensor):
+        if x.dim() == 2:
+            x = x.unsqueeze(0)
+        with torch.no_grad():
+            self.x.copy_(x)
+        self.graph.replay()
+        return self.loss, self.grad

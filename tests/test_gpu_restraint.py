@@ -85,3 +85,47 @@ def test_restraint_gradient(gpu):
     print(f"chi2 engine {loss.item():.6e} oracle {oracle_chi2_and_grad(plan, base, ids, symbols, box, T, s, sel)[0]:.6e}; grad rel-L2 {rel:.2e}", flush=True)
     assert rel <= 2e-4
     eng.close()
+
+
+def test_restraint_step_graph_replays(gpu):
+    """The captured restraint step (autograd.RestraintStep) against the
+    eager autograd step on three coordinate sets, with eager engine work on
+    the same plan and synchronisation between the replays: each replay's
+    loss and gradient equal the eager step's."""
+    import torch
+    from pycusaxs_amd.autograd import RestraintStep, chi2, saxs_profile
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    box, base, target_pos, symbols, ids, _ = setup()
+    n3, nn = resolve_grid((N_CELL,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 6, tuple(symbols), 2.0, DQ, QCUT, "exact"), box,
+                     max_atoms=len(base), max_frames=1)
+    idt = torch.from_numpy(ids).cuda()
+    with torch.no_grad():
+        T, C = saxs_profile(torch.from_numpy(target_pos.astype(np.float32)).cuda()[None], eng, idt)
+    sel = (C[0] > 0) & (torch.arange(C.shape[1], device=C.device) >= 1)
+    s = 0.01 * T[0].abs() + 1e-30
+
+    def loss_fn(I):
+        return chi2(I[0], T[0], s, sel)
+
+    def eager(x):
+        xg = x.clone().requires_grad_(True)
+        I, _ = saxs_profile(xg, eng, idt)
+        loss = loss_fn(I)
+        loss.backward()
+        return loss.detach(), xg.grad
+
+    xs = [torch.from_numpy(O.frame_positions(base, 4, 10 + k, 0.3).astype(np.float32)).cuda()[None]
+          for k in range(3)]
+    step = RestraintStep(eng, idt, xs[0], loss_fn)
+    for k, x in enumerate(xs):
+        l_ref, g_ref = eager(x)                     # eager work on the same plan
+        torch.cuda.synchronize()
+        loss, grad = step(x)
+        torch.cuda.synchronize()
+        rel = ((grad - g_ref).norm() / g_ref.norm()).item()
+        print(f"[replay {k}] loss {loss.item():.6e} (eager {l_ref.item():.6e}), grad rel-L2 {rel:.2e}",
+              flush=True)
+        assert torch.allclose(loss, l_ref, rtol=1e-9, atol=0)
+        assert rel <= 1e-6
+    eng.close()
