@@ -22,6 +22,7 @@
 #ifndef SAXS_XDR_H
 #define SAXS_XDR_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -59,6 +60,18 @@ int saxs_xdr_read(saxs_xdr* h, const int64_t* frames, int count, float* xyz, flo
                   double* time, int64_t* step, int threads);
 
 const char* saxs_xdr_last_error(void);
+
+/* Host-side copy helper with no counterpart in the reference (its frames
+ * reach the GPU through one host copy each): `threads` host threads (the
+ * caller included, 1..64) kept for the copier's life split one memory copy
+ * into whole 64-byte lines each.  Between copies the workers spin for about
+ * 0.2 ms before they block, so a frame loop that copies every few tens of
+ * microseconds wakes no thread.  saxs_copier_copy returns when the copy is
+ * done; calls on one copier must not overlap. */
+typedef struct saxs_copier saxs_copier;
+int saxs_copier_create(int threads, saxs_copier** out);
+int saxs_copier_copy(saxs_copier* c, void* dst, const void* src, size_t bytes);
+void saxs_copier_destroy(saxs_copier* c);
 
 void saxs_xdr_close(saxs_xdr* h);
 

@@ -124,6 +124,88 @@ class Pool {
 
 }  // namespace
 
+// Host threads that split one memory copy.  Between copies the workers spin
+// on the job counter for about 0.2 ms before they block: the frame loop of
+// a run hands over a frame every few tens of microseconds, and a blocking
+// wake-up per frame costs about as much as the copy it would shorten.  The
+// calling thread copies the first part itself and waits for the others.
+struct saxs_copier {
+  explicit saxs_copier(int threads) : nthreads(threads) {
+    for (int i = 1; i < nthreads; ++i) workers.emplace_back([this, i] { loop(i); });
+  }
+  ~saxs_copier() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop.store(true);
+    }
+    cv.notify_all();
+    for (auto& t : workers) t.join();
+  }
+
+  void part(int i) const {
+    // parts of whole 64-byte lines, the last part takes the remainder
+    const size_t lines = bytes / 64, per = lines / nthreads;
+    const size_t b0 = (size_t)i * per * 64;
+    const size_t b1 = i + 1 == nthreads ? bytes : b0 + per * 64;
+    if (b1 > b0) std::memcpy(dst + b0, src + b0, b1 - b0);
+  }
+
+  void copy(char* d, const char* s_, size_t n) {
+    if (nthreads == 1 || n < (size_t)nthreads * 4096) {
+      if (n) std::memcpy(d, s_, n);
+      return;
+    }
+    dst = d;
+    src = s_;
+    bytes = n;
+    remaining.store(nthreads - 1, std::memory_order_relaxed);
+    int sleeping;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      job.fetch_add(1, std::memory_order_release);
+      sleeping = sleepers;
+    }
+    if (sleeping) cv.notify_all();
+    part(0);
+    while (remaining.load(std::memory_order_acquire) > 0) std::this_thread::yield();
+  }
+
+  void loop(int index) {
+    unsigned long long seen = 0;
+    for (;;) {
+      int spins = 0;
+      while (job.load(std::memory_order_acquire) == seen && !stop.load()) {
+        if (++spins < kSpin) {
+          __builtin_ia32_pause();
+        } else {
+          std::unique_lock<std::mutex> lk(mu);
+          ++sleepers;
+          cv.wait(lk, [&] { return job.load() != seen || stop.load(); });
+          --sleepers;
+          spins = 0;
+        }
+      }
+      if (stop.load()) return;
+      seen = job.load(std::memory_order_acquire);
+      part(index);
+      remaining.fetch_sub(1, std::memory_order_release);
+    }
+  }
+
+  static constexpr int kSpin = 5000;       // pause instructions, ~0.2 ms
+  const int nthreads;
+  std::vector<std::thread> workers;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::atomic<unsigned long long> job{0};
+  std::atomic<int> remaining{0};
+  std::atomic<bool> stop{false};
+  int sleepers = 0;                        // guarded by mu
+  char* dst = nullptr;
+  const char* src = nullptr;
+  size_t bytes = 0;
+};
+
 struct saxs_xdr {
   int fd = -1;
   const unsigned char* data = nullptr;
@@ -654,6 +736,22 @@ int saxs_xdr_read(saxs_xdr* h, const int64_t* frames, int count, float* xyz, flo
 }
 
 const char* saxs_xdr_last_error(void) { return g_error.c_str(); }
+
+int saxs_copier_create(int threads, saxs_copier** out) {
+  if (!out) return fail(SAXS_XDR_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (threads < 1 || threads > 64) return fail(SAXS_XDR_EINVAL, "threads %d outside [1, 64]", threads);
+  *out = new saxs_copier(threads);
+  return SAXS_XDR_OK;
+}
+
+int saxs_copier_copy(saxs_copier* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return fail(SAXS_XDR_EINVAL, "copier/dst/src is NULL");
+  c->copy(static_cast<char*>(dst), static_cast<const char*>(src), bytes);
+  return SAXS_XDR_OK;
+}
+
+void saxs_copier_destroy(saxs_copier* c) { delete c; }
 
 void saxs_xdr_close(saxs_xdr* h) {
   if (!h) return;

@@ -39,7 +39,7 @@ from .formfactors import table
 XDR_OK, XDR_EINVAL, XDR_EIO, XDR_EFORMAT = 0, -1, -2, -3
 XDR_TRR, XDR_XTC = 1, 2
 EXPORTED = ("saxs_xdr_open", "saxs_xdr_info", "saxs_xdr_read", "saxs_xdr_last_error",
-            "saxs_xdr_close")
+            "saxs_xdr_close", "saxs_copier_create", "saxs_copier_copy", "saxs_copier_destroy")
 
 _lib = None
 
@@ -64,6 +64,12 @@ def load():
     lib.saxs_xdr_last_error.restype = C.c_char_p
     lib.saxs_xdr_close.argtypes = [vp]
     lib.saxs_xdr_close.restype = None
+    lib.saxs_copier_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.saxs_copier_create.restype = C.c_int
+    lib.saxs_copier_copy.argtypes = [vp, vp, vp, C.c_size_t]
+    lib.saxs_copier_copy.restype = C.c_int
+    lib.saxs_copier_destroy.argtypes = [vp]
+    lib.saxs_copier_destroy.restype = None
     for name in ("saxs_xdr_open", "saxs_xdr_info", "saxs_xdr_read"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
@@ -402,3 +408,33 @@ class XdrTopology:
     def __repr__(self) -> str:
         return (f"XdrTopology(tpr='{self.tpr_file}', n_atoms={self.n_atoms}, "
                 f"n_frames={self.n_frames})")
+
+
+class HostCopier:
+    """Host threads kept for the copier's life that split one memory copy
+    (``saxs_copier_*`` in the reader library): the run path copies each
+    frame into its pinned staging buffer with it.  ``copy(dst, src)`` takes
+    two C-contiguous numpy arrays of the same byte size and returns when the
+    copy is done."""
+
+    def __init__(self, threads: int = 4):
+        self.lib = load()
+        self._h = C.c_void_p()
+        _check(self.lib.saxs_copier_create(int(threads), C.byref(self._h)))
+        self.threads = int(threads)
+
+    def copy(self, dst: np.ndarray, src: np.ndarray):
+        if dst.nbytes != src.nbytes or not dst.flags.c_contiguous or not src.flags.c_contiguous:
+            raise ValueError("copier needs two C-contiguous arrays of the same byte size")
+        _check(self.lib.saxs_copier_copy(self._h, dst.ctypes.data, src.ctypes.data, dst.nbytes))
+
+    def close(self):
+        if self._h:
+            self.lib.saxs_copier_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

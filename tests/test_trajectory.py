@@ -352,3 +352,30 @@ def test_corrupted_files_under_address_sanitizer(tmp_path):
                              capture_output=True, text=True, timeout=300)
         assert run.returncode == 0, run.stderr[-2000:]
         assert "1500 cases" in run.stdout
+
+
+def test_host_copier_copies_exactly():
+    """The run path's parallel host copy: byte-exact for sizes that are not
+    multiples of the 64-byte split, below the parallel threshold, and over
+    repeated calls (workers spinning between calls and, after a pause,
+    woken from their sleep)."""
+    import time
+    from pycusaxs_amd.trajectory import HostCopier
+    rng = np.random.default_rng(5)
+    for threads in (1, 2, 4, 7):
+        cp = HostCopier(threads)
+        for nbytes in (0, 1, 63, 4096, 65537, 1_200_012, 3 * 100_000 * 4):
+            src = rng.integers(0, 256, size=nbytes, dtype=np.uint8)
+            dst = np.zeros(nbytes, dtype=np.uint8)
+            cp.copy(dst, src)
+            assert np.array_equal(dst, src), (threads, nbytes)
+        src = rng.standard_normal((100_000, 3)).astype(np.float32)
+        for k in range(50):
+            dst = np.empty_like(src)
+            cp.copy(dst, src + k)
+            assert np.array_equal(dst, src + k)
+            if k == 25:
+                time.sleep(0.01)                    # workers go to sleep
+        cp.close()
+    with pytest.raises(ValueError):
+        HostCopier(2).copy(np.zeros(10, np.float32), np.zeros(11, np.float32))
