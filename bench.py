@@ -253,9 +253,9 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simula
     """pycusaxs_cuda.run end to end on the trajectory workload: host frames
     (the 100 frames cycled to 1,000) staged through pinned double buffers and
     a copy stream, element gather on the device, batches of 16 frames.
-    Steady-state rate from runs of 200 and 1,000 frames (the difference
-    removes plan setup and output).  With box_kind, the box changes every
-    frame (changing_boxes)."""
+    Steady-state rate from runs of 200 and 1,000 frames, the best of three
+    of each (the difference removes plan setup and output).  With box_kind,
+    the box changes every frame (changing_boxes)."""
     import tempfile
     import pycusaxs_cuda
     syms = [symbols[i] for i in ids_h]
@@ -280,8 +280,10 @@ def leg_run_path(torch, xyz_h, symbols, ids_h, engine_fps, box_kind=None, simula
 
     run(F - 1)                                              # warm-up
     short = F // 5
-    t_short = run(short - 1)
-    t_full = run(F - 1)
+    # best of three of each: one pair of runs of a few tens of milliseconds
+    # swung by 2x with the host's timing noise
+    t_short = min(run(short - 1) for _ in range(3))
+    t_full = min(run(F - 1) for _ in range(3))
     steady = (F - short) / max(t_full - t_short, 1e-9)
     return {"frames_per_s": round(steady, 1), "frames": F, "wall_s_full": round(t_full, 3),
             "engine_only_frames_per_s": round(engine_fps, 1),
@@ -346,8 +348,10 @@ def leg_run_file(torch, xyz_h, symbols, ids_h, engine_fps, fmt):
 
     run(F - 1)                                              # warm-up
     short = F // 5
-    t_short = run(short - 1)
-    t_full = run(F - 1)
+    # best of three of each: one pair of runs of a few tens of milliseconds
+    # swung by 2x with the host's timing noise
+    t_short = min(run(short - 1) for _ in range(3))
+    t_full = min(run(F - 1) for _ in range(3))
     steady = (F - short) / max(t_full - t_short, 1e-9)
     size = os.path.getsize(path)
     topo.close()
