@@ -1084,7 +1084,8 @@ template <int P, int H>
 __global__ void __launch_bounds__(kWG) k_adj_zg(Params p, Buffers b, AdjBuffers g) {
   constexpr int R1 = 2 * H;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x, f = blockIdx.y;
+  const int2 tf = xcd_tile_frame();
+  const int tile = tf.x, f = tf.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
   const int X0 = bx * p.tx, Y0 = by * p.ty;
   const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
@@ -1262,7 +1263,8 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
   constexpr int R2 = N / 16;
   constexpr int ROUT = HALF ? R2 / 2 : R2;             // stage-1 outputs that can be z < nz
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x, f = blockIdx.y;
+  const int2 tf = xcd_tile_frame();
+  const int tile = tf.x, f = tf.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
   const int X0 = bx * p.tx, Y0 = by * p.ty;
   const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
@@ -1493,14 +1495,18 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
 template <int P>
 __global__ void __launch_bounds__(kWG) k_grad_finalize(Params p, Buffers b, AdjBuffers g) {
   __shared__ int s_eb[kMaxElements + 1];             // element runs of the frame, from f N
-  const int f = blockIdx.y;
+  // a frame's blocks on one XCD: the scattered 12-byte gradient writes fill
+  // whole lines in that XCD's L2 before they reach HBM
+  const unsigned lg = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int f = (int)(lg / gridDim.x);
+  const int blk = (int)(lg - (unsigned)f * gridDim.x);
   const int K = frame_keys(p), T = K / p.E;   // keys per frame, per element
   const int f0 = f * p.N;
   for (int e = threadIdx.x; e <= p.E; e += blockDim.x)
     s_eb[e] = e < p.E ? b.offsets[f * K + e * T] - f0
                       : b.offsets[f * K + K - 1] + b.counts[f * K + K - 1] - f0;
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blk * blockDim.x + threadIdx.x;
   if (i >= p.N) return;
   {
     const int ei = b.elem[i];

@@ -109,6 +109,15 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned lin, unsigned total)
   return c * q + min(c, r) + i;
 }
 
+// (column tile, frame) of a workgroup of a (tiles, frames) grid, a frame's
+// tiles kept on one XCD: neighbouring tiles fetch the same atom records
+// (each record reaches about 2.4 tiles), which then come from that XCD's L2.
+__device__ __forceinline__ int2 xcd_tile_frame() {
+  const unsigned lg = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const unsigned f = lg / gridDim.x;
+  return make_int2((int)(lg - f * gridDim.x), (int)f);
+}
+
 // Bulk-solvent pad of (frame, element) fe in units of 2^-32 atoms per voxel:
 // the border-slab mean (padding::avg, saxsDeviceKernels.cu:446-465) or the
 // caller's constant (padding::given, Options.h:7-11).
@@ -521,8 +530,11 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, i
 
 template <int P>
 __global__ void __launch_bounds__(kWG) k_scatter(Params p, Buffers b) {
-  const int f = blockIdx.y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // a frame's blocks on one XCD: its scattered 16-byte record writes then
+  // fill whole lines in that XCD's L2 before they reach HBM
+  const unsigned lg = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int f = (int)(lg / gridDim.x);
+  const int i = (int)(lg - (unsigned)f * gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= p.N) return;
   const float* x = b.xyz + ((size_t)f * p.N + i) * 3;
   const int2 ks = b.keyslot[(size_t)f * p.N + i];
@@ -944,7 +956,8 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   constexpr int R2 = N / 16;
   constexpr int RIN = HALF ? 8 : 16;                   // stage-0 inputs that can be nonzero
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x, f = blockIdx.y;
+  const int2 tf = xcd_tile_frame();
+  const int tile = tf.x, f = tf.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
   const int X0 = bx * p.tx, Y0 = by * p.ty;
   const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
@@ -1227,7 +1240,8 @@ template <int P, int H>
 __global__ void __launch_bounds__(kWG) k_spread_zg(Params p, Buffers b) {
   constexpr int R1 = 2 * H, YS = R1 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x, f = blockIdx.y;
+  const int2 tf = xcd_tile_frame();
+  const int tile = tf.x, f = tf.y;
   const int bx = tile % p.ntx, by = tile / p.ntx;
   const int X0 = bx * p.tx, Y0 = by * p.ty;
   const int wx = min(p.tx, p.n[0] - X0), wy = min(p.ty, p.n[1] - Y0);
