@@ -191,8 +191,12 @@ __host__ __device__ inline size_t adjx_p_lds_bytes(const Params& p) {
   return (size_t)p.kcx * (17 * (N / 16) + 1) * sizeof(float2) + N * sizeof(float2);
 }
 
-template <int N, bool HALF>
-__global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers g) {
+// LOAD: phase 1 reads A from `amp`, saved by the forward k_xfft_p<N, true,
+// HALF, true> of the same frames (same workgroups, threads and items, so
+// the same values) instead of recomputing it from Y.
+template <int N, bool HALF, bool LOAD = false>
+__global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers g,
+                                                 const float2* amp = nullptr) {
   constexpr int R2 = N / 16;
   constexpr int LY = 17 * R2 + 1;
   constexpr int KL = N / 128;
@@ -227,7 +231,16 @@ __global__ void __launch_bounds__(512) k_adj_x_p(Params p, Buffers b, AdjBuffers
     }
   __syncthreads();
   // phase 1: A at the kept outputs
-  for (int e = 0; e < p.E; ++e) {
+  if constexpr (LOAD) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u)
+      if (u * (int)blockDim.x < kc * 16) {           // the same for the whole workgroup
+#pragma unroll
+        for (int o = 0; o < NO; ++o) A[u][o] = amp[amp_index(blk, u, o, NO)];
+      }
+  }
+  for (int e = 0; e < (LOAD ? 0 : p.E); ++e) {
     const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
     for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
       const int j = item / kc, k = item - j * kc;

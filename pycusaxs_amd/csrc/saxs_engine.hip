@@ -388,6 +388,10 @@ struct saxs_plan {
   bool seg16 = false;          // ... with 16-bit counters (segments <= kSeg16MaxLen atoms)
   int seg_kr = 0;              // keys per k_prep_seg pass (a frame's keys in <= 4 passes)
   bool x_pruned = false;       // x pass and its adjoint by k_xfft_p / k_adj_x_p
+  bool fwd_only = false;       // created with SAXS_FLAG_FORWARD_ONLY
+  float2* xamp = nullptr;      // x-pass amplitudes saved by saxs_forward_frames (k_xfft_p)
+  size_t xamp_elems = 0;
+  bool xamp_valid = false;     // xamp holds the last saxs_forward_frames' amplitudes
   int x_gen_h = 0;             // x pass by k_xfft_g<x_gen_h>, its adjoint by k_adj_x_g (0: Stockham)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
   // saxs_forward_boxes: per-frame cells, q geometries, coefficients and
@@ -556,7 +560,7 @@ void free_plan(saxs_plan* pl) {
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
                   pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp,
-                  pl->gpart, pl->segoff, pl->pad_given};
+                  pl->gpart, pl->segoff, pl->pad_given, pl->xamp};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete pl;
@@ -696,11 +700,33 @@ saxs::Params call_params(saxs_plan* pl, int n_atoms, int n_frames, const float* 
   return p;
 }
 
+// Store of the pruned x pass's amplitudes at the kept outputs, which
+// saxs_forward_frames saves for the x adjoint of a backward that reuses its
+// spectra (k_xfft_p / k_adj_x_p, amp_index layout): sized for the current
+// kept ranges, grown when they grow.
+int ensure_xamp(saxs_plan* pl) {
+  const saxs::Params& p = pl->p;
+  const int Nx = p.nn[0];
+  const size_t bx = (size_t)(((p.kcx * (Nx / 16) + 63) / 64) * 64);
+  const size_t need = (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXItems * (2 * (Nx / 128)) * bx;
+  if (need <= pl->xamp_elems) return SAXS_OK;
+  if (pl->xamp) {
+    HIP_TRY(hipFree(pl->xamp));
+    pl->xamp = nullptr;
+    pl->bytes -= pl->xamp_elems * sizeof(float2);
+    pl->xamp_elems = 0;
+  }
+  const int rc = dev_alloc(&pl->xamp, need, pl->bytes);
+  if (rc) return rc;
+  pl->xamp_elems = need;
+  return SAXS_OK;
+}
+
 // Enqueues the forward pipeline up to the per-block histograms (or, with
 // with_x false, up to the y-pass output for the adjoint).
 int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, int n_atoms,
                         int n_frames, const float* oc_diag, hipStream_t st, bool with_x = true,
-                        const Work* work = nullptr) {
+                        const Work* work = nullptr, float2* amp = nullptr) {
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   const Work W = work ? *work : plan_work(pl);
   saxs::Buffers b = make_buffers(pl, xyz, elem, &W);
@@ -878,8 +904,11 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     const dim3 bx((unsigned)(((p.kcx * (Nx / 16) + 63) / 64) * 64));
 #define SAXS_XP(NN)                                                                          \
   do {                                                                                       \
-    if (xhalf) hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, true>), gx, bx, saxs::xfft_p_lds_bytes<NN>(p), st, p, b); \
-    else hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, false>), gx, bx, saxs::xfft_p_lds_bytes<NN>(p), st, p, b); \
+    const size_t lx = saxs::xfft_p_lds_bytes<NN>(p);                                         \
+    if (amp && xhalf) hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, true, true>), gx, bx, lx, st, p, b, amp); \
+    else if (amp) hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, false, true>), gx, bx, lx, st, p, b, amp); \
+    else if (xhalf) hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, true>), gx, bx, lx, st, p, b, nullptr); \
+    else hipLaunchKernelGGL((saxs::k_xfft_p<NN, true, false>), gx, bx, lx, st, p, b, nullptr); \
   } while (0)
     if (Nx == 128) SAXS_XP(128); else if (Nx == 256) SAXS_XP(256); else SAXS_XP(512);
 #undef SAXS_XP
@@ -983,6 +1012,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   if (tx + d->order - 1 > p.n[0]) tx = p.n[0] - d->order + 1;
   if (ty + d->order - 1 > p.n[1]) ty = (p.n[1] - d->order + 1) & ~1;
   pl->spread_all = (d->flags & SAXS_FLAG_PER_ELEMENT_SPREAD) == 0;
+  pl->fwd_only = (d->flags & SAXS_FLAG_FORWARD_ONLY) != 0;
   pl->pruned_z = (d->flags & SAXS_FLAG_NO_PRUNED_Z) == 0;
   if (d->tile_x <= 0 && d->tile_y <= 0 && tx == 4 && ty == 8) {
     // default tile on long generic columns: 4 x 4 once the generic pruned
@@ -1349,7 +1379,15 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   if (!Hf || !Cf) return fail(SAXS_EINVAL, "Hf/Cf pointer is NULL");
   hipStream_t st = (hipStream_t)stream;
   pl->last_frames_valid = false;
-  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st);
+  pl->xamp_valid = false;
+  // a plan with a backward keeps the x pass's amplitudes for its x adjoint
+  const bool save_amp = pl->x_pruned && !pl->fwd_only && n_atoms > 0;
+  if (save_amp) {
+    rc = ensure_xamp(pl);
+    if (rc) return rc;
+  }
+  rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, true, nullptr,
+                           save_amp ? pl->xamp : nullptr);
   if (rc) return rc;
   const int nblk = pl->p.ky * pl->p.nchx;
   {
@@ -1357,6 +1395,7 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     hipLaunchKernelGGL(saxs::k_hist_reduce, dim3(pl->p.num_bins, n_frames), dim3(saxs::kWG), 0,
                        st, pl->partial, nblk, 1, pl->p.num_bins, Hf, Cf, 1);
   }
+  pl->xamp_valid = save_amp;
   pl->last_frames_valid = true;
   pl->last_xyz = xyz;
   pl->last_N = n_atoms;
@@ -1396,7 +1435,8 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
                    pl->bytes);
     if (rc) return rc;
   }
-  if (flags & SAXS_BWD_REUSE_FORWARD) {
+  const bool reuse = (flags & SAXS_BWD_REUSE_FORWARD) != 0;
+  if (reuse) {
     if (!pl->last_frames_valid || pl->last_xyz != xyz || pl->last_N != n_atoms ||
         pl->last_F != n_frames)
       return fail(SAXS_EINVAL, "no matching saxs_forward_frames state to reuse");
@@ -1404,7 +1444,10 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     rc = launch_forward_core(pl, xyz, elem, n_atoms, n_frames, oc_diag, st, false);
     if (rc) return rc;
   }
+  // the x adjoint reads the forward's saved amplitudes instead of recomputing them
+  const float2* amp = (reuse && pl->xamp_valid && pl->x_pruned) ? pl->xamp : nullptr;
   pl->last_frames_valid = false;           // the adjoint overwrites the spectra
+  pl->xamp_valid = false;
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
   saxs::AdjBuffers g{gH, grad, pl->padg, pl->gpart, pl->nsx, pl->nsy};
@@ -1422,8 +1465,11 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
       const dim3 bx((unsigned)(((p.kcx * (Nx / 16) + 63) / 64) * 64));
 #define SAXS_AXP(NN)                                                                         \
   do {                                                                                       \
-    if (xhalf) hipLaunchKernelGGL((saxs::k_adj_x_p<NN, true>), gx, bx, saxs::adjx_p_lds_bytes<NN>(p), st, p, b, g); \
-    else hipLaunchKernelGGL((saxs::k_adj_x_p<NN, false>), gx, bx, saxs::adjx_p_lds_bytes<NN>(p), st, p, b, g); \
+    const size_t lx = saxs::adjx_p_lds_bytes<NN>(p);                                         \
+    if (amp && xhalf) hipLaunchKernelGGL((saxs::k_adj_x_p<NN, true, true>), gx, bx, lx, st, p, b, g, amp); \
+    else if (amp) hipLaunchKernelGGL((saxs::k_adj_x_p<NN, false, true>), gx, bx, lx, st, p, b, g, amp); \
+    else if (xhalf) hipLaunchKernelGGL((saxs::k_adj_x_p<NN, true>), gx, bx, lx, st, p, b, g, nullptr); \
+    else hipLaunchKernelGGL((saxs::k_adj_x_p<NN, false>), gx, bx, lx, st, p, b, g, nullptr); \
   } while (0)
       if (Nx == 128) SAXS_AXP(128); else if (Nx == 256) SAXS_AXP(256); else SAXS_AXP(512);
 #undef SAXS_AXP

@@ -1796,8 +1796,17 @@ __host__ __device__ inline size_t xfft_p_lds_bytes(const Params& p) {
 
 constexpr int kXItems = 2;       // stage-1 items per thread (kc * 16 <= 512 threads * 2)
 
-template <int N, bool ENDS, bool HALF>
-__global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
+// Entry (u, o) of thread t of x-pass workgroup blk in the saved amplitude
+// array: [blk][u][o][t] (coalesced along t); blk = (f ky + jy) nchx + ch.
+// Only the item rounds u that hold items (u blockDim < kc 16) are stored.
+__device__ __forceinline__ size_t amp_index(int blk, int u, int o, int NO) {
+  return (((size_t)blk * kXItems + u) * NO + o) * blockDim.x + threadIdx.x;
+}
+
+// SAVE: the amplitudes A at the kept outputs also go to `amp` (amp_index),
+// for an x adjoint that follows to read instead of recomputing them.
+template <int N, bool ENDS, bool HALF, bool SAVE = false>
+__global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b, float2* amp = nullptr) {
   constexpr int R2 = N / 16;
   constexpr int LY = 17 * R2 + 1;
   constexpr int NO = ENDS ? 2 * (N / 128) : R2;     // kept outputs per item, at most
@@ -1874,6 +1883,15 @@ __global__ void __launch_bounds__(512) k_xfft_p(Params p, Buffers b) {
       }
     }
     __syncthreads();
+  }
+  if constexpr (SAVE) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u)
+      if (u * (int)blockDim.x < kc * 16) {           // the same for the whole workgroup
+#pragma unroll
+        for (int o = 0; o < NO; ++o) amp[amp_index(blk, u, o, NO)] = A[u][o];
+      }
   }
 #pragma unroll
   for (int u = 0; u < kXItems; ++u) {

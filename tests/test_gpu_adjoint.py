@@ -164,6 +164,37 @@ def test_backward_reuse_matches_recompute(gpu):
     eng.close()
 
 
+def test_backward_reuse_saved_amplitudes(gpu):
+    """On a 128-point pruned x axis the reusing backward's x adjoint reads the
+    amplitudes forward_frames saved instead of recomputing them from the
+    spectra: the gradient matches the recomputing backward's, over several
+    frames, and a second forward_frames/backward pair with other positions
+    uses the new amplitudes."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    L, n = 64.0, 64
+    box = np.diag([L] * 3)
+    base, syms = O.synthetic_system(12000, L, seed=31)
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    assert nn[0] == 128
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.6), box, 12000, 3)
+    idt = torch.from_numpy(ids).cuda()
+    for seed in (0, 1):
+        fr = np.stack([O.frame_positions(base, 40 + seed, f, 0.3) for f in range(3)])
+        xyz = torch.from_numpy(fr).cuda()
+        Hf, _ = eng.forward_frames(xyz, idt)
+        gH = torch.randn(Hf.shape, dtype=torch.float64, device="cuda",
+                         generator=torch.Generator("cuda").manual_seed(seed)) / Hf.abs().clamp_min(1.0)
+        g_ru = eng.backward(xyz, idt, gH, generation=eng.generation)   # saved amplitudes
+        g_re = eng.backward(xyz, idt, gH)                              # recomputed
+        torch.cuda.synchronize()
+        err = ((g_ru - g_re).abs().max() / g_re.abs().max()).item()
+        print(f"[saved amplitudes, seed {seed}] max diff / max |g| = {err:.3e}", flush=True)
+        assert torch.allclose(g_ru, g_re, rtol=1e-6, atol=1e-7 * g_re.abs().max().item())
+    eng.close()
+
+
 def _engine_grad(n, L, pos, syms, qcut, sigma, flags, tile=(4, 4), order=4):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
