@@ -46,17 +46,17 @@ HBM_PEAK_GBS = 8000.0
 # TCC_EA0_RDREQ_128B + 64 B x TCC_EA0_RDREQ_64B, writes = WRITE_SIZE; the same
 # counters give a 512 MiB elementwise copy exactly.  Commands and the
 # per-kernel breakdown: profiles/hbm_traffic_trajectory_100k_128.md.
-MEASURED_TRAFFIC = {"prep": 229.7e6, "scatter": 734.7e6, "spread_z": 1710.1e6, "yfft": 1710.1e6,
-                    "xfft_epi": 401.0e6, "adj_x": 799.4e6, "adj_y": 1828.7e6, "adj_z": 2045.5e6,
-                    "grad_sum": 921.5e6}
+MEASURED_TRAFFIC = {"prep": 229.7e6, "scatter": 440.2e6, "spread_z": 1483.2e6, "yfft": 1710.1e6,
+                    "xfft_epi": 401.0e6, "adj_x": 439.9e6, "adj_y": 1829.0e6, "adj_z": 1794.6e6,
+                    "grad_sum": 773.1e6}
 
 # Wave64 VALU instructions per launch of each kernel for the default workload,
 # from rocprofv3 SQ_INSTS_VALU on MI355X (commands and the per-kernel table:
 # profiles/sq_counters_trajectory_100k_128.md).  A CU issues at most one wave64
 # VALU instruction per clock, so the chip's issue peak is 256 x 2.4e9 per second.
-MEASURED_VALU = {"prep": 35.2e6, "scatter": 12.5e6, "spread_z": 739.9e6, "yfft": 145.8e6,
-                 "xfft_epi": 41.1e6, "adj_x": 77.2e6, "adj_y": 162.5e6, "adj_z": 811.7e6,
-                 "grad_sum": 57.6e6}
+MEASURED_VALU = {"prep": 35.2e6, "scatter": 13.3e6, "spread_z": 739.9e6, "yfft": 145.8e6,
+                 "xfft_epi": 41.1e6, "adj_x": 42.7e6, "adj_y": 162.5e6, "adj_z": 790.1e6,
+                 "grad_sum": 57.3e6}
 VALU_ISSUE_PEAK = 256 * 2.4e9          # wave64 VALU instructions per second
 
 
@@ -419,7 +419,9 @@ def kernel_bytes(eng, frames_per_launch, n_atoms):
         "spread_z": F * (16.0 * n_atoms + grid + zb),     # atoms in, grid, z output out
         "yfft": F * (zb + yb),                            # z output in, y output out
         "xfft_epi": F * yb,                               # y output in (+ the bins)
-        "adj_x": F * 2 * yb,                              # y tiles re-read, adjoint x out
+        # the forward-reusing backward's pruned x adjoint reads the amplitudes
+        # saved by the forward at the kept points instead of the y tiles
+        "adj_x": F * (8.0 * kx * ky * kz + yb),           # amplitudes in, adjoint x out
         "adj_y": F * (yb + zb),                           # adjoint x in, adjoint y out
         "adj_z": F * (zb + grid + 16.0 * n_atoms),        # adjoint y in, potential grid, atoms in
         "grad_sum": F * 12.0 * n_atoms,                   # gradient out

@@ -231,7 +231,11 @@ int saxs_forward_boxes(saxs_plan* plan, const float* xyz, const int32_t* elem, i
 
 /* Per-frame histograms instead of an accumulated one: Hf, Cf are device
  * float64 [n_frames][num_bins] and are overwritten.  Used by the adjoint
- * (per-frame binned intensities). */
+ * (per-frame binned intensities).  On a plan with a backward (no
+ * SAXS_FLAG_FORWARD_ONLY) whose x pass is the pruned power-of-two one, it
+ * also keeps that pass's amplitudes at the kept outputs for a following
+ * SAXS_BWD_REUSE_FORWARD backward (a buffer of the plan, allocated on first
+ * use: at most max_frames x ky x nchx x 2 x (nn_x / 64) x 512 x 8 bytes). */
 int saxs_forward_frames(saxs_plan* plan, const float* xyz, const int32_t* elem,
                         int32_t n_atoms, int32_t n_frames, const float* oc_diag,
                         double* Hf, double* Cf, void* stream);
@@ -246,11 +250,12 @@ int saxs_backward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_
                   void* stream);
 
 /* saxs_backward with options.  SAXS_BWD_REUSE_FORWARD: skip the forward
- * recomputation and use the sorted atoms and y-pass spectra left in the
- * plan's workspace by the immediately preceding saxs_forward_frames call on
- * this plan, which must have had the same positions, elements, n_atoms,
- * n_frames and box (the caller guarantees it; nothing else may run on the
- * plan in between). */
+ * recomputation and use the sorted atoms and y-pass spectra (and the x-pass
+ * amplitudes, where saxs_forward_frames kept them) left in the plan's
+ * workspace by the immediately preceding saxs_forward_frames call on this
+ * plan, which must have had the same positions, elements, n_atoms, n_frames
+ * and box (the caller guarantees it; nothing else may run on the plan in
+ * between). */
 #define SAXS_BWD_REUSE_FORWARD 1
 int saxs_backward_ex(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
                      int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
