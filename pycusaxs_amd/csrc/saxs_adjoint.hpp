@@ -1428,7 +1428,10 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
 #pragma unroll
       for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
-    __syncthreads();                                  // lines complete
+    // lines complete: with R2 = 16 thread t takes line t / 16 in both stages
+    // (a wavefront's own lines; its LDS operations execute in order)
+    if constexpr (R2 == 16) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
     // stage 1 (item = line l, butterfly s): radix-R2 DFT; x = conj(out):
     // column 2l = Re, column 2l+1 = -Im, written over line l
     double part = 0.0;

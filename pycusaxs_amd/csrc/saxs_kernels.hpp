@@ -1088,7 +1088,11 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) dst[r] = v[r];
     }
-    __syncthreads();                                 // lines complete
+    // lines complete: with R2 = 16 thread t takes line t / 16 in both stages,
+    // so a wavefront reads in stage 1 only the lines it wrote in stage 0 (its
+    // LDS operations execute in order); otherwise lines cross wavefronts
+    if constexpr (R2 == 16) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
     // stage 1 (item = line l, butterfly s): kept outputs Z(k), k = s + 16 rr;
     // the two real columns of the line are separated with Z(N - k), which
     // lane (16 - s) mod 16 of the same line holds; then clear line l
