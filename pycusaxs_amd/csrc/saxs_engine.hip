@@ -263,6 +263,24 @@ int yg_half(bool pruned_z, const saxs::Params& p) {
   return 0;
 }
 
+// Half-length H of the generic y transform whose stage 1 forms every output
+// (k_yfft_gf<H, R2>, N = 2H R2 with R2 = 16 or 32), or 0 where it does not
+// apply: what yg_half needs, and most outputs kept (Ky + 1 >= N / 8), where
+// 2H radix-R2 DFTs a line cost less than the direct sums.
+int yf_half(bool pruned_z, const saxs::Params& p, int& R2) {
+  const int Ny = p.nn[1];
+  if (!pruned_z || 2 * p.n[1] > Ny || p.ky >= Ny || 8 * (p.Ky + 1) < Ny) return 0;
+  for (int r2 : {16, 32}) {
+    if (Ny % (2 * r2) != 0) continue;
+    const int H = Ny / (2 * r2);
+    if (H < 5 || H > 10) continue;
+    if (saxs::yfft_gf_lds_bytes(Ny, r2) > kLdsLimit) continue;
+    R2 = r2;
+    return H;
+  }
+  return 0;
+}
+
 // Whether the spreading kernel and the z adjoint that will run for column
 // tile (p.tx, p.ty) fit: the pruned kernels where they apply, else the
 // all-element or per-element Stockham ones (the launch picks in that order).
@@ -393,6 +411,7 @@ struct saxs_plan {
   size_t xamp_elems = 0;
   bool xamp_valid = false;     // xamp holds the last saxs_forward_frames' amplitudes
   int x_gen_h = 0;             // x pass by k_xfft_g<x_gen_h>, its adjoint by k_adj_x_g (0: Stockham)
+  int x_full_h = 0, x_full_r2 = 0;  // forward x pass by k_xfft_gf<x_full_h, x_full_r2> (0: none)
   int nsx = 1, nsy = 1;        // tile offsets one stencil reaches along x, y
   // saxs_forward_boxes: per-frame cells, q geometries, coefficients and
   // tables (allocated on first use), and the pinned staging of the uploads
@@ -465,6 +484,21 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
       if (Nx % (2 * H) != 0 || Nx / (2 * H) > 64) continue;
       if (saxs::xfft_g_lds_bytes(p, 2 * H) > kLdsLimit) continue;
       pl->x_gen_h = H;
+      break;
+    }
+  }
+  // ... whose forward forms every stage-1 output by radix-R2 DFTs where
+  // most outputs are kept (N = 2H R2, R2 = 16 or 32; one item a thread in
+  // each stage over the same z chunks)
+  pl->x_full_h = pl->x_full_r2 = 0;
+  if (pl->x_gen_h && 8 * (p.Kx + 1) >= Nx) {
+    for (int r2 : {16}) {      // radix 32: 32 accumulated outputs a thread exceed the registers
+      if (Nx % (2 * r2) != 0) continue;
+      const int H = Nx / (2 * r2);
+      if (H < 5 || H > 10 || p.kcx * 2 * H > saxs::kWG || p.kcx * r2 > saxs::kWG) continue;
+      if (saxs::xfft_gf_lds_bytes(p, r2) > kLdsLimit) continue;
+      pl->x_full_h = H;
+      pl->x_full_r2 = r2;
       break;
     }
   }
@@ -872,6 +906,25 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
 #undef SAXS_YP
+  } else if (int yr2 = 0; const int yfh = yf_half(pl->pruned_z, p, yr2)) {
+    const int LW = saxs::kWG / yr2;
+    const dim3 gyf((p.n[0] * p.kz + LW - 1) / LW, nfe);
+    const size_t lq = saxs::yfft_gf_lds_bytes(Ny, yr2);
+#define SAXS_YF(HH, RR) hipLaunchKernelGGL((saxs::k_yfft_gf<HH, RR>), gyf, dim3(saxs::kWG), lq, st, p, b)
+#define SAXS_YFH(RR)                                                                   \
+  do {                                                                                 \
+    switch (yfh) {                                                                     \
+      case 10: SAXS_YF(10, RR); break;                                                 \
+      case 9: SAXS_YF(9, RR); break;                                                   \
+      case 8: SAXS_YF(8, RR); break;                                                   \
+      case 7: SAXS_YF(7, RR); break;                                                   \
+      case 6: SAXS_YF(6, RR); break;                                                   \
+      default: SAXS_YF(5, RR); break;                                                  \
+    }                                                                                  \
+  } while (0)
+    if (yr2 == 16) SAXS_YFH(16); else SAXS_YFH(32);
+#undef SAXS_YFH
+#undef SAXS_YF
   } else if (const int yh = yg_half(pl->pruned_z, p)) {
     const int R2 = Ny / (2 * yh), LW = saxs::kWG / R2;
     const dim3 gyg((p.n[0] * p.kz + LW - 1) / LW, nfe);
@@ -912,6 +965,26 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } while (0)
     if (Nx == 128) SAXS_XP(128); else if (Nx == 256) SAXS_XP(256); else SAXS_XP(512);
 #undef SAXS_XP
+  } else if (pl->x_full_h) {
+    const dim3 gx(p.ky, p.nchx, n_frames);
+    const size_t lq = saxs::xfft_gf_lds_bytes(p, pl->x_full_r2);
+    // kept outputs of an item among the KLC first and last of its 16
+    const int klc = p.Kx / (2 * pl->x_full_h) + 1 <= 4 ? 4 : 8;
+#define SAXS_XF(HH, KK) hipLaunchKernelGGL((saxs::k_xfft_gf<HH, 16, KK>), gx, dim3(saxs::kWG), lq, st, p, b)
+#define SAXS_XFH(KK)                                                                   \
+  do {                                                                                 \
+    switch (pl->x_full_h) {                                                            \
+      case 10: SAXS_XF(10, KK); break;                                                 \
+      case 9: SAXS_XF(9, KK); break;                                                   \
+      case 8: SAXS_XF(8, KK); break;                                                   \
+      case 7: SAXS_XF(7, KK); break;                                                   \
+      case 6: SAXS_XF(6, KK); break;                                                   \
+      default: SAXS_XF(5, KK); break;                                                  \
+    }                                                                                  \
+  } while (0)
+    if (klc == 4) SAXS_XFH(4); else SAXS_XFH(8);
+#undef SAXS_XFH
+#undef SAXS_XF
   } else if (pl->x_gen_h) {
     const dim3 gx(p.ky, p.nchx, n_frames);
     const size_t lq = saxs::xfft_g_lds_bytes(p, 2 * pl->x_gen_h);
@@ -1286,11 +1359,13 @@ int saxs_forward_boxes(saxs_plan* pl, const float* xyz, const int32_t* elem, int
   // kept ranges of the batch's key box (restored when the launches are queued)
   const saxs::Params saved = pl->p;
   const bool saved_xp = pl->x_pruned;
-  const int saved_xg = pl->x_gen_h;
+  const int saved_xg = pl->x_gen_h, saved_xfh = pl->x_full_h, saved_xfr = pl->x_full_r2;
   auto restore = [&]() {
     pl->p = saved;
     pl->x_pruned = saved_xp;
     pl->x_gen_h = saved_xg;
+    pl->x_full_h = saved_xfh;
+    pl->x_full_r2 = saved_xfr;
   };
   pl->p.point_keys = fb->key_type;
   for (int a = 0; a < 3; ++a) pl->p.kmax[a] = fb->key_type ? fb->kmax[a] : 0;

@@ -1698,6 +1698,96 @@ __global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
   }
 }
 
+// ---- y transform for N = 2H R2 (R2 = 16 or 32), most outputs kept -------
+//
+// Stage 0 as in k_yfft_g (butterfly j < R2 of a line: two length-H DFTs of
+// the inputs y = j + R2 r, r < H).  Stage 1 forms every output: item
+// (residue s < 2H, line l) takes the radix-R2 DFT over j of T_j[s] W_N^{js},
+// which gives Y(s + 2H r'), r' < R2, and writes the kept ones (m <= Ky or
+// m >= N - Ky).  A line then costs 2H radix-R2 DFTs against Horner's
+// 2 (Ky + 1) R2 complex multiply-adds, fewer once Ky reaches about N / 8
+// (the reference CLI's defaults keep half the axis).  The plan guarantees
+// n_y <= N / 2 and 2 Ky + 1 < N.
+template <int R2>
+__host__ __device__ constexpr int yf_lines() { return kWG / R2; }
+
+__host__ __device__ inline size_t yfft_gf_lds_bytes(int N, int R2) {
+  const int R1 = N / R2;
+  return (size_t)(kWG / R2) * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H, int R2>
+__global__ void __launch_bounds__(kWG) k_yfft_gf(Params p, Buffers b) {
+  static_assert(R2 == 16 || R2 == 32, "radix-R2 stage 1");
+  constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, LW = yf_lines<R2>();
+  constexpr int LY = (R2 * YS) | 1;                  // yg_line_stride(R1, R2)
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tbuf = reinterpret_cast<float2*>(smem);
+  float2* tw = tbuf + LW * LY;                       // W_N^m
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
+  const int g0 = (int)(grp % gridDim.x) * LW;
+  const float inv_kz = 1.0f / (float)kz;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[i];
+  const float2* src = b.zbuf + fe * (size_t)nx * ny * kz;
+  // stage 0 (item = butterfly j, line l; l fastest): LW R2 = kWG items
+  const int j0 = threadIdx.x / LW, l0 = threadIdx.x % LW;
+  float2 xe[H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) xe[r] = make_float2(0.0f, 0.0f);
+  if (g0 + l0 < nlines) {
+    const int g = g0 + l0;
+    const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+    const float2* col = src + (size_t)x * ny * kz + k;
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const int y = j0 + R2 * r;
+      if (y < ny) xe[r] = col[(size_t)y * kz];
+    }
+  }
+  __syncthreads();                                   // twiddles
+  {
+    float2 xo[H];
+    xo[0] = xe[0];
+#pragma unroll
+    for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);   // W_R1^r = W_N^{r R2}
+    Radix<H>::dft(xe);
+    Radix<H>::dft(xo);
+    float2* dst = tbuf + l0 * LY + YS * j0;
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+      dst[2 * t] = xe[t];
+      dst[2 * t + 1] = xo[t];
+    }
+  }
+  __syncthreads();                                   // lines complete
+  // stage 1 (item = residue s, line l; l fastest)
+  const int Ky = p.Ky, ky = p.ky;
+  const size_t row = (size_t)nx * kz;
+  for (int it = threadIdx.x; it < LW * R1; it += blockDim.x) {
+    const int s = it / LW, l = it % LW;
+    const int g = g0 + l;
+    if (g >= nlines) continue;
+    const float2* yl = tbuf + l * LY + s;
+    float2 t[R2];
+    t[0] = yl[0];
+#pragma unroll
+    for (int j = 1; j < R2; ++j) t[j] = cmul(yl[YS * j], tw[j * s]);   // j s < N
+    if constexpr (R2 == 16) dft16<false>(t);
+    else dft32(t);
+    const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+    float2* out = b.ybuf + (fe * ky * nx + x) * kz + k;     // + jy nx kz
+#pragma unroll
+    for (int rr = 0; rr < R2; ++rr) {
+      const int m = s + R1 * rr;
+      if (m <= Ky) out[(size_t)m * row] = t[rr];
+      else if (m >= N - Ky) out[(size_t)(ky - (N - m)) * row] = t[rr];
+    }
+  }
+}
+
 template <int POW2>
 __global__ void __launch_bounds__(kWG) k_xfft_epi(Params p, Buffers b) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2051,6 +2141,135 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
             atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
             atomicAdd(&Cs[bin], origin ? 1u : 2u);
           }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = b.partial + (((size_t)f * p.ky + jy) * p.nchx + ch) * 2 * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    out[i] = Hs[i];
+    out[nb + i] = (double)Cs[i];
+  }
+}
+
+// ---- x transform for N = 2H R2 (R2 = 16), most outputs kept, fused
+// with the form-factor sum, moduli and histogram -----------------------------
+//
+// Same workgroups, result and accounting as k_xfft_g (line = kz value k of
+// the chunk, for the workgroup's jy): stage 0 per element as there; stage 1
+// forms every output, item (residue s < 2H, line k) taking the radix-R2 DFT
+// over j of T_j[s] W_N^{js}, which gives X(s + 2H r'), r' < R2; each thread
+// accumulates A = sum_e f_e(q) F_e at its item's kept outputs (m <= Kx or
+// m >= N - Kx) in registers: with KL = floor(Kx / 2H) + 1 <= KLC those are
+// among r' < KLC and r' >= R2 - KLC, the 2 KLC outputs it keeps (the
+// transform's other outputs are then never formed).  The plan
+// guarantees kcx 2H <= kWG and kcx R2 <= kWG (one item a thread in each
+// stage), n_x <= N / 2 and 2 Kx + 1 < N.
+__host__ __device__ inline size_t xfft_gf_lds_bytes(const Params& p, int R2) {
+  const int N = p.nn[0], R1 = N / R2;
+  return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) +
+         (size_t)p.kcx * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H, int R2, int KLC>
+__global__ void __launch_bounds__(kWG, 3) k_xfft_gf(Params p, Buffers b) {
+  static_assert(R2 == 16, "radix-16 stage 1");
+  static_assert(2 * KLC <= R2, "kept outputs at the ends");
+  constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, NA = 2 * KLC;
+  constexpr int LY = (R2 * YS) | 1;                  // yg_line_stride(R1, R2)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], nb = p.num_bins, Kx = p.Kx;
+  double* Hs = reinterpret_cast<double*>(smem);
+  unsigned int* Cs = reinterpret_cast<unsigned int*>(Hs + nb);
+  float2* lines = reinterpret_cast<float2*>(smem + (((size_t)nb * 12 + 15) & ~size_t(15)));
+  float2* tw = lines + (size_t)p.kcx * LY;           // W_N^m
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    Hs[i] = 0.0;
+    Cs[i] = 0u;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = (p.ky == p.nn[1]) ? jy : (jy <= p.Ky ? jy : p.nn[1] - (p.ky - jy));
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const float inv_kc = 1.0f / (float)kc;
+  // this thread's stage-1 item (residue s, line k; k fastest); the shell key
+  // of its output rr (lut_len where not kept) is formed where it is used
+  const bool act1 = (int)threadIdx.x < kc * R1;
+  const int s1 = act1 ? div_small(threadIdx.x, kc, inv_kc) : 0;
+  const int k1 = act1 ? (int)threadIdx.x - s1 * kc : 0;
+  const int mz = k0 + k1;
+  auto key = [&](int rr) -> int {
+    const int m = s1 + R1 * rr;
+    return (m <= Kx || m >= N - Kx) ? shell_key(m <= Kx ? m : m - N, my, mz, p) : p.lut_len;
+  };
+  auto out_of = [](int o) { return o < KLC ? o : R2 - NA + o; };   // accumulator -> r'
+  float2 A[NA];
+#pragma unroll
+  for (int o = 0; o < NA; ++o) A[o] = make_float2(0.0f, 0.0f);
+  // stage-0 item (butterfly j, line k; k fastest)
+  const bool act0 = (int)threadIdx.x < kc * R2;
+  const int j0 = act0 ? div_small(threadIdx.x, kc, inv_kc) : 0;
+  const int kk0 = act0 ? (int)threadIdx.x - j0 * kc : 0;
+  __syncthreads();                                   // twiddles
+  for (int e = 0; e < p.E; ++e) {
+    if (act0) {
+      const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0 + kk0;
+      float2 xe[H], xo[H];
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int x = j0 + R2 * r;
+        xe[r] = x < nx ? src[(size_t)x * kz] : make_float2(0.0f, 0.0f);
+      }
+      xo[0] = xe[0];
+#pragma unroll
+      for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);   // W_R1^r = W_N^{r R2}
+      Radix<H>::dft(xe);
+      Radix<H>::dft(xo);
+      float2* dst = lines + kk0 * LY + YS * j0;
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        dst[2 * t] = xe[t];
+        dst[2 * t + 1] = xo[t];
+      }
+    }
+    __syncthreads();                                 // lines complete
+    if (act1) {
+      const float* ff = b.lut_ff + ((size_t)f * p.E * b.lut_fstride + (size_t)e * p.lut_len);
+      const float2* yl = lines + k1 * LY + s1;
+      float2 t[R2];
+      t[0] = yl[0];
+#pragma unroll
+      for (int j = 1; j < R2; ++j) t[j] = cmul(yl[YS * j], tw[j * s1]);   // j s < N
+      dft16<false>(t);
+#pragma unroll
+      for (int o = 0; o < NA; ++o) {
+        const int rr = out_of(o);
+        const int nsq = key(rr);
+        if (nsq < p.lut_len) {
+          const float fv = ff[nsq];
+          A[o].x += fv * t[rr].x;
+          A[o].y += fv * t[rr].y;
+        }
+      }
+    }
+    __syncthreads();                                 // lines are rewritten
+  }
+  if (act1) {
+#pragma unroll
+    for (int o = 0; o < NA; ++o) {
+      const int rr = out_of(o);
+      const int nsq = key(rr);
+      if (nsq < p.lut_len) {
+        const int ix = s1 + R1 * rr;
+        const int bin = b.lut_bin[(size_t)f * b.lut_fstride + nsq];
+        if (bin >= 0 && bin < nb) {
+          const float mod = b.modx[ix] * b.mody[iy] * b.modz[mz];
+          const float I = (A[o].x * A[o].x + A[o].y * A[o].y) * mod;
+          const bool origin = (ix == 0) && (iy == 0) && (mz == 0);
+          atomicAdd(&Hs[bin], origin ? (double)I : 2.0 * (double)I);
+          atomicAdd(&Cs[bin], origin ? 1u : 2u);
         }
       }
     }

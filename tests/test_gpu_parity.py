@@ -342,7 +342,12 @@ def test_generic_pruned_z_matches_full_transform(gpu, n, tile):
 @pytest.mark.parametrize("n,qcut", [((48, 70, 64), 0.6),     # y: N 140 = 20 x 7; z 128 radix 16
                                     ((40, 81, 45), 0.5),     # y: 162 = 18 x 9; z: 90 = 18 x 5
                                     ((36, 105, 50), 0.45),   # y: 210 = 14 x 15
-                                    ((48, 70, 64), 2.0)])    # K_y = 33 >= N/8: most outputs kept
+                                    ((48, 70, 64), 2.0),     # K_y = 33 >= N/8: most outputs kept
+                                    # most outputs kept on N = 2H x 16: every stage-1 output
+                                    # formed by radix-16 DFTs (k_yfft_gf)
+                                    ((32, 80, 32), 2.2),     # 160 = 10 x 16, K_y = 21
+                                    ((32, 96, 32), 2.5),     # 192 = 12 x 16, K_y = 28
+                                    ((32, 160, 32), 2.5)])   # 320 = 20 x 16, K_y = 47
 def test_generic_pruned_y_against_oracle(gpu, n, qcut):
     """The output-pruned y transform for N = 2H R2 on non-cubic cells (each
     axis its own transform length), against the oracle."""
@@ -360,7 +365,12 @@ def test_generic_pruned_y_against_oracle(gpu, n, qcut):
 @pytest.mark.parametrize("n,qcut", [((70, 64, 48), 0.6),     # x: N 140 = 14 x 10
                                     ((81, 40, 45), 0.5),     # x: 162 = 18 x 9
                                     ((105, 36, 50), 0.45),   # x: 210 = 14 x 15
-                                    ((70, 64, 48), 2.0)])    # K_x = 33 >= N/8: most outputs kept
+                                    ((70, 64, 48), 2.0),     # K_x = 33 >= N/8: most outputs kept
+                                    # most outputs kept on N = 2H x 16: every stage-1 output
+                                    # formed by radix-16 DFTs (k_xfft_gf)
+                                    ((80, 32, 32), 2.2),     # 160 = 10 x 16, K_x = 21
+                                    ((96, 32, 32), 2.5),     # 192 = 12 x 16, K_x = 28
+                                    ((160, 32, 32), 2.5)])   # 320 = 20 x 16, K_x = 47
 def test_generic_pruned_x_against_oracle(gpu, n, qcut):
     """The output-pruned x transform fused with the form-factor sum, moduli
     and histogram, for N = 2H R2 on non-cubic cells, against the oracle."""
