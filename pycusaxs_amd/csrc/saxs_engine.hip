@@ -487,15 +487,15 @@ int derive_ranges(saxs_plan* pl, int lut_len) {
       break;
     }
   }
-  // ... whose forward forms every stage-1 output by radix-R2 DFTs where
-  // most outputs are kept (N = 2H R2, R2 = 16 or 32; one item a thread in
-  // each stage over the same z chunks)
+  // ... whose forward forms every stage-1 output by radix-16 DFTs where
+  // most outputs are kept (N = 2H 16; one stage-1 item a thread over the
+  // same z chunks)
   pl->x_full_h = pl->x_full_r2 = 0;
   if (pl->x_gen_h && 8 * (p.Kx + 1) >= Nx) {
     for (int r2 : {16}) {      // radix 32: 32 accumulated outputs a thread exceed the registers
       if (Nx % (2 * r2) != 0) continue;
       const int H = Nx / (2 * r2);
-      if (H < 5 || H > 10 || p.kcx * 2 * H > saxs::kWG || p.kcx * r2 > saxs::kWG) continue;
+      if (H < 5 || H > 10 || p.kcx * 2 * H > saxs::kWG) continue;
       if (saxs::xfft_gf_lds_bytes(p, r2) > kLdsLimit) continue;
       pl->x_full_h = H;
       pl->x_full_r2 = r2;

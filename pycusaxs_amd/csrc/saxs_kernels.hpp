@@ -2163,9 +2163,8 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
 // accumulates A = sum_e f_e(q) F_e at its item's kept outputs (m <= Kx or
 // m >= N - Kx) in registers: with KL = floor(Kx / 2H) + 1 <= KLC those are
 // among r' < KLC and r' >= R2 - KLC, the 2 KLC outputs it keeps (the
-// transform's other outputs are then never formed).  The plan
-// guarantees kcx 2H <= kWG and kcx R2 <= kWG (one item a thread in each
-// stage), n_x <= N / 2 and 2 Kx + 1 < N.
+// transform's other outputs are then never formed).  The plan guarantees
+// kcx 2H <= kWG (one stage-1 item a thread), n_x <= N / 2 and 2 Kx + 1 < N.
 __host__ __device__ inline size_t xfft_gf_lds_bytes(const Params& p, int R2) {
   const int N = p.nn[0], R1 = N / R2;
   return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) +
@@ -2208,13 +2207,11 @@ __global__ void __launch_bounds__(kWG, 3) k_xfft_gf(Params p, Buffers b) {
   float2 A[NA];
 #pragma unroll
   for (int o = 0; o < NA; ++o) A[o] = make_float2(0.0f, 0.0f);
-  // stage-0 item (butterfly j, line k; k fastest)
-  const bool act0 = (int)threadIdx.x < kc * R2;
-  const int j0 = act0 ? div_small(threadIdx.x, kc, inv_kc) : 0;
-  const int kk0 = act0 ? (int)threadIdx.x - j0 * kc : 0;
   __syncthreads();                                   // twiddles
   for (int e = 0; e < p.E; ++e) {
-    if (act0) {
+    // stage 0 (item = butterfly j, line k; k fastest)
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j0 = div_small(item, kc, inv_kc), kk0 = item - j0 * kc;
       const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0 + kk0;
       float2 xe[H], xo[H];
 #pragma unroll
