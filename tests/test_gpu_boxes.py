@@ -86,6 +86,62 @@ def test_npt_batch_of_boxes(gpu, kind):
     assert rel.max() <= 1e-5
 
 
+@pytest.mark.parametrize("kind", ["cubic", "triclinic"])
+def test_npt_batch_most_outputs_kept(gpu, kind):
+    """A 160-point FFT grid (80^3 cells) at q <= 3.5 keeps more than an
+    eighth of the y and x axes, where the y and x passes form every stage-1
+    output by radix-16 DFTs (160 = 10 x 16): boxes changing every frame
+    (per-frame device tables) against the per-box path and the oracle."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    n_cell, qcut = 80, 3.5
+    boxes, frames, syms = _frames(kind, 90 + len(kind))
+    boxes, frames = boxes[:3], frames[:3]
+    symbols, ids = O.element_ids(syms)
+    xyz = torch.from_numpy(np.stack(frames))
+    idt = torch.from_numpy(ids)
+    n3, nn = resolve_grid((n_cell,) * 3, 2.0)
+    assert nn == (160, 160, 160)
+
+    def engine(F):
+        return SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, DQ, qcut), boxes[0],
+                          max_atoms=2500, max_frames=F)
+    a = engine(len(frames))
+    a.forward_boxes(xyz, idt, np.stack(boxes))
+    b = engine(1)
+    for f, box in enumerate(boxes):
+        b.forward(xyz[f:f + 1], idt, box=box)
+    torch.cuda.synchronize()
+    kx, ky, _ = b.kept()
+    Ha, Ca = a.H.cpu().numpy(), a.C.cpu().numpy()
+    Hb, Cb = b.H.cpu().numpy(), b.C.cpu().numpy()
+    a.close()
+    b.close()
+    assert 8 * ((kx + 1) // 2) >= 160 and 8 * ((ky + 1) // 2) >= 160, (kx, ky)
+    assert np.array_equal(Ca, Cb)
+    sel = (np.arange(len(Ca)) >= 1) & (Cb > 0)
+    rel = np.abs(Ha[sel] - Hb[sel]) / np.abs(Hb[sel])
+    print(f"[{kind}, 160 points] batch vs per-box path: max rel {rel.max():.2e}", flush=True)
+    assert rel.max() <= 1e-6
+    plan0 = O.make_plan((n_cell,) * 3, boxes[0], 4, 2.0, None, DQ, qcut)
+
+    def make(box):
+        p = O.make_plan((n_cell,) * 3, box, 4, 2.0, None, DQ, qcut)
+        p.hp = O.HistParams(plan0.hp.dq, plan0.hp.qcut, plan0.hp.num_bins, p.hp.spacing)
+        return p
+    Ho = np.zeros(plan0.hp.num_bins)
+    Co = np.zeros(plan0.hp.num_bins)
+    for f, box in zip(frames, boxes):
+        h, c = O.frame_histogram(make(box), f, ids, symbols, box)
+        Ho += h
+        Co += c
+    assert np.array_equal(Ca, Co)
+    rel = np.abs(Ha[sel] / Ca[sel] - Ho[sel] / Co[sel]) / np.abs(Ho[sel] / Co[sel])
+    print(f"[{kind}, 160 points] batch vs oracle: max rel {rel.max():.2e} over {sel.sum()} bins",
+          flush=True)
+    assert rel.max() <= 1e-5
+
+
 @pytest.mark.parametrize("kind", ["cubic", "orthorhombic", "mixed"])
 def test_nvt_batch_of_boxes(gpu, kind):
     """NVT: every frame binned with frame 0's q (its own cut and form
