@@ -984,6 +984,167 @@ __global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers
   }
 }
 
+// ---- x adjoint for N = 2H x 16, most outputs kept: transpose of k_xfft_gf ----
+//
+// Same workgroups, threads and stage-1 items (residue s, line k) as
+// k_xfft_gf.  Phase 1: A at the item's kept outputs, read from where the
+// forward k_xfft_gf<..., SAVE> put it (LOAD: the same values) or recomputed
+// by its stages; Abar = 2 mod w gH_b A (k_adj_x).  Phase 2 per element: item
+// (s, k) puts v(r') = conj(f_e Abar) at its kept outputs, takes the radix-16
+// DFT over r' and the factor W_N^{js}, Tbar_j[s] = sum_{M = s mod 2H} v(M)
+// W_N^{jM} (into LDS); butterfly (j, k) then forms
+// xbar(j + R2 r) = sum_s Tbar_j[s] W_2H^{rs}, r < H, by two length-H DFTs of
+// the even and odd s, and writes Ybar(x) = conj(xbar) for x < n_x over Y_e.
+__host__ __device__ inline size_t adj_x_gf_lds_bytes(const Params& p, int R2) {
+  const int N = p.nn[0], R1 = N / R2;
+  return (size_t)p.kcx * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+}
+
+template <int H, int R2, int KLC, bool LOAD>
+__global__ void __launch_bounds__(kWG, 3) k_adj_x_gf(Params p, Buffers b, AdjBuffers g,
+                                                     const float2* amp) {
+  static_assert(R2 == 16, "radix-16 stage 1");
+  static_assert(2 * KLC <= R2, "kept outputs at the ends");
+  constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, NA = 2 * KLC;
+  constexpr int LY = (R2 * YS) | 1;                  // yg_line_stride(R1, R2)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
+  const int kz = p.kz, k0 = ch * p.kcx, kc = min(p.kcx, kz - k0);
+  const int nx = p.n[0], nb = p.num_bins, Kx = p.Kx;
+  float2* lines = reinterpret_cast<float2*>(smem);
+  float2* tw = lines + (size_t)p.kcx * LY;           // W_N^m
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twx[i];
+  const int iy = kept_to_index(jy, p.ky, p.Ky, p.nn[1]);
+  const int my = iy < p.nf[1] ? iy : iy - p.nn[1];
+  const float inv_kc = 1.0f / (float)kc;
+  const bool act1 = (int)threadIdx.x < kc * R1;
+  const int s1 = act1 ? div_small(threadIdx.x, kc, inv_kc) : 0;
+  const int k1 = act1 ? (int)threadIdx.x - s1 * kc : 0;
+  const int mz = k0 + k1;
+  auto key = [&](int rr) -> int {
+    const int m = s1 + R1 * rr;
+    return (m <= Kx || m >= N - Kx) ? shell_key(m <= Kx ? m : m - N, my, mz, p) : p.lut_len;
+  };
+  auto out_of = [](int o) { return o < KLC ? o : R2 - NA + o; };   // accumulator -> r'
+  float2 A[NA];
+#pragma unroll
+  for (int o = 0; o < NA; ++o) A[o] = make_float2(0.0f, 0.0f);
+  __syncthreads();                                   // twiddles
+  if constexpr (LOAD) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int o = 0; o < NA; ++o) A[o] = amp[amp_gf_index(blk, o, NA)];
+  } else {
+    for (int e = 0; e < p.E; ++e) {                  // k_xfft_gf's stages
+      for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+        const int j0 = div_small(item, kc, inv_kc), kk0 = item - j0 * kc;
+        const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0 + kk0;
+        float2 xe[H], xo[H];
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+          const int x = j0 + R2 * r;
+          xe[r] = x < nx ? src[(size_t)x * kz] : make_float2(0.0f, 0.0f);
+        }
+        xo[0] = xe[0];
+#pragma unroll
+        for (int r = 1; r < H; ++r) xo[r] = cmul(xe[r], tw[r * R2]);
+        Radix<H>::dft(xe);
+        Radix<H>::dft(xo);
+        float2* dst = lines + kk0 * LY + YS * j0;
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+          dst[2 * t] = xe[t];
+          dst[2 * t + 1] = xo[t];
+        }
+      }
+      __syncthreads();
+      if (act1) {
+        const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+        const float2* yl = lines + k1 * LY + s1;
+        float2 t[R2];
+        t[0] = yl[0];
+#pragma unroll
+        for (int j = 1; j < R2; ++j) t[j] = cmul(yl[YS * j], tw[j * s1]);
+        dft16<false>(t);
+#pragma unroll
+        for (int o = 0; o < NA; ++o) {
+          const int rr = out_of(o);
+          const int nsq = key(rr);
+          if (nsq < p.lut_len) {
+            const float fv = ff[nsq];
+            A[o].x += fv * t[rr].x;
+            A[o].y += fv * t[rr].y;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Abar = 2 mod w gH_b A at in-range points, 0 elsewhere
+#pragma unroll
+  for (int o = 0; o < NA; ++o) {
+    const int rr = out_of(o);
+    const int nsq = act1 ? key(rr) : p.lut_len;
+    float2 ab = make_float2(0.0f, 0.0f);
+    if (nsq < p.lut_len) {
+      const int bin = b.lut_bin[nsq];
+      if (bin >= 0 && bin < nb) {
+        const int ix = s1 + R1 * rr;
+        const float mod = b.modx[ix] * b.mody[iy] * b.modz[mz];
+        const bool origin = (ix == 0) && (iy == 0) && (mz == 0);
+        const float sc = 2.0f * mod * (origin ? 1.0f : 2.0f) * (float)g.gH[(size_t)f * nb + bin];
+        ab = make_float2(sc * A[o].x, sc * A[o].y);
+      }
+    }
+    A[o] = ab;
+  }
+  // phase 2: per element, Ybar_e over Y_e
+  for (int e = 0; e < p.E; ++e) {
+    if (act1) {
+      const float* ff = b.lut_ff + (size_t)e * p.lut_len;
+      float2 t[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) t[r] = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int o = 0; o < NA; ++o) {
+        const int rr = out_of(o);
+        const int nsq = key(rr);
+        if (nsq < p.lut_len) {
+          const float fv = ff[nsq];
+          t[rr] = make_float2(fv * A[o].x, -fv * A[o].y);   // conj(f Abar)
+        }
+      }
+      dft16<false>(t);
+      float2* dst = lines + k1 * LY + s1;
+      dst[0] = t[0];
+#pragma unroll
+      for (int j = 1; j < R2; ++j) dst[YS * j] = cmul(t[j], tw[j * s1]);   // j s < N
+    }
+    __syncthreads();
+    float2* ydst = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
+    for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
+      const int j = div_small(item, kc, inv_kc), k = item - j * kc;
+      const float2* yl = lines + k * LY + YS * j;
+      float2 ev[H], od[H];
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        ev[t] = yl[2 * t];
+        od[t] = yl[2 * t + 1];
+      }
+      Radix<H>::dft(ev);
+      Radix<H>::dft(od);
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int x = j + R2 * r;
+        if (x >= nx) continue;
+        const float2 o = r == 0 ? cadd(ev[0], od[0]) : cadd(ev[r], cmul(od[r], tw[r * R2]));
+        ydst[(size_t)x * kz + k] = make_float2(o.x, -o.y);
+      }
+    }
+    __syncthreads();                                 // lines rewritten next element
+  }
+}
+
 // Adjoint of spread_cols: for each staged atom, the in-tile part of
 // sum_stencil drho~ grad(th_x th_y th_z), with drho~ in columns of stride cs
 // whose z halo repeats z < P - 1 after nz.  The stencil is unrolled and

@@ -734,15 +734,18 @@ saxs::Params call_params(saxs_plan* pl, int n_atoms, int n_frames, const float* 
   return p;
 }
 
-// Store of the pruned x pass's amplitudes at the kept outputs, which
+// Store of the x pass's amplitudes at the kept outputs, which
 // saxs_forward_frames saves for the x adjoint of a backward that reuses its
-// spectra (k_xfft_p / k_adj_x_p, amp_index layout): sized for the current
-// kept ranges, grown when they grow.
+// spectra (k_xfft_p / k_adj_x_p, amp_index layout, or k_xfft_gf /
+// k_adj_x_gf, amp_gf_index layout): sized for the current kept ranges,
+// grown when they grow.
 int ensure_xamp(saxs_plan* pl) {
   const saxs::Params& p = pl->p;
   const int Nx = p.nn[0];
   const size_t bx = (size_t)(((p.kcx * (Nx / 16) + 63) / 64) * 64);
-  const size_t need = (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXItems * (2 * (Nx / 128)) * bx;
+  const size_t need = pl->x_pruned
+      ? (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXItems * (2 * (Nx / 128)) * bx
+      : (size_t)pl->max_frames * p.ky * p.nchx * 16 * saxs::kWG;   // k_xfft_gf: <= 16 a thread
   if (need <= pl->xamp_elems) return SAXS_OK;
   if (pl->xamp) {
     HIP_TRY(hipFree(pl->xamp));
@@ -970,7 +973,11 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     const size_t lq = saxs::xfft_gf_lds_bytes(p, pl->x_full_r2);
     // kept outputs of an item among the KLC first and last of its 16
     const int klc = p.Kx / (2 * pl->x_full_h) + 1 <= 4 ? 4 : 8;
-#define SAXS_XF(HH, KK) hipLaunchKernelGGL((saxs::k_xfft_gf<HH, 16, KK>), gx, dim3(saxs::kWG), lq, st, p, b)
+#define SAXS_XF(HH, KK)                                                                \
+  do {                                                                                 \
+    if (amp) hipLaunchKernelGGL((saxs::k_xfft_gf<HH, 16, KK, true>), gx, dim3(saxs::kWG), lq, st, p, b, amp); \
+    else hipLaunchKernelGGL((saxs::k_xfft_gf<HH, 16, KK>), gx, dim3(saxs::kWG), lq, st, p, b, nullptr); \
+  } while (0)
 #define SAXS_XFH(KK)                                                                   \
   do {                                                                                 \
     switch (pl->x_full_h) {                                                            \
@@ -1456,7 +1463,7 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   pl->last_frames_valid = false;
   pl->xamp_valid = false;
   // a plan with a backward keeps the x pass's amplitudes for its x adjoint
-  const bool save_amp = pl->x_pruned && !pl->fwd_only && n_atoms > 0;
+  const bool save_amp = (pl->x_pruned || pl->x_full_h) && !pl->fwd_only && n_atoms > 0;
   if (save_amp) {
     rc = ensure_xamp(pl);
     if (rc) return rc;
@@ -1520,7 +1527,7 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     if (rc) return rc;
   }
   // the x adjoint reads the forward's saved amplitudes instead of recomputing them
-  const float2* amp = (reuse && pl->xamp_valid && pl->x_pruned) ? pl->xamp : nullptr;
+  const float2* amp = (reuse && pl->xamp_valid && (pl->x_pruned || pl->x_full_h)) ? pl->xamp : nullptr;
   pl->last_frames_valid = false;           // the adjoint overwrites the spectra
   pl->xamp_valid = false;
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
@@ -1548,6 +1555,28 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } while (0)
       if (Nx == 128) SAXS_AXP(128); else if (Nx == 256) SAXS_AXP(256); else SAXS_AXP(512);
 #undef SAXS_AXP
+    } else if (pl->x_full_h) {
+      const size_t lq = saxs::adj_x_gf_lds_bytes(p, 16);
+      const int klc = p.Kx / (2 * pl->x_full_h) + 1 <= 4 ? 4 : 8;
+#define SAXS_AXF(HH, KK)                                                               \
+  do {                                                                                 \
+    if (amp) hipLaunchKernelGGL((saxs::k_adj_x_gf<HH, 16, KK, true>), gx, dim3(saxs::kWG), lq, st, p, b, g, amp); \
+    else hipLaunchKernelGGL((saxs::k_adj_x_gf<HH, 16, KK, false>), gx, dim3(saxs::kWG), lq, st, p, b, g, nullptr); \
+  } while (0)
+#define SAXS_AXFH(KK)                                                                  \
+  do {                                                                                 \
+    switch (pl->x_full_h) {                                                            \
+      case 10: SAXS_AXF(10, KK); break;                                                \
+      case 9: SAXS_AXF(9, KK); break;                                                  \
+      case 8: SAXS_AXF(8, KK); break;                                                  \
+      case 7: SAXS_AXF(7, KK); break;                                                  \
+      case 6: SAXS_AXF(6, KK); break;                                                  \
+      default: SAXS_AXF(5, KK); break;                                                 \
+    }                                                                                  \
+  } while (0)
+      if (klc == 4) SAXS_AXFH(4); else SAXS_AXFH(8);
+#undef SAXS_AXFH
+#undef SAXS_AXF
     } else if (pl->x_gen_h) {
       const size_t lq = saxs::adj_x_g_lds_bytes(p, 2 * pl->x_gen_h);
 #define SAXS_AXG(HH) hipLaunchKernelGGL(saxs::k_adj_x_g<HH>, gx, dim3(saxs::kWG), lq, st, p, b, g)

@@ -2165,14 +2165,20 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
 // among r' < KLC and r' >= R2 - KLC, the 2 KLC outputs it keeps (the
 // transform's other outputs are then never formed).  The plan guarantees
 // kcx 2H <= kWG (one stage-1 item a thread), n_x <= N / 2 and 2 Kx + 1 < N.
+// SAVE: the accumulators also go to `amp` ([workgroup][accumulator][thread],
+// amp_gf_index) for the x adjoint of a backward that follows.
 __host__ __device__ inline size_t xfft_gf_lds_bytes(const Params& p, int R2) {
   const int N = p.nn[0], R1 = N / R2;
   return (((size_t)p.num_bins * 12 + 15) & ~size_t(15)) +
          (size_t)p.kcx * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
 }
 
-template <int H, int R2, int KLC>
-__global__ void __launch_bounds__(kWG, 3) k_xfft_gf(Params p, Buffers b) {
+__device__ __forceinline__ size_t amp_gf_index(int blk, int o, int NA) {
+  return ((size_t)blk * NA + o) * blockDim.x + threadIdx.x;
+}
+
+template <int H, int R2, int KLC, bool SAVE = false>
+__global__ void __launch_bounds__(kWG, 3) k_xfft_gf(Params p, Buffers b, float2* amp = nullptr) {
   static_assert(R2 == 16, "radix-16 stage 1");
   static_assert(2 * KLC <= R2, "kept outputs at the ends");
   constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, NA = 2 * KLC;
@@ -2252,6 +2258,11 @@ __global__ void __launch_bounds__(kWG, 3) k_xfft_gf(Params p, Buffers b) {
       }
     }
     __syncthreads();                                 // lines are rewritten
+  }
+  if constexpr (SAVE) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int o = 0; o < NA; ++o) amp[amp_gf_index(blk, o, NA)] = A[o];
   }
   if (act1) {
 #pragma unroll

@@ -164,21 +164,23 @@ def test_backward_reuse_matches_recompute(gpu):
     eng.close()
 
 
-def test_backward_reuse_saved_amplitudes(gpu):
-    """On a 128-point pruned x axis the reusing backward's x adjoint reads the
-    amplitudes forward_frames saved instead of recomputing them from the
-    spectra: the gradient matches the recomputing backward's, over several
+@pytest.mark.parametrize("L,n,qcut", [(64.0, 64, 0.6),     # 128-point pruned x axis (k_xfft_p)
+                                      (60.0, 80, 2.2)])    # 160 = 10 x 16, most kept (k_xfft_gf)
+def test_backward_reuse_saved_amplitudes(gpu, L, n, qcut):
+    """The reusing backward's x adjoint reads the amplitudes forward_frames
+    saved instead of recomputing them from the spectra (on a 128-point
+    pruned x axis, and on a 160-point axis whose x pass forms every stage-1
+    output): the gradient matches the recomputing backward's, over several
     frames, and a second forward_frames/backward pair with other positions
     uses the new amplitudes."""
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
-    L, n = 64.0, 64
     box = np.diag([L] * 3)
     base, syms = O.synthetic_system(12000, L, seed=31)
     symbols, ids = O.element_ids(syms)
     n3, nn = resolve_grid((n,) * 3, 2.0)
-    assert nn[0] == 128
-    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.6), box, 12000, 3)
+    assert nn[0] == 2 * n
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, qcut), box, 12000, 3)
     idt = torch.from_numpy(ids).cuda()
     for seed in (0, 1):
         fr = np.stack([O.frame_positions(base, 40 + seed, f, 0.3) for f in range(3)])
@@ -296,7 +298,10 @@ def test_adjoint_generic_pruned_z_against_oracle(gpu, n, L, order, qcut):
 
 
 @pytest.mark.parametrize("n,L,qcut", [(36, 27.0, 1.6),    # N 72 = 12 x 6: K = 13 >= N/8
-                                      (40, 30.0, 1.5)])   # N 80 = 16 x 5: K = 14 >= N/8
+                                      (40, 30.0, 1.5),    # N 80 = 16 x 5: K = 14 >= N/8
+                                      # N 160 = 10 x 16, K = 21: every stage-1 output
+                                      # formed (k_xfft_gf and its adjoint k_adj_x_gf)
+                                      (80, 60.0, 2.2)])
 def test_adjoint_generic_y_x_many_kept_against_oracle(gpu, n, L, qcut):
     """The generic y and x transforms and their adjoints where most outputs
     are kept (N/8 <= K < N/2, the q range of the reference CLI's defaults),
