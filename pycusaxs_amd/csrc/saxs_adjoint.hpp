@@ -984,6 +984,93 @@ __global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers
   }
 }
 
+// ---- y adjoint for N = 2H x 16, most outputs kept: transpose of k_yfft_gf ----
+//
+// yf_adj_lines consecutive (x, kz) lines a workgroup, as k_yfft_gf's.
+// Item (residue s < 2H, line l) reads v(M) = conj(Ybar(M)) at its kept
+// outputs M = s + 2H r' (M <= Ky or M >= N - Ky), takes the radix-16 DFT
+// over r' and the factor W_N^{js}: Tbar_j[s] = sum_{M = s mod 2H} v(M)
+// W_N^{jM} (into LDS); butterfly (j, l) then forms
+// xbar(j + 16 r) = sum_s Tbar_j[s] W_2H^{rs}, r < H, by two length-H DFTs of
+// the even and odd s, and writes Zbar(y) = conj(xbar) for y < n_y.
+__host__ __device__ inline size_t adj_y_gf_lds_bytes(int N) {
+  return (size_t)yf_adj_lines(N / 16, 16) * yg_line_stride(N / 16, 16) * sizeof(float2) +
+         (size_t)N * sizeof(float2);
+}
+
+template <int H>
+__global__ void __launch_bounds__(kWG) k_adj_y_gf(Params p, Buffers b) {
+  constexpr int R2 = 16, R1 = 2 * H, YS = R1 + 1, N = R1 * R2, LW = yf_adj_lines(R1, R2);
+  constexpr int LY = (R2 * YS) | 1;                  // yg_line_stride(R1, R2)
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tbuf = reinterpret_cast<float2*>(smem);
+  float2* tw = tbuf + LW * LY;                       // W_N^m
+  const int nx = p.n[0], ny = p.n[1], kz = p.kz;
+  const int nlines = nx * kz;
+  const unsigned grp = xcd_contiguous(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const size_t fe = grp / gridDim.x;
+  const int g0 = (int)(grp % gridDim.x) * LW;
+  const float inv_kz = 1.0f / (float)kz;
+  const int Ky = p.Ky, ky = p.ky;
+  const size_t row = (size_t)nx * kz;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[i];
+  __syncthreads();                                   // twiddles
+  // transposed stage 1 (item = residue s, line l; l fastest)
+  for (int it = threadIdx.x; it < LW * R1; it += blockDim.x) {
+    const int s = it / LW, l = it % LW;
+    const int g = g0 + l;
+    float2 t[R2];
+#pragma unroll
+    for (int rr = 0; rr < R2; ++rr) t[rr] = make_float2(0.0f, 0.0f);
+    if (g < nlines) {
+      const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+      const float2* in = b.ybuf + (fe * ky * nx + x) * kz + k;   // + jy nx kz
+#pragma unroll
+      for (int rr = 0; rr < R2; ++rr) {
+        const int m = s + R1 * rr;
+        int jy = -1;
+        if (m <= Ky) jy = m;
+        else if (m >= N - Ky) jy = ky - (N - m);
+        if (jy >= 0) {
+          const float2 v = in[(size_t)jy * row];
+          t[rr] = make_float2(v.x, -v.y);
+        }
+      }
+    }
+    dft16<false>(t);
+    float2* dst = tbuf + l * LY + s;
+    dst[0] = t[0];
+#pragma unroll
+    for (int j = 1; j < R2; ++j) dst[YS * j] = cmul(t[j], tw[j * s]);   // j s < N
+  }
+  __syncthreads();
+  // transposed stage 0 (item = butterfly j, line l; l fastest): LW R2 <= kWG items
+  {
+    const int j = threadIdx.x / LW, l = threadIdx.x % LW;
+    const int g = g0 + l;
+    if (j < R2 && g < nlines) {
+      const float2* yl = tbuf + l * LY + YS * j;
+      float2 ev[H], od[H];
+#pragma unroll
+      for (int t = 0; t < H; ++t) {
+        ev[t] = yl[2 * t];
+        od[t] = yl[2 * t + 1];
+      }
+      Radix<H>::dft(ev);
+      Radix<H>::dft(od);
+      const int x = div_small(g, kz, inv_kz), k = g - x * kz;
+      float2* dst = b.zbuf + ((fe * nx + x) * (size_t)ny) * kz + k;
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        const int y = j + R2 * r;
+        if (y >= ny) continue;
+        const float2 o = r == 0 ? cadd(ev[0], od[0]) : cadd(ev[r], cmul(od[r], tw[r * R2]));
+        dst[(size_t)y * kz] = make_float2(o.x, -o.y);
+      }
+    }
+  }
+}
+
 // ---- x adjoint for N = 2H x 16, most outputs kept: transpose of k_xfft_gf ----
 //
 // Same workgroups, threads and stage-1 items (residue s, line k) as

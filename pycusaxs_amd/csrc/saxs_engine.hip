@@ -910,7 +910,7 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
     if (Ny == 128) SAXS_YP(128); else if (Ny == 256) SAXS_YP(256); else SAXS_YP(512);
 #undef SAXS_YP
   } else if (int yr2 = 0; const int yfh = yf_half(pl->pruned_z, p, yr2)) {
-    const int LW = saxs::kWG / yr2;
+    const int LW = saxs::yf_lines(Ny / yr2, yr2);
     const dim3 gyf((p.n[0] * p.kz + LW - 1) / LW, nfe);
     const size_t lq = saxs::yfft_gf_lds_bytes(Ny, yr2);
 #define SAXS_YF(HH, RR) hipLaunchKernelGGL((saxs::k_yfft_gf<HH, RR>), gyf, dim3(saxs::kWG), lq, st, p, b)
@@ -1616,6 +1616,22 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   } while (0)
     if (Ny == 128) SAXS_AYP(128); else if (Ny == 256) SAXS_AYP(256); else SAXS_AYP(512);
 #undef SAXS_AYP
+  } else if (int r2f = 0, hf = yf_half(pl->pruned_z, p, r2f); hf && r2f == 16) {
+    // the transpose of the full-output y pass (N = 2H x 16, most outputs kept)
+    Probe pr(pl, SAXS_K_ADJ_Y, st);
+    const int LW = saxs::yf_adj_lines(Ny / 16, 16);
+    const dim3 gay((p.n[0] * p.kz + LW - 1) / LW, (unsigned)(n_frames * p.E));
+    const size_t lq = saxs::adj_y_gf_lds_bytes(Ny);
+#define SAXS_AYF(HH) hipLaunchKernelGGL(saxs::k_adj_y_gf<HH>, gay, dim3(saxs::kWG), lq, st, p, b)
+    switch (hf) {
+      case 10: SAXS_AYF(10); break;
+      case 9: SAXS_AYF(9); break;
+      case 8: SAXS_AYF(8); break;
+      case 7: SAXS_AYF(7); break;
+      case 6: SAXS_AYF(6); break;
+      default: SAXS_AYF(5); break;
+    }
+#undef SAXS_AYF
   } else if (const int yh = yg_half(pl->pruned_z, p)) {
     Probe pr(pl, SAXS_K_ADJ_Y, st);
     const int R2 = Ny / (2 * yh), LW = saxs::kWG / R2;

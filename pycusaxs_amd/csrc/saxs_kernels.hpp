@@ -1706,20 +1706,28 @@ __global__ void __launch_bounds__(kWG) k_yfft_g(Params p, Buffers b) {
 // which gives Y(s + 2H r'), r' < R2, and writes the kept ones (m <= Ky or
 // m >= N - Ky).  A line then costs 2H radix-R2 DFTs against Horner's
 // 2 (Ky + 1) R2 complex multiply-adds, fewer once Ky reaches about N / 8
-// (the reference CLI's defaults keep half the axis).  The plan guarantees
-// n_y <= N / 2 and 2 Ky + 1 < N.
-template <int R2>
-__host__ __device__ constexpr int yf_lines() { return kWG / R2; }
+// (the reference CLI's defaults keep half the axis).  A workgroup takes
+// yf_lines consecutive lines.  The plan guarantees n_y <= N / 2 and
+// 2 Ky + 1 < N.
+// Lines a workgroup of the full-output y pass takes (one stage-0 item a
+// thread; stage 1 then takes two rounds where 2H > R2), and of its adjoint
+// (one item a thread in both stages: with the global reads in its first
+// stage a second round would expose their latency again).
+__host__ __device__ constexpr int yf_lines(int R1, int R2) { return kWG / R2; }
+__host__ __device__ constexpr int yf_adj_lines(int R1, int R2) {
+  return kWG / (R1 > R2 ? R1 : R2);
+}
 
 __host__ __device__ inline size_t yfft_gf_lds_bytes(int N, int R2) {
   const int R1 = N / R2;
-  return (size_t)(kWG / R2) * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
+  return (size_t)yf_lines(R1, R2) * yg_line_stride(R1, R2) * sizeof(float2) +
+         (size_t)N * sizeof(float2);
 }
 
 template <int H, int R2>
 __global__ void __launch_bounds__(kWG) k_yfft_gf(Params p, Buffers b) {
   static_assert(R2 == 16 || R2 == 32, "radix-R2 stage 1");
-  constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, LW = yf_lines<R2>();
+  constexpr int R1 = 2 * H, YS = R1 + 1, N = R1 * R2, LW = yf_lines(R1, R2);
   constexpr int LY = (R2 * YS) | 1;                  // yg_line_stride(R1, R2)
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tbuf = reinterpret_cast<float2*>(smem);
@@ -1732,12 +1740,13 @@ __global__ void __launch_bounds__(kWG) k_yfft_gf(Params p, Buffers b) {
   const float inv_kz = 1.0f / (float)kz;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = b.twy[i];
   const float2* src = b.zbuf + fe * (size_t)nx * ny * kz;
-  // stage 0 (item = butterfly j, line l; l fastest): LW R2 = kWG items
+  // stage 0 (item = butterfly j, line l; l fastest): LW R2 <= kWG items
   const int j0 = threadIdx.x / LW, l0 = threadIdx.x % LW;
+  const bool act0 = j0 < R2;
   float2 xe[H];
 #pragma unroll
   for (int r = 0; r < H; ++r) xe[r] = make_float2(0.0f, 0.0f);
-  if (g0 + l0 < nlines) {
+  if (act0 && g0 + l0 < nlines) {
     const int g = g0 + l0;
     const int x = div_small(g, kz, inv_kz), k = g - x * kz;
     const float2* col = src + (size_t)x * ny * kz + k;
@@ -1748,7 +1757,7 @@ __global__ void __launch_bounds__(kWG) k_yfft_gf(Params p, Buffers b) {
     }
   }
   __syncthreads();                                   // twiddles
-  {
+  if (act0) {
     float2 xo[H];
     xo[0] = xe[0];
 #pragma unroll
