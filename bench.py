@@ -388,7 +388,20 @@ def leg_reference_defaults(torch, xyz_h, symbols, ids_h, steps):
     torch.cuda.synchronize()
     kt = {k: round(v[0] / v[1], 4) for k, v in eng.kernel_times().items()}
     eng.set_timing(False)
-    out = {"engine_frames_per_s": round(fps, 1), "kept": list(eng.kept()), "fft_grid": nn[0],
+    # forward + backward (the backward reusing the forward's spectra), as a
+    # restraint at these defaults would run it
+    Hf, _ = eng.forward_frames(xyz, ids)
+    gH = torch.ones_like(Hf) / Hf.abs().clamp_min(1.0)
+    eng.backward(xyz, ids, gH, generation=eng.generation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.forward_frames(xyz, ids)
+        eng.backward(xyz, ids, gH, generation=eng.generation)
+    torch.cuda.synchronize()
+    fb = steps * F / (time.perf_counter() - t0)
+    out = {"engine_frames_per_s": round(fps, 1), "fwd_bwd_frames_per_s": round(fb, 1),
+           "kept": list(eng.kept()), "fft_grid": nn[0],
            "kernel_ms_per_50_frames": kt,
            "workload": "reference CLI defaults: 100,000 atoms, 100 A, grid 128, scale 2.5 "
                        "(320^3 FFT), bin 0.01 (reset to 0.02513), qcut 2.0, 80 bins"}
