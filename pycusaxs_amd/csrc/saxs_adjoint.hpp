@@ -799,8 +799,11 @@ __host__ __device__ inline size_t adj_x_g_lds_bytes(const Params& p, int R1) {
   return (size_t)p.kcx * yg_line_stride(R1, N / R1) * sizeof(float2) + (size_t)N * sizeof(float2);
 }
 
-template <int H>
-__global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers g) {
+// LOAD: A read from where the forward k_xfft_g<H, true> saved it (its
+// pairs by the same threads and items), not recomputed from Y.
+template <int H, bool LOAD = false>
+__global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers g,
+                                                 const float2* amp = nullptr) {
   constexpr int R1 = 2 * H, YS = R1 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
@@ -818,8 +821,17 @@ __global__ void __launch_bounds__(kWG) k_adj_x_g(Params p, Buffers b, AdjBuffers
 #pragma unroll
   for (int u = 0; u < kXgItems; ++u) A[u][0] = A[u][1] = make_float2(0.0f, 0.0f);
   __syncthreads();                                          // twiddles
+  if constexpr (LOAD) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int u = 0; u < kXgItems; ++u)
+      if (u * (int)blockDim.x < nitems) {                   // the same for the whole workgroup
+        A[u][0] = amp[amp_g_index(blk, u, 0)];
+        A[u][1] = amp[amp_g_index(blk, u, 1)];
+      }
+  }
   // A = sum_e f_e F_e at the kept pairs (as k_xfft_g)
-  for (int e = 0; e < p.E; ++e) {
+  for (int e = 0; e < (LOAD ? 0 : p.E); ++e) {
     const float2* src = b.ybuf + ((((size_t)f * p.E + e) * p.ky + jy) * nx) * kz + k0;
     for (int item = threadIdx.x; item < kc * R2; item += blockDim.x) {
       const int j = div_small(item, kc, inv_kc), k = item - j * kc;

@@ -736,16 +736,17 @@ saxs::Params call_params(saxs_plan* pl, int n_atoms, int n_frames, const float* 
 
 // Store of the x pass's amplitudes at the kept outputs, which
 // saxs_forward_frames saves for the x adjoint of a backward that reuses its
-// spectra (k_xfft_p / k_adj_x_p, amp_index layout, or k_xfft_gf /
-// k_adj_x_gf, amp_gf_index layout): sized for the current kept ranges,
-// grown when they grow.
+// spectra (k_xfft_p / k_adj_x_p, amp_index layout; k_xfft_gf / k_adj_x_gf,
+// amp_gf_index; k_xfft_g / k_adj_x_g, amp_g_index): sized for the current
+// kept ranges, grown when they grow.
 int ensure_xamp(saxs_plan* pl) {
   const saxs::Params& p = pl->p;
   const int Nx = p.nn[0];
   const size_t bx = (size_t)(((p.kcx * (Nx / 16) + 63) / 64) * 64);
-  const size_t need = pl->x_pruned
-      ? (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXItems * (2 * (Nx / 128)) * bx
-      : (size_t)pl->max_frames * p.ky * p.nchx * 16 * saxs::kWG;   // k_xfft_gf: <= 16 a thread
+  const size_t need =
+      pl->x_pruned ? (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXItems * (2 * (Nx / 128)) * bx
+      : pl->x_full_h ? (size_t)pl->max_frames * p.ky * p.nchx * 16 * saxs::kWG   // k_xfft_gf: <= 16 a thread
+                     : (size_t)pl->max_frames * p.ky * p.nchx * saxs::kXgItems * 2 * saxs::kWG;
   if (need <= pl->xamp_elems) return SAXS_OK;
   if (pl->xamp) {
     HIP_TRY(hipFree(pl->xamp));
@@ -995,7 +996,11 @@ int launch_forward_core(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   } else if (pl->x_gen_h) {
     const dim3 gx(p.ky, p.nchx, n_frames);
     const size_t lq = saxs::xfft_g_lds_bytes(p, 2 * pl->x_gen_h);
-#define SAXS_XG(HH) hipLaunchKernelGGL(saxs::k_xfft_g<HH>, gx, dim3(saxs::kWG), lq, st, p, b)
+#define SAXS_XG(HH)                                                                    \
+  do {                                                                                 \
+    if (amp) hipLaunchKernelGGL((saxs::k_xfft_g<HH, true>), gx, dim3(saxs::kWG), lq, st, p, b, amp); \
+    else hipLaunchKernelGGL((saxs::k_xfft_g<HH>), gx, dim3(saxs::kWG), lq, st, p, b, nullptr); \
+  } while (0)
     switch (pl->x_gen_h) {
       case 10: SAXS_XG(10); break;
       case 9: SAXS_XG(9); break;
@@ -1463,7 +1468,8 @@ int saxs_forward_frames(saxs_plan* pl, const float* xyz, const int32_t* elem, in
   pl->last_frames_valid = false;
   pl->xamp_valid = false;
   // a plan with a backward keeps the x pass's amplitudes for its x adjoint
-  const bool save_amp = (pl->x_pruned || pl->x_full_h) && !pl->fwd_only && n_atoms > 0;
+  const bool save_amp =
+      (pl->x_pruned || pl->x_full_h || pl->x_gen_h) && !pl->fwd_only && n_atoms > 0;
   if (save_amp) {
     rc = ensure_xamp(pl);
     if (rc) return rc;
@@ -1527,7 +1533,7 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
     if (rc) return rc;
   }
   // the x adjoint reads the forward's saved amplitudes instead of recomputing them
-  const float2* amp = (reuse && pl->xamp_valid && (pl->x_pruned || pl->x_full_h)) ? pl->xamp : nullptr;
+  const float2* amp = (reuse && pl->xamp_valid) ? pl->xamp : nullptr;
   pl->last_frames_valid = false;           // the adjoint overwrites the spectra
   pl->xamp_valid = false;
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
@@ -1579,7 +1585,11 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
 #undef SAXS_AXF
     } else if (pl->x_gen_h) {
       const size_t lq = saxs::adj_x_g_lds_bytes(p, 2 * pl->x_gen_h);
-#define SAXS_AXG(HH) hipLaunchKernelGGL(saxs::k_adj_x_g<HH>, gx, dim3(saxs::kWG), lq, st, p, b, g)
+#define SAXS_AXG(HH)                                                                   \
+  do {                                                                                 \
+    if (amp) hipLaunchKernelGGL((saxs::k_adj_x_g<HH, true>), gx, dim3(saxs::kWG), lq, st, p, b, g, amp); \
+    else hipLaunchKernelGGL((saxs::k_adj_x_g<HH>), gx, dim3(saxs::kWG), lq, st, p, b, g, nullptr); \
+  } while (0)
       switch (pl->x_gen_h) {
         case 10: SAXS_AXG(10); break;
         case 9: SAXS_AXG(9); break;

@@ -2051,8 +2051,16 @@ __host__ __device__ inline size_t xfft_g_lds_bytes(const Params& p, int R1) {
          (size_t)p.kcx * yg_line_stride(R1, R2) * sizeof(float2) + (size_t)N * sizeof(float2);
 }
 
-template <int H>
-__global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
+// Entry (u, o) of thread t of generic x-pass workgroup blk in the saved
+// amplitudes: [blk][u][o][t] (k_xfft_g's pairs, o = 0 for m, 1 for N - m).
+__device__ __forceinline__ size_t amp_g_index(int blk, int u, int o) {
+  return (((size_t)blk * kXgItems + u) * 2 + o) * blockDim.x + threadIdx.x;
+}
+
+// SAVE: the accumulated pairs also go to `amp` (amp_g_index) for the x
+// adjoint of a backward that follows.
+template <int H, bool SAVE = false>
+__global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b, float2* amp = nullptr) {
   constexpr int R1 = 2 * H, YS = R1 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int jy = blockIdx.x, ch = blockIdx.y, f = blockIdx.z;
@@ -2129,6 +2137,15 @@ __global__ void __launch_bounds__(kWG) k_xfft_g(Params p, Buffers b) {
       }
     }
     __syncthreads();
+  }
+  if constexpr (SAVE) {
+    const int blk = (f * p.ky + jy) * p.nchx + ch;
+#pragma unroll
+    for (int u = 0; u < kXgItems; ++u)
+      if (u * (int)blockDim.x < nitems) {            // the same for the whole workgroup
+        amp[amp_g_index(blk, u, 0)] = A[u][0];
+        amp[amp_g_index(blk, u, 1)] = A[u][1];
+      }
   }
 #pragma unroll
   for (int u = 0; u < kXgItems; ++u) {

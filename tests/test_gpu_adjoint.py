@@ -165,14 +165,16 @@ def test_backward_reuse_matches_recompute(gpu):
 
 
 @pytest.mark.parametrize("L,n,qcut", [(64.0, 64, 0.6),     # 128-point pruned x axis (k_xfft_p)
-                                      (60.0, 80, 2.2)])    # 160 = 10 x 16, most kept (k_xfft_gf)
+                                      (60.0, 80, 2.2),     # 160 = 10 x 16, most kept (k_xfft_gf)
+                                      (52.5, 70, 0.6)])    # 140 = 14 x 10, few kept (k_xfft_g)
 def test_backward_reuse_saved_amplitudes(gpu, L, n, qcut):
     """The reusing backward's x adjoint reads the amplitudes forward_frames
     saved instead of recomputing them from the spectra (on a 128-point
-    pruned x axis, and on a 160-point axis whose x pass forms every stage-1
-    output): the gradient matches the recomputing backward's, over several
-    frames, and a second forward_frames/backward pair with other positions
-    uses the new amplitudes."""
+    pruned x axis, on a 160-point axis whose x pass forms every stage-1
+    output and on a 140-point axis with Horner sums): the gradient matches
+    the recomputing backward's, over several frames, and a second
+    forward_frames/backward pair with other positions uses the new
+    amplitudes."""
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
     box = np.diag([L] * 3)
@@ -193,7 +195,9 @@ def test_backward_reuse_saved_amplitudes(gpu, L, n, qcut):
         torch.cuda.synchronize()
         err = ((g_ru - g_re).abs().max() / g_re.abs().max()).item()
         print(f"[saved amplitudes, seed {seed}] max diff / max |g| = {err:.3e}", flush=True)
-        assert torch.allclose(g_ru, g_re, rtol=1e-6, atol=1e-7 * g_re.abs().max().item())
+        # the same amplitudes (power-of-two and full-output x passes: bit-identical),
+        # or the forward's Horner sums against the adjoint's direct sums (k_xfft_g)
+        assert torch.allclose(g_ru, g_re, rtol=1e-5, atol=1e-6 * g_re.abs().max().item())
     eng.close()
 
 
