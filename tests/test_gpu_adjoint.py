@@ -201,6 +201,37 @@ def test_backward_reuse_saved_amplitudes(gpu, L, n, qcut):
     eng.close()
 
 
+def test_backward_reuse_after_box_growth(gpu):
+    """A box that grows 10 % between forward_frames/backward pairs keeps more
+    frequencies (the plan's amplitude buffer grows with them): each reusing
+    backward matches the recomputing one on the new box."""
+    import torch
+    from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
+    L, n = 64.0, 64
+    base, syms = O.synthetic_system(10000, L, seed=33)
+    symbols, ids = O.element_ids(syms)
+    n3, nn = resolve_grid((n,) * 3, 2.0)
+    eng = SaxsEngine(SystemSpec(n3, nn, 4, tuple(symbols), 2.0, 0.0, 0.6), np.diag([L] * 3),
+                     10000, 2)
+    idt = torch.from_numpy(ids).cuda()
+    kept = []
+    for scale in (1.0, 1.1):
+        box = np.diag([L * scale] * 3)
+        fr = np.stack([O.frame_positions(base * scale, 33, f, 0.3) for f in range(2)])
+        xyz = torch.from_numpy(fr.astype(np.float32)).cuda()
+        Hf, _ = eng.forward_frames(xyz, idt, box=box)
+        kept.append(eng.kept())
+        gH = torch.ones_like(Hf) / Hf.abs().clamp_min(1.0)
+        g_ru = eng.backward(xyz, idt, gH, generation=eng.generation)   # the plan is on `box`
+        g_re = eng.backward(xyz, idt, gH)
+        torch.cuda.synchronize()
+        err = ((g_ru - g_re).abs().max() / g_re.abs().max()).item()
+        print(f"[box x {scale}] kept {eng.kept()}: reuse vs recompute {err:.3e}", flush=True)
+        assert torch.allclose(g_ru, g_re, rtol=1e-5, atol=1e-6 * g_re.abs().max().item())
+    assert kept[1][1] > kept[0][1], kept
+    eng.close()
+
+
 def _engine_grad(n, L, pos, syms, qcut, sigma, flags, tile=(4, 4), order=4):
     import torch
     from pycusaxs_amd.engine import SaxsEngine, SystemSpec, resolve_grid
