@@ -778,6 +778,12 @@ constexpr int kAllRecs = kPrefetch * kWG;          // staged records per round
 // records at 0.05 atoms per cell; denser tiles take more rounds).
 constexpr int kZpRecs = 160;
 constexpr int kZpPre = (kZpRecs + kWG - 1) / kWG;  // records prefetched per thread
+// k_spread_zp at N = 512 (a 256-point z axis): its 512-entry twiddle table
+// and 17 x 32 line entries per column leave room for four workgroups a CU
+// only with 60 staged records, and its registers are capped to match (four
+// waves a SIMD); the extra rounds cost less than the occupancy gains.
+__host__ __device__ constexpr int zp_spread_recs(int N) { return N == 512 ? 60 : kZpRecs; }
+__host__ __device__ constexpr int zp_spread_waves(int N) { return N == 512 ? 4 : 1; }
 
 __host__ __device__ inline size_t spread_all_lds_bytes(const Params& p) {
   const int cs = p.order == 6 ? spread_col_stride<6>(p.n[2]) : spread_col_stride<4>(p.n[2]);
@@ -942,7 +948,7 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
   const int N = p.nn[2];
   const int cs = zp_col_stride(p.n[2], p.order, N);
   const size_t A = (size_t)p.tx * p.ty * cs * sizeof(unsigned long long);
-  const size_t B = (size_t)kZpRecs * sizeof(float4);
+  const size_t B = (size_t)zp_spread_recs(N) * sizeof(float4);
   const size_t tw = (size_t)N * sizeof(float2);          // W_N^{j s} as [j][s]
   const size_t seg = (size_t)p.E * (28 * sizeof(int) + sizeof(unsigned long long));
   return A + B + tw + seg;
@@ -951,7 +957,8 @@ __host__ __device__ inline size_t spread_p_lds_bytes(const Params& p) {
 // EXACT: n_z == N/2 (every sigma-2 power-of-two grid), so every stage-0
 // input z = j + R2 r, r < 8, lies in the column and needs no bounds test.
 template <int P, int N, bool HALF, bool EXACT = false>
-__global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
+__global__ void __launch_bounds__(kWG, zp_spread_waves(N)) k_spread_zp(Params p, Buffers b) {
+  constexpr int RECS = zp_spread_recs(N);
   static_assert(!EXACT || HALF, "EXACT implies HALF");
   constexpr int R2 = N / 16;
   constexpr int RIN = HALF ? 8 : 16;                   // stage-0 inputs that can be nonzero
@@ -968,7 +975,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   const int cs = zp_col_stride(nz, P, N);
   const int LY = 2 * cs;                               // line stride (float2)
   const size_t A_b = (size_t)ncol * cs * sizeof(unsigned long long);
-  const size_t B_b = (size_t)kZpRecs * sizeof(float4);
+  const size_t B_b = (size_t)RECS * sizeof(float4);
   unsigned long long* rho = reinterpret_cast<unsigned long long*>(smem);
   float2* ybuf = reinterpret_cast<float2*>(smem);
   float4* recs = reinterpret_cast<float4*>(smem + A_b);
@@ -1008,7 +1015,7 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
     return b.sorted[s_start[e * 9 + sgm] + (a - pref[sgm])];
   };
   {
-    const int tot0 = min(s_pref[9], kZpRecs);
+    const int tot0 = min(s_pref[9], RECS);
 #pragma unroll
     for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
@@ -1020,15 +1027,15 @@ __global__ void __launch_bounds__(kWG) k_spread_zp(Params p, Buffers b) {
   for (int e = 0; e < E; ++e) {
     const int total = s_pref[e * 10 + 9];
     __syncthreads();                                 // records staged, columns clear
-    spread_cols<P, kSpreadLanes>(recs, min(total, kZpRecs), rho, p, X0, Y0, wx, wy, cs);
-    for (int base = kZpRecs; base < total; base += kZpRecs) {
-      const int chunk = min(kZpRecs, total - base);
+    spread_cols<P, kSpreadLanes>(recs, min(total, RECS), rho, p, X0, Y0, wx, wy, cs);
+    for (int base = RECS; base < total; base += RECS) {
+      const int chunk = min(RECS, total - base);
       __syncthreads();
       for (int i = threadIdx.x; i < chunk; i += blockDim.x) recs[i] = fetch(e, base + i);
       __syncthreads();
       spread_cols<P, kSpreadLanes>(recs, chunk, rho, p, X0, Y0, wx, wy, cs);
     }
-    const int tot_next = min((e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0, kZpRecs);
+    const int tot_next = min((e + 1 < E) ? s_pref[(e + 1) * 10 + 9] : 0, RECS);
 #pragma unroll
     for (int u = 0; u < kZpPre; ++u) {
       const int a = threadIdx.x + u * blockDim.x;
