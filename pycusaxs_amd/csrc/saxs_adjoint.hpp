@@ -27,6 +27,7 @@ struct AdjBuffers {
   const double* gH;     // [F][num_bins]
   float* grad;          // [F][N][3]
   double* padg;         // [F][E] sum over the cell of drho~
+  double* padt;         // [F][E][ntx nty] the same sum over each column tile
   float* gpart;         // [F][nsx*nsy][N][3] per-tile partial gradients, entry
                         // (f, slot, r): record r of frame f's sorted run
   int nsx, nsy;         // tile offsets an atom's stencil reaches along x, y
@@ -63,10 +64,19 @@ __device__ __forceinline__ int kept_to_index(int j, int k, int K, int N) {
   return (k == N) ? j : (j <= K ? j : N - (k - j));
 }
 
-__global__ void k_fill_zero(float* a, size_t n, double* d, int nd) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0.0f;
-  if (i < (size_t)nd) d[i] = 0.0;
+// padg[fe] = sum over the column tiles of padt[fe][tile], in a fixed order
+// (each lane sums tiles lane, lane + 64, ..., then a fixed shuffle tree), so
+// the pad term of the gradient does not depend on the order in which the
+// tiles ran.  One wavefront per (frame, element).
+__global__ void __launch_bounds__(kWG) k_pad_reduce(const double* padt, double* padg, int ntiles,
+                                                    int nfe) {
+  const int fe = blockIdx.x * (kWG / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (fe >= nfe) return;
+  const double* src = padt + (size_t)fe * ntiles;
+  double s = 0.0;
+  for (int t = lane; t < ntiles; t += 64) s += src[t];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) padg[fe] = s;
 }
 
 // x-pass adjoint: per (kept m_y, kz chunk, frame).  Recomputes A from Y,
@@ -579,14 +589,14 @@ __global__ void __launch_bounds__(kWG) k_adj_z(Params p, Buffers b, AdjBuffers g
     drho[i] = val;
     part += (double)val;
   }
-  // block sum of drho~ over this tile -> padg[f][e]
+  // block sum of drho~ over this tile -> padt[f][e][tile]
   for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_red[w];
-    atomicAdd(&g.padg[fe], t);
+    g.padt[fe * (p.ntx * p.nty) + tile] = t;
   }
 
   // gather: every atom whose stencil reaches this tile accumulates
@@ -1507,7 +1517,7 @@ __global__ void __launch_bounds__(kWG) k_adj_zg(Params p, Buffers b, AdjBuffers 
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
-      atomicAdd(&g.padg[fe], sum);
+      g.padt[fe * (p.ntx * p.nty) + tile] = sum;
     }
     gather_cols<P>(recs, staged, drho, p, X0, Y0, wx, wy, cs, g, f, bx, by);
     for (int base = kZpRecs; base < total; base += kZpRecs) {
@@ -1743,7 +1753,7 @@ __global__ void __launch_bounds__(kWG, 3) k_adj_zp(Params p, Buffers b, AdjBuffe
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_red[w];
-      atomicAdd(&g.padg[fe], sum);
+      g.padt[fe * (p.ntx * p.nty) + tile] = sum;
     }
     gather_cols<P>(recs, staged, drho, p, X0, Y0, wx, wy, LY, g, f, bx, by);
     for (int base = kZpRecs; base < total; base += kZpRecs) {

@@ -400,6 +400,7 @@ struct saxs_plan {
   saxs::Params pp{};           // plain-path geometry (the whole FFT grid as "cell")
   int plain_pairs = 0;
   double* padg = nullptr;      // [F*E] adjoint pad terms
+  double* padt = nullptr;      // [F*E*ntx*nty] adjoint pad terms per column tile
   float* gpart = nullptr;      // [F][nsx*nsy][N][3] adjoint partials (first backward)
   int* segoff = nullptr;       // [(512 + max_frames) * keys per frame] segmented sort
   bool seg_sort = false;       // one frame's keys fit the LDS histogram of k_prep_seg
@@ -593,7 +594,7 @@ void free_plan(saxs_plan* pl) {
   void* ptrs[] = {pl->srcx, pl->srcy, pl->twz, pl->twy, pl->twx, pl->modx, pl->mody,
                   pl->modz, pl->lut_bin, pl->lut_ff, pl->counts, pl->offsets, pl->keyslot,
                   pl->sorted, pl->pad_sum, pl->zbuf, pl->ybuf, pl->partial, pl->hframe,
-                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->yspec, pl->padp,
+                  pl->grid, pl->sup, pl->spec, pl->acc, pl->padg, pl->padt, pl->yspec, pl->padp,
                   pl->gpart, pl->segoff, pl->pad_given, pl->xamp};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1224,6 +1225,7 @@ int saxs_plan_create(const saxs_desc* d, saxs_plan** out) {
   ALLOC(pl->pad_sum, F * E);
   ALLOC(pl->pad_given, E);
   ALLOC(pl->padg, F * E);
+  ALLOC(pl->padt, F * E * p.ntx * p.nty);
   ALLOC(pl->zbuf, F * E * p.n[0] * p.n[1] * pl->kz_cap);
   ALLOC(pl->ybuf, F * E * pl->ky_cap * p.n[0] * pl->kz_cap);
   ALLOC(pl->partial, F * pl->ky_cap * pl->nchx_cap * 2 * p.num_bins);
@@ -1538,10 +1540,7 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   pl->xamp_valid = false;
   saxs::Params p = call_params(pl, n_atoms, n_frames, oc_diag);
   saxs::Buffers b = make_buffers(pl, xyz, elem);
-  saxs::AdjBuffers g{gH, grad, pl->padg, pl->gpart, pl->nsx, pl->nsy};
-  const int npadg = n_frames * p.E;
-  hipLaunchKernelGGL(saxs::k_fill_zero, dim3((unsigned)((npadg + 255) / 256)), dim3(256), 0, st,
-                     grad, (size_t)0, pl->padg, npadg);
+  saxs::AdjBuffers g{gH, grad, pl->padg, pl->padt, pl->gpart, pl->nsx, pl->nsy};
   const int px = saxs::fft_radix_set(p.fx), py = saxs::fft_radix_set(p.fy),
             pz = saxs::fft_radix_set(p.fz);
   {
@@ -1722,6 +1721,10 @@ int saxs_backward_ex(saxs_plan* pl, const float* xyz, const int32_t* elem, int32
   }
   if (n_atoms > 0) {
     Probe pr(pl, SAXS_K_GRAD_SUM, st);
+    const int nfe = n_frames * p.E;
+    constexpr int wpb = saxs::kWG / 64;    // k_pad_reduce: one wavefront per (frame, element)
+    hipLaunchKernelGGL(saxs::k_pad_reduce, dim3((unsigned)((nfe + wpb - 1) / wpb)), dim3(saxs::kWG), 0, st,
+                       pl->padt, pl->padg, p.ntx * p.nty, nfe);
     dim3 ga((n_atoms + saxs::kWG - 1) / saxs::kWG, n_frames);
     if (p.order == 4)
       hipLaunchKernelGGL(saxs::k_grad_finalize<4>, ga, dim3(saxs::kWG), 0, st, p, b, g);
