@@ -232,10 +232,14 @@ int saxs_forward_boxes(saxs_plan* plan, const float* xyz, const int32_t* elem, i
 /* Per-frame histograms instead of an accumulated one: Hf, Cf are device
  * float64 [n_frames][num_bins] and are overwritten.  Used by the adjoint
  * (per-frame binned intensities).  On a plan with a backward (no
- * SAXS_FLAG_FORWARD_ONLY) whose x pass is the pruned power-of-two one, it
- * also keeps that pass's amplitudes at the kept outputs for a following
+ * SAXS_FLAG_FORWARD_ONLY) whose x pass is one of the output-pruned ones
+ * (power-of-two, generic, or generic with most outputs kept), it also keeps
+ * that pass's amplitudes at the kept outputs for a following
  * SAXS_BWD_REUSE_FORWARD backward (a buffer of the plan, allocated on first
- * use: at most max_frames x ky x nchx x 2 x (nn_x / 64) x 512 x 8 bytes). */
+ * use and grown when a new box keeps more frequencies: at most
+ * max_frames x ky x nchx x E bytes with E = 16 nn_x x 8 for the
+ * power-of-two pass, 4096 x 8 for the generic pass with most outputs kept
+ * and 2048 x 8 for the other generic one). */
 int saxs_forward_frames(saxs_plan* plan, const float* xyz, const int32_t* elem,
                         int32_t n_atoms, int32_t n_frames, const float* oc_diag,
                         double* Hf, double* Cf, void* stream);
@@ -243,8 +247,9 @@ int saxs_forward_frames(saxs_plan* plan, const float* xyz, const int32_t* elem,
 /* Adjoint of saxs_forward_frames: given gH = dL/dH per frame
  * (device float64 [n_frames][num_bins]), writes dL/dx per atom into grad
  * (device float32 [n_frames][n_atoms][3]; overwritten).  Recomputes the
- * forward spectra internally (no stored state between calls).  No reference
- * counterpart: pyCuSaxs has no gradient. */
+ * forward spectra internally (no stored state between calls).  Repeated
+ * calls on the same inputs return the same bits (no floating-point atomics).
+ * No reference counterpart: pyCuSaxs has no gradient. */
 int saxs_backward(saxs_plan* plan, const float* xyz, const int32_t* elem, int32_t n_atoms,
                   int32_t n_frames, const float* oc_diag, const double* gH, float* grad,
                   void* stream);
